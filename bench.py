@@ -1,0 +1,158 @@
+"""Benchmark of the delta-SPH dam-break step on one MI355X.
+
+Workload: the 3-D dam break of 2.0M fluid particles -- a 1.0 x 0.5 x 0.5 m
+column in a 3.2 x 1.0 x 1.0 m tank, dx = 0.005 m, 3 layers of dummy wall
+particles (1.41M), Wendland C2 with h = 1.3 dx, Tait EOS, delta = 0.1,
+alpha = 0.02, CFL 0.2.  A step is one full predictor-corrector step
+(2 x hash, sort, cell ranges, gather, kernel A, kernel B, integrator)
+replayed from a captured hipGraph with the state resident in HBM.
+
+Prints one JSON line: particle-steps/s (fluid particles x steps / s), the
+roofline of the dominant kernel measured with HIP events on its stream, and a
+CPU baseline (the oracle's FP32 step on the host cores, a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP32_VALU_PEAK_TFLOPS = 157.3
+
+
+def _configs():
+    from tmp_sph_amd import sph
+    return {
+        "c3": ("3-D dam break, 1.0x0.5x0.5 m column in 3.2x1.0x1.0 m tank, dx=0.005 m",
+               sph.dambreak_cfg(dim=3, dx=0.005, tank=(3.2, 1.0, 1.0), column=(1.0, 0.5, 0.5))),
+        "c2": ("2-D dam break, 1.0x0.5 m column in 4.0x1.0 m tank, dx=0.001 m",
+               sph.dambreak_cfg(dim=2, dx=0.001)),
+        "c1": ("2-D dam break, 1.0x0.5 m column in 4.0x1.0 m tank, dx=0.01 m",
+               sph.dambreak_cfg(dim=2, dx=0.01)),
+    }
+
+
+def _algorithmic_bytes(name: str, dim: int, n: int, nf: int) -> float:
+    """Compulsory HBM bytes of one launch of a kernel (each array touched once).
+
+    nsB: PR (16 B) of every target; VP (16 B), dofv (4 B) and key (4 B) of
+    fluid targets; RATE (16 B) written for every target.  nsA: PR, VP, key read
+    (36 B) and dofv written (4 B) for every target, wall p and rho (8 B).
+    """
+    nw = n - nf
+    if name == "nsB":
+        return 16.0 * n + 24.0 * nf + 16.0 * n
+    if name == "nsA":
+        return 40.0 * n + 8.0 * nw
+    raise KeyError(name)
+
+
+def cpu_baseline(case, seconds_target: float) -> dict:
+    from oracle import oracle as orc
+    threads = orc.num_threads(0)
+    # one step first, then as many as fit in the time target (at least 1 more)
+    _, _, _, s1 = orc.run(case.params, case, 1, mode=0)
+    k = max(1, min(20, int(seconds_target / max(s1, 1e-9))))
+    _, _, _, sk = orc.run(case.params, case, k, mode=0)
+    return {"value": case.n_fluid * k / sk, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 reference step (serial C++ + OpenMP), {k} full steps of the same case "
+                      f"from rest, {case.n} particles ({case.n_fluid} fluid), {sk:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--timing-steps", type=int, default=10, help="eager steps with per-kernel HIP events")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nbr-block", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
+              "the slab decomposition it would use is described in DESIGN.md", file=sys.stderr)
+        return 2
+
+    from tmp_sph_amd import sph
+    desc, cfg = _configs()[args.config]
+    t0 = time.time()
+    case = sph.make_case(cfg)
+    prm = case.params
+    print(f"# case {args.config}: N={case.n} fluid={case.n_fluid} dt={prm.timeStep:.4g} "
+          f"(generated in {time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+    sim = sph.Simulation(case)
+    if args.nbr_block:
+        sim.set_tuning(nbr_block=args.nbr_block)
+    sim.capture()
+    sim.replay(args.warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.replay(args.steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = case.n_fluid * args.steps / elapsed
+    print(f"# timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # per-kernel device time (HIP events on the simulation stream, eager steps)
+    sim.set_tuning(flags=1)
+    sim.step(args.timing_steps)
+    sim.synchronize()
+    kt = sim.kernel_times()
+    stats = sim.pair_stats()
+    front, height = sim.probe(band=0.02)
+    sim.close()
+    dom = max(("nsA", "nsB"), key=lambda k: kt.get(k, 0.0))
+    us = kt[dom]
+    byts = _algorithmic_bytes(dom, prm.dim, case.n, case.n_fluid)
+    achieved = byts / (us * 1e-6) / 1e9
+    # counted flops: 9 per candidate + 55 per in-support pair (kernel B), 9 + 25 (kernel A)
+    flops = (9.0 * stats["candB"] + 55.0 * stats["hitsB"]) if dom == "nsB" else \
+        (9.0 * stats["candA"] + 25.0 * stats["hitsA"])
+    valu = flops / (us * 1e-6) / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{dom}_{args.config}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    out = {
+        "metric": "particle-steps/s @2M (plain) + learned vs plain speedup at <=2% front error",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 positions)",
+        "data": "synthetic dam-break lattice at rest, hydrostatic pressure",
+        "config": {"workload": desc, "particles_total": case.n, "particles_fluid": case.n_fluid,
+                   "dt": prm.timeStep, "kernel_h_over_dx": 1.3, "eos": "tait7" if prm.eos == 1 else "linear",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": us,
+                     "algorithmic_bytes_per_launch": byts,
+                     "valu": {"achieved_tflops": valu, "peak_tflops": FP32_VALU_PEAK_TFLOPS,
+                              "frac": valu / FP32_VALU_PEAK_TFLOPS, "flops_per_launch": flops}},
+        "kernels_us": kt,
+        "pair_stats": stats,
+        "probe_after_run": {"front_m": front, "column_height_m": height},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,242 @@
+/*
+ * sph_capi.h -- C ABI of the MI355X (gfx950) delta-SPH dam-break step.
+ *
+ * Two layers are exported:
+ *
+ *  1. The reference call surface, one entry point per reference host wrapper,
+ *     with the same argument roles, the same device-pointer ownership (the
+ *     caller allocates every array) and the same array layouts (AoS float[3N]
+ *     for positions, velocities and accelerations, int[N] for types).  The
+ *     process-wide parameter block set by sph_set_params() replaces the
+ *     reference's `__constant__ SimParams par`.  A caller that drove the
+ *     reference step (opt_src_v0/sph/sph_run.cu:269-309) can call these in the
+ *     same order.
+ *
+ *  2. A simulation context (sph_sim_*) that owns sorted SoA state with FP64
+ *     positions and runs whole predictor-corrector steps on one HIP stream,
+ *     optionally replayed from a captured hipGraph.  This is the fast path.
+ *
+ * Every function returns an int status: SPH_OK (0) on success, a negative
+ * SPH_E* code on a usage error, or a positive hipError_t value when the HIP
+ * runtime failed.  sph_last_error() returns a message for the last failure
+ * on the calling thread.  Nothing here prints or pauses.
+ *
+ * 2-D convention: the plane is (x, z); gravity acts along z; y is 0 for every
+ * particle and the grid has one cell along y (gridydim == 1).
+ *
+ * Particle types follow the reference (src/sph/sph_run.cu:36):
+ * 0 wall (dummy particle), 1 fluid, anything else is treated as non-fluid.
+ */
+#ifndef SPH_CAPI_H
+#define SPH_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_OK 0
+#define SPH_E_INVAL (-1)     /* bad argument (null pointer, size out of range) */
+#define SPH_E_NOPARAMS (-2)  /* sph_set_params was never called */
+#define SPH_E_STATE (-3)     /* call sequence error (e.g. graph replay before capture) */
+#define SPH_E_NODEV (-4)     /* no HIP device visible */
+
+#define SPH_EOS_LINEAR 0     /* p = cs^2 (rho - rho0)   (src/sph/sph_run.cu:467) */
+#define SPH_EOS_TAIT 1       /* p = rho0 cs^2 / gamma ((rho/rho0)^gamma - 1) */
+
+/*
+ * Parameter block.  Replaces SimParams (src/header/data_struct.cuh:4-44) and
+ * its copy to `__constant__ par` (src/sph/sph_run.cu:105).  Only fields the
+ * step reads are kept; derived constants are filled by the builders below.
+ */
+typedef struct sph_params {
+    /* escape box: a fluid particle outside [min,max] on any axis is parked
+       (src/sph/sph_run.cu:437-447; the reference hard-codes 0..3.23, 0..1, 0..1) */
+    double domain_min[3];
+    double domain_max[3];
+
+    int dim;             /* 2 or 3 */
+    int numParticles;    /* N, fluid + wall */
+    int numFluid;        /* informational */
+    int gridxdim, gridydim, gridzdim;
+    int numCells;        /* gridxdim * gridydim * gridzdim */
+    int hash_max;        /* reference value X*Y*Z + X*Y + X (src/sph/sph_run.cu:92); unused by the kernels */
+    int eos;             /* SPH_EOS_LINEAR or SPH_EOS_TAIT */
+    int park_type;       /* type written to a parked particle (1 keeps it fluid as in src/v0/v1/v3) */
+
+    float gridxmin, gridymin, gridzmin;
+    float gridSize;      /* cell edge */
+    float particleR;     /* lattice spacing dx */
+    float h, kh, eta, delta, afa;
+    float adh;           /* Wendland C2 normalisation: 21/(16 pi h^3) in 3-D, 7/(4 pi h^2) in 2-D */
+    float particleMass;  /* dx^dim * rho0 */
+    float restDensity;   /* rho0 */
+    float cs;            /* speed of sound c0 */
+    float gravity;       /* along z, negative */
+    float timeStep;      /* full step dt; each half-step advances dt/2 */
+    float gamma;         /* Tait exponent */
+    float tait_B;        /* rho0 cs^2 / gamma */
+    float damper;        /* 1: acoustic damper dofv term on (ns.cu:212), 0: off */
+    float rr_cut;        /* largest float rr with sqrtf(rr)/h <= 2 in FP32 (same pair set as the reference test q <= 2) */
+    float park_pos[3];   /* where an escaped particle is parked (src: (0, 0.5, -0.08)) */
+    float pad_;
+} sph_params;
+
+/* Dam-break case description (no reference counterpart: the reference reads
+   its initial state from .dat files that are not shipped, src/io/io.cu:6-111). */
+typedef struct sph_dambreak_cfg {
+    int dim;             /* 2 or 3 */
+    int eos;             /* SPH_EOS_LINEAR / SPH_EOS_TAIT */
+    int wall_layers;     /* dummy wall layers (3) */
+    int pad_;
+    double dx;           /* lattice spacing */
+    double tank[3];      /* tank extent x, y, z (y ignored in 2-D) */
+    double column[3];    /* water column extent x, y, z at the x=0, y=0 corner */
+    double h_over_dx;    /* 1.3 */
+    double rho0;         /* 1000 */
+    double g;            /* -9.81 */
+    double c0;           /* <= 0: 10 sqrt(2 |g| H) with H the column height */
+    double cfl;          /* dt = cfl * h / c0 */
+    double delta, alpha; /* delta-SPH coefficient, artificial viscosity */
+    double gamma;        /* Tait exponent (7) */
+    double grid_pad;     /* padding of the cell grid around the tank (0.1 m in the reference) */
+    double cell_over_kh; /* cell edge / (2h); 1 reproduces the reference grid */
+} sph_dambreak_cfg;
+
+/* ---------------------------------------------------------------- runtime */
+/* sizeof of the ABI structs as compiled into the library (binding check). */
+int sph_abi_sizes(int* params_size, int* cfg_size, int* tuning_size);
+const char* sph_last_error(void);
+int sph_device_count(int* count);
+int sph_set_device(int device);
+int sph_set_stream(void* hip_stream);          /* stream used by the reference-surface calls (default: null stream) */
+int sph_device_synchronize(void);
+int sph_malloc(void** dptr, unsigned long long bytes);
+int sph_free(void* dptr);
+int sph_memcpy_htod(void* dst, const void* src, unsigned long long bytes);
+int sph_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes);
+int sph_memcpy_dtod(void* dst, const void* src, unsigned long long bytes);
+int sph_memset(void* dptr, int value, unsigned long long bytes);
+
+/* ------------------------------------------------------------- parameters */
+/* Replaces cudaMemcpyToSymbol(par, temps, sizeof(SimParams)) (src/sph/sph_run.cu:105). */
+int sph_set_params(const sph_params* p);
+int sph_get_params(sph_params* p);
+/* The reference's host recipe (src/sph/sph_run.cu:11-22, 60-101): Kleefsman tank
+   3.221 x 1 x 1, cs = 25, dt = 2e-4, h = 1.5 dx, delta = 0.2, afa = 0.05,
+   linear EOS, 0.1 m grid padding; float/double evaluation order kept so the
+   derived floats are the reference's.  dx = 0.01 in the reference. */
+int sph_params_reference(sph_params* p, float dx, int numParticles, int numFluid);
+/* Parameters of a dam-break case (no reference counterpart). */
+int sph_params_dambreak(const sph_dambreak_cfg* cfg, sph_params* p);
+/* Counts of a dam-break case without generating it. */
+int sph_case_dambreak_count(const sph_dambreak_cfg* cfg, int* n_total, int* n_fluid);
+/* Generates the lattice case into caller-owned host arrays of capacity cap:
+   pos/vel AoS float[3N], rho/p float[N], type int[N].  Walls come first,
+   then fluid.  Fluid at rest with hydrostatic pressure; walls at p = 0,
+   rho = rho0.  Fills *p_out and the counts. */
+int sph_case_dambreak(const sph_dambreak_cfg* cfg, int cap, float* pos, float* vel, float* rho,
+                      float* p, int* type, sph_params* p_out, int* n_total, int* n_fluid);
+
+/* ------------------------------------------- reference call surface (device pointers) */
+/* calcHash (src/header/mesh.cuh:20, src/lib/mesh.cu:56-70): hash = z*X*Y + y*X + x of
+   floorf((p - gridmin)/gridSize) per axis, idx[i] = i.  Cell coordinates are clamped
+   into the grid (the reference reads out of bounds there instead). */
+int sph_calcHash(const float* pos, int* particleHash, int* particleIndex, int numParticles);
+/* sortParticles (src/header/mesh.cuh:23, src/lib/mesh.cu:73-86): stable ascending sort of
+   the keys carrying the indices, in place.  Keys must lie in [0, numCells). */
+int sph_sortParticles(int* particleHash, int* particleIndex, int numParticles);
+/* reorder (opt_src_v0/header/mesh.cuh:38, opt_src_v0/lib/mesh.cu:106-177): cellStart is
+   set to -1 for every cell, then cellStart/cellEnd mark the sorted range of every
+   non-empty cell (cellEnd of empty cells is left untouched, as in the reference), and
+   the particle arrays are gathered into sorted order.  table_l[i] = table_r[idx[i]] +
+   idx[i] - i tracks original ids.  oldZone/sortedZone/table_* may be null. */
+int sph_reorder(const float* oldPos, const float* oldVel, const float* oldDensity,
+                const float* oldPressure, const int* oldType, const int* oldZone,
+                float* sortedPos, float* sortedVel, float* sortedDensity, float* sortedPressure,
+                int* sortedType, int* sortedZone, const int* particleHash,
+                const int* particleIndex, int* cellStart, int* cellEnd, int numParticles,
+                int* table_l, const int* table_r);
+/* SPH_NS_simpleversion (src/header/ns.cuh:4, opt_src_v1/equation/ns.cu:5-230): kernel A
+   (wall pressure extrapolation, velocity divergence dofv) then kernel B (drho/dt with
+   delta-SPH diffusion, dv/dt with pressure, artificial viscosity, acoustic damper and
+   gravity).  Wall pressure/density are written in place.  rhop_sum and w_sum are
+   accepted and unused, as in the reference from v0 on. */
+int sph_SPH_NS_simpleversion(const float* sortedPos, const float* sortedVel,
+                             float* sortedDensity, float* sortedPressure, const int* sortedType,
+                             float* densitydt, float* Veldt, const int* cellStart,
+                             const int* cellEnd, int numParticles, const int* particleHash,
+                             int timestep, float* dofv, float* rhop_sum, float* w_sum);
+/* PC_prediction (src/header/sph.cuh:26; opt_src_v0/sph/sph_run.cu:388-435): fluid only;
+   saves (x, v, rho) into the tmp arrays at the sorted slot and advances half a step. */
+int sph_PC_prediction(float* sortedPos, float* sortedVel, float* sortedDensity,
+                      float* sortedPressure, const float* densitydt, const float* Veldt,
+                      const int* particleHash, const int* particleIndex, float* Pos_tmp,
+                      float* Vel_tmp, float* density_tmp, int numParticles,
+                      int* sortedType);
+/* PC_correction (src/header/sph.cuh:28; opt_src_v0/sph/sph_run.cu:437-484): fluid only;
+   reads the tmp arrays through particleIndex (the half-step-2 sort permutation). */
+int sph_PC_correction(float* sortedPos, float* sortedVel, float* sortedDensity,
+                      float* sortedPressure, const float* densitydt, const float* Veldt,
+                      const int* particleHash, const int* particleIndex, const float* Pos_tmp,
+                      const float* Vel_tmp, const float* density_tmp, int numParticles,
+                      int* sortedType);
+
+/* ------------------------------------------------ simulation context (fast path) */
+typedef struct sph_sim sph_sim;
+
+/* Launch-shape knobs of the fast path, filled with defaults by sph_sim_create. */
+typedef struct sph_tuning {
+    int nbr_block;       /* threads per block of the neighbour-loop kernels */
+    int nbr_variant;     /* neighbour-loop kernel variant (0 = per-target rows) */
+    int sort_ipt;        /* reserved */
+    int flags;           /* reserved */
+} sph_tuning;
+
+/* Creates a context on the current device from host arrays in the layout of
+   sph_case_dambreak (AoS float[3N] pos/vel).  Positions are widened to FP64. */
+int sph_sim_create(const sph_params* p, int n, const float* pos, const float* vel,
+                   const float* rho, const float* prs, const int* type, sph_sim** out);
+int sph_sim_destroy(sph_sim* s);
+int sph_sim_get_tuning(sph_sim* s, sph_tuning* t);
+int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t);
+/* Runs nsteps full predictor-corrector steps (eager launches). */
+int sph_sim_step(sph_sim* s, int nsteps);
+/* Captures one full step into a hipGraph (replaces any previous capture). */
+int sph_sim_capture(sph_sim* s);
+/* Replays the captured step nsteps times. */
+int sph_sim_replay(sph_sim* s, int nsteps);
+int sph_sim_synchronize(sph_sim* s);
+/* Copies the state back in ORIGINAL particle order: pos/vel AoS float[3N]
+   (positions rounded from FP64), rho/p float[N], type int[N].  Any may be null. */
+int sph_sim_download(sph_sim* s, float* pos, float* vel, float* rho, float* p, int* type);
+/* FP64 positions in original order, AoS double[3N]. */
+int sph_sim_download_pos64(sph_sim* s, double* pos);
+/* Surge front (max x over in-domain fluid) and column height (max z over fluid
+   with x <= band) computed on the device; values are written to host doubles. */
+int sph_sim_probe(sph_sim* s, double band, double* front, double* height);
+/* Average device time of each kernel launched by the last timed sph_sim_step /
+   sph_sim_replay when timing is enabled (flags bit 0 of sph_tuning); names are
+   written as a ';'-separated list.  Microseconds per launch. */
+int sph_sim_kernel_times(sph_sim* s, char* names, int names_cap, double* us, int* count,
+                         int cap);
+/* Number of steps completed by this context. */
+int sph_sim_steps_done(sph_sim* s, long long* steps);
+/* Evaluates one half-step's neighbour phase on the current state without
+   integrating: hash, sort, cell ranges, gather, kernel A, kernel B (the
+   reference's calcHash..SPH_NS_simpleversion sequence).  Host outputs, any may
+   be null: drho[N], acc AoS[3N], dofv[N], p/rho[N] after the wall update
+   (original particle order); keys/perm[N] in sorted order, where perm holds
+   the slot of the current state each sorted entry came from.  Rates of
+   non-fluid particles are written as 0 (the integrator never reads them). */
+int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, float* p, float* rho, int* keys,
+                 int* perm);
+/* Neighbour-pair statistics of the current state (for roofline accounting and
+   tuning): candidate pairs visited and pairs inside the support, summed over
+   all targets (kernel A) and over fluid targets (kernel B). */
+int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA, long long* candB,
+                       long long* hitsB);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH_CAPI_H */

@@ -1,0 +1,126 @@
+"""GPU parity of the neighbour-grid phase through the C ABI: calcHash,
+sortParticles and reorder (src/lib/mesh.cu:26-159, opt_src_v0/lib/mesh.cu:106-177)
+must be bit-exact against the oracle on the same inputs."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import case2d, case3d
+
+pytestmark = pytest.mark.gpu
+
+
+def _hash_sort_gpu(sph, pos):
+    n = pos.shape[0]
+    dpos = sph.DeviceArray.from_host(pos.astype(np.float32))
+    h = sph.DeviceArray((n,), np.int32)
+    i = sph.DeviceArray((n,), np.int32)
+    sph.calcHash(dpos, h, i, n)
+    hh = h.numpy()
+    sph.sortParticles(h, i, n)
+    return hh, h.numpy(), i.numpy()
+
+
+def _random_positions(prm, n, rng):
+    lo = np.array([prm.gridxmin, prm.gridymin, prm.gridzmin], np.float64)
+    size = np.array([prm.gridxdim, prm.gridydim, prm.gridzdim], np.float64) * prm.gridSize
+    pos = (lo + rng.random((n, 3)) * size).astype(np.float32)
+    # a quarter of the points exactly on cell faces (float32 of min + k*size)
+    k = n // 4
+    cells = rng.integers(0, [prm.gridxdim, prm.gridydim, prm.gridzdim], size=(k, 3))
+    pos[:k] = (lo + cells * np.float64(prm.gridSize)).astype(np.float32)
+    if prm.dim == 2:
+        pos[:, 1] = 0.0
+    return pos
+
+
+@pytest.mark.parametrize("which", ["ref3d", "c2d", "c3d"])
+def test_hash_sort_bit_exact(gpu, orc, which):
+    sph = gpu
+    rng = np.random.default_rng(7)
+    if which == "ref3d":
+        prm = sph.reference_params()
+    elif which == "c2d":
+        prm = case2d(0.01).params
+    else:
+        prm = case3d(0.02).params
+    sph.set_params(prm)
+    for n in (1, 2, 63, 64, 65, 4095, 4096, 4097, 100_003):
+        pos = _random_positions(prm, n, rng)
+        h_gpu, hs_gpu, idx_gpu = _hash_sort_gpu(sph, pos)
+        h_ref, i_ref = orc.calcHash(prm, pos)
+        np.testing.assert_array_equal(h_gpu, h_ref, err_msg=f"hash n={n}")
+        hs_ref, is_ref = orc.sortParticles(prm, h_ref, i_ref)
+        np.testing.assert_array_equal(hs_gpu, hs_ref, err_msg=f"sorted keys n={n}")
+        np.testing.assert_array_equal(idx_gpu, is_ref, err_msg=f"sorted idx n={n}")
+
+
+def test_sort_many_ties_and_full_range(gpu, orc):
+    """Heavy ties (stability) and keys spanning the whole cell range."""
+    sph = gpu
+    prm = sph.reference_params()
+    sph.set_params(prm)
+    rng = np.random.default_rng(3)
+    n = 300_000
+    for keys in (rng.integers(0, 5, n), rng.integers(0, prm.numCells, n), np.full(n, prm.numCells - 1),
+                 np.arange(n)[::-1] % prm.numCells):
+        keys = keys.astype(np.int32)
+        idx = np.arange(n, dtype=np.int32)
+        dk = sph.DeviceArray.from_host(keys)
+        di = sph.DeviceArray.from_host(idx)
+        sph.sortParticles(dk, di, n)
+        want_i = np.argsort(keys, kind="stable").astype(np.int32)
+        np.testing.assert_array_equal(di.numpy(), want_i)
+        np.testing.assert_array_equal(dk.numpy(), keys[want_i])
+
+
+def test_empty_input(gpu):
+    sph = gpu
+    prm = sph.reference_params()
+    sph.set_params(prm)
+    h = sph.DeviceArray((1,), np.int32)
+    i = sph.DeviceArray((1,), np.int32)
+    sph.calcHash(None, h, i, 0)
+    sph.sortParticles(h, i, 0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_reorder_bit_exact(gpu, orc, dim):
+    sph = gpu
+    case = case2d(0.02) if dim == 2 else case3d(0.04)
+    prm = case.params
+    sph.set_params(prm)
+    n = case.n
+    rng = np.random.default_rng(11)
+    perm = rng.permutation(n)  # shuffled input order
+    pos, vel = case.pos[perm], rng.standard_normal((n, 3)).astype(np.float32)
+    rho, p, typ = case.rho[perm], case.p[perm], case.type[perm]
+    zone = rng.integers(0, 9, n).astype(np.int32)
+    table_r = rng.integers(-50, 50, n).astype(np.int32)
+    h_ref, i_ref = orc.calcHash(prm, pos)
+    hs, is_ = orc.sortParticles(prm, h_ref, i_ref)
+    want = orc.reorder(prm, pos, vel, rho, p, typ, hs, is_, table_r)
+
+    d = {k: sph.DeviceArray.from_host(v) for k, v in dict(pos=pos, vel=vel, rho=rho, p=p, typ=typ, zone=zone,
+                                                             hs=hs, is_=is_, table_r=table_r).items()}
+    out = {k: sph.DeviceArray(s, t) for k, s, t in (("pos", (n, 3), np.float32), ("vel", (n, 3), np.float32),
+                                                    ("rho", (n,), np.float32), ("p", (n,), np.float32),
+                                                    ("typ", (n,), np.int32), ("zone", (n,), np.int32),
+                                                    ("table_l", (n,), np.int32))}
+    cs = sph.DeviceArray((prm.numCells,), np.int32)
+    ce = sph.DeviceArray((prm.numCells,), np.int32)
+    ce.fill_bytes(0)
+    sph.reorder(d["pos"], d["vel"], d["rho"], d["p"], d["typ"], d["zone"], out["pos"], out["vel"], out["rho"],
+                out["p"], out["typ"], out["zone"], d["hs"], d["is_"], cs, ce, n, out["table_l"], d["table_r"])
+    np.testing.assert_array_equal(out["pos"].numpy(), want["pos"])
+    np.testing.assert_array_equal(out["vel"].numpy(), want["vel"])
+    np.testing.assert_array_equal(out["rho"].numpy(), want["rho"])
+    np.testing.assert_array_equal(out["p"].numpy(), want["p"])
+    np.testing.assert_array_equal(out["typ"].numpy(), want["type"])
+    np.testing.assert_array_equal(out["zone"].numpy(), zone[is_])
+    np.testing.assert_array_equal(out["table_l"].numpy(), want["table_l"])
+    cs_g, ce_g = cs.numpy(), ce.numpy()
+    np.testing.assert_array_equal(cs_g, want["cellStart"])  # -1 marks empty cells
+    nonempty = want["cellStart"] != -1
+    np.testing.assert_array_equal(ce_g[nonempty], want["cellEnd"][nonempty])

@@ -1,0 +1,294 @@
+// C ABI of the MI355X delta-SPH step: runtime helpers, the reference call
+// surface and the simulation context (fast path).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sph_kernels.h"
+
+namespace sph {
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_mu;
+sph_params g_params;
+bool g_params_set = false;
+hipStream_t g_stream = nullptr;
+
+// Scratch of the reference-surface sortParticles, grown on demand like the
+// temporary storage thrust::sort_by_key allocates.
+struct SortPool {
+  int cap = 0;
+  int* keys_alt = nullptr;
+  int* vals_alt = nullptr;
+  unsigned* hist = nullptr;
+  unsigned* totals = nullptr;
+} g_pool;
+}  // namespace
+
+int set_error(int code, const char* msg) {
+  g_last_error = msg ? msg : "";
+  return code;
+}
+
+int hip_status(int hip_err, const char* where) {
+  if (hip_err == hipSuccess) return SPH_OK;
+  std::string m = std::string(where) + ": " + hipGetErrorString((hipError_t)hip_err);
+  g_last_error = m;
+  return hip_err > 0 ? hip_err : 1000 - hip_err;
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+#define SPH_HIP(call, where)                         \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_status(e_, where); \
+  } while (0)
+
+namespace {
+
+int current_params(sph_params* out, hipStream_t* s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_params_set) return set_error(SPH_E_NOPARAMS, "sph_set_params was not called");
+  *out = g_params;
+  *s = g_stream;
+  return SPH_OK;
+}
+
+int check_params(const sph_params* p) {
+  if (!p) return set_error(SPH_E_INVAL, "null params");
+  if (p->dim != 2 && p->dim != 3) return set_error(SPH_E_INVAL, "params.dim must be 2 or 3");
+  if (p->gridxdim <= 0 || p->gridydim <= 0 || p->gridzdim <= 0)
+    return set_error(SPH_E_INVAL, "params: grid dimensions must be positive");
+  if ((long long)p->gridxdim * p->gridydim * p->gridzdim != (long long)p->numCells)
+    return set_error(SPH_E_INVAL, "params: numCells != gridxdim*gridydim*gridzdim");
+  if (p->dim == 2 && p->gridydim != 1) return set_error(SPH_E_INVAL, "params: 2-D needs gridydim == 1");
+  if (!(p->h > 0.0f) || !(p->gridSize > 0.0f) || !(p->restDensity > 0.0f))
+    return set_error(SPH_E_INVAL, "params: h, gridSize and restDensity must be positive");
+  if (!(p->gridSize >= p->kh * 0.999999f))
+    return set_error(SPH_E_INVAL, "params: gridSize must be at least 2h for the 3-cell stencil");
+  if (p->eos != SPH_EOS_LINEAR && p->eos != SPH_EOS_TAIT) return set_error(SPH_E_INVAL, "params: unknown eos");
+  if (p->eos == SPH_EOS_TAIT && !(p->tait_B > 0.0f && p->gamma > 0.0f))
+    return set_error(SPH_E_INVAL, "params: Tait EOS needs tait_B > 0 and gamma > 0");
+  return SPH_OK;
+}
+
+int ensure_pool(int n) {
+  if (g_pool.cap >= n && g_pool.keys_alt) return SPH_OK;
+  if (g_pool.keys_alt) {
+    hipFree(g_pool.keys_alt);
+    hipFree(g_pool.vals_alt);
+    hipFree(g_pool.hist);
+    hipFree(g_pool.totals);
+    g_pool = SortPool();
+  }
+  const size_t nn = (size_t)(n > 1 ? n : 1);
+  SPH_HIP(hipMalloc(&g_pool.keys_alt, nn * sizeof(int)), "sort scratch");
+  SPH_HIP(hipMalloc(&g_pool.vals_alt, nn * sizeof(int)), "sort scratch");
+  SPH_HIP(hipMalloc(&g_pool.hist, (size_t)radix_hist_words((int)nn) * sizeof(unsigned)), "sort scratch");
+  SPH_HIP(hipMalloc(&g_pool.totals, 256 * sizeof(unsigned)), "sort scratch");
+  g_pool.cap = (int)nn;
+  return SPH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- runtime
+extern "C" int sph_abi_sizes(int* params_size, int* cfg_size, int* tuning_size) {
+  if (params_size) *params_size = (int)sizeof(sph_params);
+  if (cfg_size) *cfg_size = (int)sizeof(sph_dambreak_cfg);
+  if (tuning_size) *tuning_size = (int)sizeof(sph_tuning);
+  return SPH_OK;
+}
+
+extern "C" const char* sph_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int sph_device_count(int* count) {
+  if (!count) return set_error(SPH_E_INVAL, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return SPH_OK;
+}
+
+extern "C" int sph_set_device(int device) {
+  SPH_HIP(hipSetDevice(device), "hipSetDevice");
+  return SPH_OK;
+}
+
+extern "C" int sph_set_stream(void* s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_stream = (hipStream_t)s;
+  return SPH_OK;
+}
+
+extern "C" int sph_device_synchronize(void) {
+  SPH_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return SPH_OK;
+}
+
+extern "C" int sph_malloc(void** p, unsigned long long bytes) {
+  if (!p) return set_error(SPH_E_INVAL, "null pointer");
+  *p = nullptr;
+  SPH_HIP(hipMalloc(p, bytes ? bytes : 1), "hipMalloc");
+  return SPH_OK;
+}
+
+extern "C" int sph_free(void* p) {
+  if (p) SPH_HIP(hipFree(p), "hipFree");
+  return SPH_OK;
+}
+
+extern "C" int sph_memcpy_htod(void* dst, const void* src, unsigned long long bytes) {
+  if (bytes) SPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  return SPH_OK;
+}
+
+extern "C" int sph_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes) {
+  if (bytes) SPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  return SPH_OK;
+}
+
+extern "C" int sph_memcpy_dtod(void* dst, const void* src, unsigned long long bytes) {
+  if (bytes) SPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+  return SPH_OK;
+}
+
+extern "C" int sph_memset(void* p, int value, unsigned long long bytes) {
+  if (bytes) SPH_HIP(hipMemset(p, value, bytes), "hipMemset");
+  return SPH_OK;
+}
+
+// ------------------------------------------------------------- parameters
+extern "C" int sph_set_params(const sph_params* p) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_params = *p;
+  g_params_set = true;
+  return SPH_OK;
+}
+
+extern "C" int sph_get_params(sph_params* p) {
+  if (!p) return set_error(SPH_E_INVAL, "null params");
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_params_set) return set_error(SPH_E_NOPARAMS, "sph_set_params was not called");
+  *p = g_params;
+  return SPH_OK;
+}
+
+// ------------------------------------------------- reference call surface
+extern "C" int sph_calcHash(const float* pos, int* hash, int* idx, int n) {
+  sph_params P;
+  hipStream_t s;
+  int rc = current_params(&P, &s);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!pos || !hash || !idx))) return set_error(SPH_E_INVAL, "sph_calcHash: bad argument");
+  SPH_HIP(launch_hash_aos(P, pos, hash, idx, n, s), "calcHash");
+  return SPH_OK;
+}
+
+extern "C" int sph_sortParticles(int* hash, int* idx, int n) {
+  sph_params P;
+  hipStream_t s;
+  int rc = current_params(&P, &s);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!hash || !idx))) return set_error(SPH_E_INVAL, "sph_sortParticles: bad argument");
+  if (n <= 1) return SPH_OK;
+  std::lock_guard<std::mutex> lk(g_mu);
+  rc = ensure_pool(n);
+  if (rc) return rc;
+  bool alt = false;
+  SPH_HIP(radix_sort_pairs(hash, idx, g_pool.keys_alt, g_pool.vals_alt, g_pool.hist, g_pool.totals, n,
+                           key_bits(P.numCells), s, &alt),
+          "sortParticles");
+  if (alt) {
+    SPH_HIP(hipMemcpyAsync(hash, g_pool.keys_alt, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s),
+            "sortParticles copy");
+    SPH_HIP(hipMemcpyAsync(idx, g_pool.vals_alt, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s),
+            "sortParticles copy");
+  }
+  // the pool is reused by the next call: keep the stream ordered behind it
+  SPH_HIP(hipStreamSynchronize(s), "sortParticles sync");
+  return SPH_OK;
+}
+
+extern "C" int sph_reorder(const float* oPos, const float* oVel, const float* oRho, const float* oP,
+                           const int* oType, const int* oZone, float* sPos, float* sVel, float* sRho,
+                           float* sP, int* sType, int* sZone, const int* hash, const int* idx,
+                           int* cellStart, int* cellEnd, int n, int* table_l, const int* table_r) {
+  sph_params P;
+  hipStream_t s;
+  int rc = current_params(&P, &s);
+  if (rc) return rc;
+  if (n < 0 || !cellStart || !cellEnd ||
+      (n > 0 && (!oPos || !oVel || !oRho || !oP || !oType || !sPos || !sVel || !sRho || !sP || !sType ||
+                 !hash || !idx)))
+    return set_error(SPH_E_INVAL, "sph_reorder: bad argument");
+  SPH_HIP(launch_reorder_aos(P, oPos, oVel, oRho, oP, oType, oZone, sPos, sVel, sRho, sP, sType, sZone,
+                             hash, idx, cellStart, cellEnd, n, table_l, table_r, s),
+          "reorder");
+  return SPH_OK;
+}
+
+extern "C" int sph_SPH_NS_simpleversion(const float* sPos, const float* sVel, float* sRho, float* sP,
+                                        const int* sType, float* densitydt, float* Veldt,
+                                        const int* cellStart, const int* cellEnd, int n,
+                                        const int* hash, int timestep, float* dofv, float* rhop_sum,
+                                        float* w_sum) {
+  (void)timestep;
+  (void)rhop_sum;
+  (void)w_sum;
+  sph_params P;
+  hipStream_t s;
+  int rc = current_params(&P, &s);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!sPos || !sVel || !sRho || !sP || !sType || !densitydt || !Veldt || !cellStart ||
+                          !cellEnd || !hash || !dofv)))
+    return set_error(SPH_E_INVAL, "sph_SPH_NS_simpleversion: bad argument");
+  SPH_HIP(launch_ns_aos(P, sPos, sVel, sRho, sP, sType, densitydt, Veldt, cellStart, cellEnd, hash, dofv,
+                        n, s),
+          "SPH_NS_simpleversion");
+  return SPH_OK;
+}
+
+static int pc_common(bool pred, float* sPos, float* sVel, float* sRho, float* sP, const float* drho,
+                     const float* acc, const int* idx, float* tPos, float* tVel, float* tRho, int n,
+                     int* sType) {
+  sph_params P;
+  hipStream_t s;
+  int rc = current_params(&P, &s);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!sPos || !sVel || !sRho || !sP || !drho || !acc || !tPos || !tVel || !tRho ||
+                          !sType || (!pred && !idx))))
+    return set_error(SPH_E_INVAL, "sph_PC: bad argument");
+  SPH_HIP(launch_pc_aos(P, pred, sPos, sVel, sRho, sP, drho, acc, idx, tPos, tVel, tRho, n, sType, s),
+          pred ? "PC_prediction" : "PC_correction");
+  return SPH_OK;
+}
+
+extern "C" int sph_PC_prediction(float* sPos, float* sVel, float* sRho, float* sP, const float* densitydt,
+                                 const float* Veldt, const int* hash, const int* idx, float* Pos_tmp,
+                                 float* Vel_tmp, float* density_tmp, int n, int* sType) {
+  (void)hash;
+  return pc_common(true, sPos, sVel, sRho, sP, densitydt, Veldt, idx, Pos_tmp, Vel_tmp, density_tmp, n,
+                   sType);
+}
+
+extern "C" int sph_PC_correction(float* sPos, float* sVel, float* sRho, float* sP, const float* densitydt,
+                                 const float* Veldt, const int* hash, const int* idx,
+                                 const float* Pos_tmp, const float* Vel_tmp, const float* density_tmp,
+                                 int n, int* sType) {
+  (void)hash;
+  return pc_common(false, sPos, sVel, sRho, sP, densitydt, Veldt, idx, const_cast<float*>(Pos_tmp),
+                   const_cast<float*>(Vel_tmp), const_cast<float*>(density_tmp), n, sType);
+}

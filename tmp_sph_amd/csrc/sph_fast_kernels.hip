@@ -1,0 +1,534 @@
+// Fast-path kernels of the delta-SPH step for gfx950.
+//
+// Layout (one particle set, sorted by cell):
+//   PR[i] = (x, y, z, +rho | -rho)  float4; the sign of w marks fluid (+)
+//   VP[i] = (vx, vy, vz, p)         float4
+//   X[k][i]                         FP64 positions, SoA; PR.xyz == (float)X
+//   ID[i], TY[i]                    original id and reference type
+// A neighbour candidate costs one 16-byte load (PR); VP and dofv are loaded
+// only for pairs inside the support.  Positions are FP64 only in the
+// integrator; hashing and the neighbour loops use the FP32 copy.
+//
+// Physics per kernel (reference formulas, reorganised for fewer divides):
+//   nsA  opt_src_v1/equation/ns.cu:30-134   wall p extrapolation, dofv
+//   nsB  opt_src_v1/equation/ns.cu:137-230  drho/dt, dv/dt
+//   pc   opt_src_v0/sph/sph_run.cu:388-484  predictor / corrector + EOS
+#include "sph_device.h"
+#include "sph_kernels.h"
+
+namespace sph {
+
+namespace {
+
+inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__global__ void hash_fast_kernel(sph_params P, const float4* __restrict__ PR, int* __restrict__ keys,
+                                 int* __restrict__ idx, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = PR[i];
+  keys[i] = cell_hash(P, p.x, p.y, p.z);
+  idx[i] = i;
+}
+
+template <int DIM>
+__global__ void cells_gather_kernel(FastSet in, FastSet out, const int* __restrict__ keys,
+                                    const int* __restrict__ idx, int* __restrict__ cellStart,
+                                    int* __restrict__ cellEnd, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int h = keys[i];
+  if (i == 0) {
+    cellStart[h] = 0;
+  } else {
+    const int hp = keys[i - 1];
+    if (h != hp) {
+      cellStart[h] = i;
+      cellEnd[hp] = i;
+    }
+  }
+  if (i == n - 1) cellEnd[h] = n;
+  const int src = idx[i];
+  out.PR[i] = in.PR[src];
+  out.VP[i] = in.VP[src];
+  out.X[0][i] = in.X[0][src];
+  if (DIM == 3) out.X[1][i] = in.X[1][src];
+  out.X[2][i] = in.X[2][src];
+  out.ID[i] = in.ID[src];
+  out.TY[i] = in.TY[src];
+}
+
+struct RowRange {
+  int s, e;
+};
+
+// Sorted index ranges of the stencil rows of cell `hash`, in the reference
+// order (z outer, y inner; the x-adjacent cells of a row are contiguous).
+template <int DIM>
+__device__ __forceinline__ int row_ranges(const sph_params& P, int hash,
+                                          const int* __restrict__ cellStart,
+                                          const int* __restrict__ cellEnd,
+                                          RowRange (&rows)[DIM == 3 ? 9 : 3]) {
+  const int X = P.gridxdim, Y = P.gridydim, Z = P.gridzdim;
+  const int XY = X * Y;
+  const int cz = hash / XY;
+  const int rem = hash - cz * XY;
+  const int cy = rem / X;
+  const int cx = rem - cy * X;
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx < X - 1 ? cx + 1 : X - 1;
+  int nr = 0;
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll
+    for (int dy = (DIM == 3 ? -1 : 0); dy <= (DIM == 3 ? 1 : 0); ++dy) {
+      const int z = cz + dz, y = cy + dy;
+      int s = 0, e = 0;
+      if (z >= 0 && z < Z && y >= 0 && y < Y) {
+        const int rowBase = z * XY + y * X;
+        s = -1;
+        for (int c = rowBase + x0; c <= rowBase + x1; ++c) {
+          const int st = cellStart[c];
+          if (st != -1) {
+            if (s < 0) s = st;
+            e = cellEnd[c];
+          }
+        }
+        if (s < 0) s = e = 0;
+      }
+      rows[nr].s = s;
+      rows[nr].e = e;
+      ++nr;
+    }
+  }
+  return nr;
+}
+
+// Kernel A: wall pressure by Shepard extrapolation with the hydrostatic
+// correction, and the velocity divergence of fluid particles.
+template <int DIM>
+__global__ __launch_bounds__(256) void nsA_fast_kernel(sph_params P, float4* PR, float4* VP,
+                                                       float* __restrict__ DV,
+                                                       const int* __restrict__ keys,
+                                                       const int* __restrict__ cellStart,
+                                                       const int* __restrict__ cellEnd, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = PR[i];
+  const float4 vi = VP[i];
+  const bool fi = pi.w > 0.0f;
+  RowRange rows[DIM == 3 ? 9 : 3];
+  const int nr = row_ranges<DIM>(P, keys[i], cellStart, cellEnd, rows);
+  const float inv_h = 1.0f / P.h;
+  const float cfr = -5.0f * P.adh / (P.h * P.h);
+  const float mg = 0.0f - P.gravity;
+  const float m = P.particleMass;
+  float sp = 0.0f, sw = 0.0f, dv = 0.0f;
+  for (int r = 0; r < nr; ++r) {
+    const int e = rows[r].e;
+    for (int j = rows[r].s; j < e; ++j) {
+      const float4 pj = PR[j];
+      const float dx = pi.x - pj.x;
+      const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
+      const float dz = pi.z - pj.z;
+      const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
+      if (rr <= P.rr_cut && j != i && pj.w > 0.0f) {
+        const float q = fast_sqrt(rr) * inv_h;
+        const float t = 1.0f - 0.5f * q;
+        const float t3 = t * t * t;
+        const float4 vj = VP[j];
+        if (!fi) {
+          const float w = t3 * t * ((2.0f * q + 1.0f) * P.adh);
+          sp += (vj.w - pj.w * (mg * dz)) * w;
+          sw += w;
+        } else {
+          const float dvdx = DIM == 3 ? (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz
+                                      : (vi.x - vj.x) * dx + (vi.z - vj.z) * dz;
+          dv -= cfr * t3 * dvdx * m * fast_rcp(pj.w);
+        }
+      }
+    }
+  }
+  DV[i] = dv;
+  if (!fi) {
+    float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
+    if (pw < 0.0f) pw = 0.0f;
+    reinterpret_cast<float*>(VP)[4 * i + 3] = pw;
+    reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
+  }
+}
+
+// Kernel B: continuity with delta-SPH diffusion and momentum with pressure,
+// artificial viscosity, acoustic damper and gravity.  Non-fluid targets are
+// skipped: the integrator never reads their rates.
+template <int DIM>
+__global__ __launch_bounds__(256) void nsB_fast_kernel(sph_params P, const float4* __restrict__ PR,
+                                                       const float4* __restrict__ VP,
+                                                       const float* __restrict__ DV,
+                                                       const int* __restrict__ keys,
+                                                       const int* __restrict__ cellStart,
+                                                       const int* __restrict__ cellEnd,
+                                                       float4* __restrict__ RATE, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = PR[i];
+  if (!(pi.w > 0.0f)) {
+    RATE[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const float4 vi = VP[i];
+  const float rho_i = pi.w;
+  const float dofv_i = P.damper * DV[i];
+  RowRange rows[DIM == 3 ? 9 : 3];
+  const int nr = row_ranges<DIM>(P, keys[i], cellStart, cellEnd, rows);
+  const float inv_h = 1.0f / P.h;
+  const float cfr = -5.0f * P.adh / (P.h * P.h);
+  const float m = P.particleMass;
+  const float dhc = P.delta * P.h * P.cs;
+  const float eta2 = P.eta * P.eta;
+  const float inv_rho_i = 1.0f / rho_i;
+  const float avc = P.h * P.cs * P.restDensity * m * inv_rho_i;
+  const float pm_i = m * inv_rho_i;
+  float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int r = 0; r < nr; ++r) {
+    const int e = rows[r].e;
+    for (int j = rows[r].s; j < e; ++j) {
+      const float4 pj = PR[j];
+      const float dx = pi.x - pj.x;
+      const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
+      const float dz = pi.z - pj.z;
+      const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
+      if (rr <= P.rr_cut && j != i) {
+        const float4 vj = VP[j];
+        const bool fj = pj.w > 0.0f;
+        const float rho_j = fabsf(pj.w);
+        const float q = fast_sqrt(rr) * inv_h;
+        const float t = 1.0f - 0.5f * q;
+        const float fr = cfr * (t * t * t);
+        const float inv_rho_j = fast_rcp(rho_j);
+        const float inv_re = fast_rcp(rr + eta2);
+        const float dvdx = DIM == 3 ? (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz
+                                    : (vi.x - vj.x) * dx + (vi.z - vj.z) * dz;
+        const float frj = fr * inv_rho_j;
+        drho += frj * m * (rho_i * dvdx + (rho_i - rho_j) * rr * dhc * inv_re);
+        float vt = -(vi.w + vj.w) * pm_i;
+        if (fj) vt += avc * (dofv_i + P.damper * DV[j] + P.afa * dvdx * inv_re);
+        vt *= frj;
+        ax += vt * dx;
+        if (DIM == 3) ay += vt * dy;
+        az += vt * dz;
+      }
+    }
+  }
+  RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
+}
+
+__device__ __forceinline__ bool outside64(const sph_params& P, double x, double y, double z) {
+  return x > P.domain_max[0] || x < P.domain_min[0] || y > P.domain_max[1] ||
+         y < P.domain_min[1] || z > P.domain_max[2] || z < P.domain_min[2];
+}
+
+// Predictor (PRED) / corrector of the reference PC scheme with FP64
+// positions.  The predictor stores the step-start state at its own sorted
+// slot; the corrector finds it through perm (the half-step-2 sort order).
+template <int DIM, bool PRED>
+__global__ void pc_fast_kernel(sph_params P, FastSet st, FastWork w, const int* __restrict__ perm,
+                               int n) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 pr = st.PR[i];
+  if (!(pr.w > 0.0f)) return;  // fluid <=> type 1 <=> positive signed density
+  const double x0 = st.X[0][i];
+  const double y0 = DIM == 3 ? st.X[1][i] : 0.0;
+  const double z0 = st.X[2][i];
+  if (outside64(P, x0, y0, z0)) {
+    st.X[0][i] = P.park_pos[0];
+    if (DIM == 3) st.X[1][i] = P.park_pos[1];
+    st.X[2][i] = P.park_pos[2];
+    const float rs = P.park_type == 1 ? P.restDensity : -P.restDensity;
+    st.PR[i] = make_float4(P.park_pos[0], DIM == 3 ? P.park_pos[1] : 0.0f, P.park_pos[2], rs);
+    st.VP[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    st.TY[i] = P.park_type;
+    return;
+  }
+  const float4 vp = st.VP[i];
+  const float4 rate = w.RATE[i];
+  const float dt = P.timeStep;
+  const double hdt = 0.5 * (double)dt;
+  double bx, by, bz;
+  float4 bv;  // (vx, vy, vz, rho) at step start
+  if (PRED) {
+    bx = x0; by = y0; bz = z0;
+    bv = make_float4(vp.x, vp.y, vp.z, pr.w);
+    w.TX[0][i] = bx;
+    if (DIM == 3) w.TX[1][i] = by;
+    w.TX[2][i] = bz;
+    w.TV[i] = bv;
+  } else {
+    const int r = perm[i];
+    bx = w.TX[0][r];
+    by = DIM == 3 ? w.TX[1][r] : 0.0;
+    bz = w.TX[2][r];
+    bv = w.TV[r];
+  }
+  float rho = (float)((double)bv.w + (double)(rate.x * dt) / 2.0);
+  if (rho < P.restDensity) rho = P.restDensity;
+  double nx = bx + (double)vp.x * hdt;
+  double ny = DIM == 3 ? by + (double)vp.y * hdt : 0.0;
+  double nz = bz + (double)vp.z * hdt;
+  float vx = (float)((double)bv.x + (double)(rate.y * dt) / 2.0);
+  float vy = DIM == 3 ? (float)((double)bv.y + (double)(rate.z * dt) / 2.0) : 0.0f;
+  float vz = (float)((double)bv.z + (double)(rate.w * dt) / 2.0);
+  if (!PRED) {
+    rho = 2 * rho - bv.w;
+    if (rho < P.restDensity) rho = P.restDensity;
+    nx = 2.0 * nx - bx;
+    ny = DIM == 3 ? 2.0 * ny - by : 0.0;
+    nz = 2.0 * nz - bz;
+    vx = 2 * vx - bv.x;
+    vy = DIM == 3 ? 2 * vy - bv.y : 0.0f;
+    vz = 2 * vz - bv.z;
+  }
+  st.X[0][i] = nx;
+  if (DIM == 3) st.X[1][i] = ny;
+  st.X[2][i] = nz;
+  st.PR[i] = make_float4((float)nx, (float)ny, (float)nz, rho);
+  st.VP[i] = make_float4(vx, vy, vz, eos_pressure(P, rho));
+}
+
+__global__ void upload_kernel(sph_params P, FastSet st, const float* __restrict__ pos,
+                              const float* __restrict__ vel, const float* __restrict__ rho,
+                              const float* __restrict__ prs, const int* __restrict__ type, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = pos[3 * i], y = P.dim == 3 ? pos[3 * i + 1] : 0.0f, z = pos[3 * i + 2];
+  const int t = type[i];
+  st.X[0][i] = (double)x;
+  if (st.X[1]) st.X[1][i] = (double)y;
+  st.X[2][i] = (double)z;
+  st.PR[i] = make_float4(x, y, z, t == 1 ? fabsf(rho[i]) : -fabsf(rho[i]));
+  st.VP[i] = make_float4(vel[3 * i], P.dim == 3 ? vel[3 * i + 1] : 0.0f, vel[3 * i + 2], prs[i]);
+  st.ID[i] = i;
+  st.TY[i] = t;
+}
+
+__global__ void download_kernel(FastSet st, float* __restrict__ pos, float* __restrict__ vel,
+                                float* __restrict__ rho, float* __restrict__ prs,
+                                int* __restrict__ type, double* __restrict__ pos64, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = st.ID[i];
+  const float4 pr = st.PR[i];
+  const float4 vp = st.VP[i];
+  const double x = st.X[0][i], y = st.X[1] ? st.X[1][i] : 0.0, z = st.X[2][i];
+  if (pos) {
+    pos[3 * o] = (float)x;
+    pos[3 * o + 1] = (float)y;
+    pos[3 * o + 2] = (float)z;
+  }
+  if (pos64) {
+    pos64[3 * o] = x;
+    pos64[3 * o + 1] = y;
+    pos64[3 * o + 2] = z;
+  }
+  if (vel) {
+    vel[3 * o] = vp.x;
+    vel[3 * o + 1] = vp.y;
+    vel[3 * o + 2] = vp.z;
+  }
+  if (rho) rho[o] = fabsf(pr.w);
+  if (prs) prs[o] = vp.w;
+  if (type) type[o] = st.TY[i];
+}
+
+// Candidate / in-support pair counts of kernels A and B (statistics only).
+template <int DIM>
+__global__ void pair_stats_kernel(sph_params P, const float4* __restrict__ PR, const int* __restrict__ keys,
+                                  const int* __restrict__ cellStart, const int* __restrict__ cellEnd,
+                                  unsigned long long* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c = 0, hA = 0, hB = 0;
+  bool fi = false;
+  if (i < n) {
+    const float4 pi = PR[i];
+    fi = pi.w > 0.0f;
+    RowRange rows[DIM == 3 ? 9 : 3];
+    const int nr = row_ranges<DIM>(P, keys[i], cellStart, cellEnd, rows);
+    for (int r = 0; r < nr; ++r)
+      for (int j = rows[r].s; j < rows[r].e; ++j) {
+        if (j == i) continue;
+        ++c;
+        const float4 pj = PR[j];
+        const float dx = pi.x - pj.x, dy = DIM == 3 ? pi.y - pj.y : 0.0f, dz = pi.z - pj.z;
+        const float rr = dx * dx + dy * dy + dz * dz;
+        if (rr <= P.rr_cut) {
+          ++hB;
+          if (pj.w > 0.0f) ++hA;
+        }
+      }
+  }
+  unsigned long long v[4] = {c, hA, fi ? c : 0ull, fi ? hB : 0ull};
+  for (int k = 0; k < 4; ++k) {
+    unsigned long long x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&out[k], x);
+  }
+}
+
+__global__ void download_rates_kernel(FastSet st, FastWork w, float* __restrict__ drho,
+                                      float* __restrict__ acc, float* __restrict__ dofv, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = st.ID[i];
+  const float4 r = w.RATE[i];
+  if (drho) drho[o] = r.x;
+  if (acc) {
+    acc[3 * o] = r.y;
+    acc[3 * o + 1] = r.z;
+    acc[3 * o + 2] = r.w;
+  }
+  if (dofv) dofv[o] = w.DV[i];
+}
+
+// Surge front (max x of in-domain fluid) and column height (max z of fluid
+// with x <= band).  Both are >= 0 inside the domain, so the IEEE bit
+// patterns order like the values and an integer atomicMax reduces them.
+__global__ void probe_kernel(sph_params P, FastSet st, double band,
+                             unsigned long long* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double front = 0.0, height = 0.0;
+  if (i < n && st.TY[i] == 1) {
+    const double x = st.X[0][i], z = st.X[2][i];
+    const double y = st.X[1] ? st.X[1][i] : 0.0;
+    if (!outside64(P, x, y, z)) {
+      front = x;
+      if (x <= band) height = z;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    front = fmax(front, __shfl_xor(front, off));
+    height = fmax(height, __shfl_xor(height, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&out[0], (unsigned long long)__double_as_longlong(front));
+    atomicMax(&out[1], (unsigned long long)__double_as_longlong(height));
+  }
+}
+
+}  // namespace
+
+hipError_t launch_hash_fast(const sph_params& P, const FastSet& in, int* keys, int* idx, int n,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(hash_fast_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, in.PR, keys,
+                     idx, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_cells_gather_fast(const sph_params& P, const FastSet& in, const FastSet& out,
+                                    const int* keys, const int* idx, int* cellStart,
+                                    int* cellEnd, int n, hipStream_t s) {
+  hipError_t e = fill_int(cellStart, -1, P.numCells, s);
+  if (e != hipSuccess || n <= 0) return e;
+  if (P.dim == 3)
+    hipLaunchKernelGGL(cells_gather_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, out,
+                       keys, idx, cellStart, cellEnd, n);
+  else
+    hipLaunchKernelGGL(cells_gather_kernel<2>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, out,
+                       keys, idx, cellStart, cellEnd, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_nsA_fast(const sph_params& P, const FastSet& st, const FastWork& w, int n,
+                           int block, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (block <= 0 || block > 256) block = 256;
+  if (P.dim == 3)
+    hipLaunchKernelGGL(nsA_fast_kernel<3>, dim3(blocks_for(n, block)), dim3(block), 0, s, P, st.PR,
+                       st.VP, w.DV, w.keys, w.cellStart, w.cellEnd, n);
+  else
+    hipLaunchKernelGGL(nsA_fast_kernel<2>, dim3(blocks_for(n, block)), dim3(block), 0, s, P, st.PR,
+                       st.VP, w.DV, w.keys, w.cellStart, w.cellEnd, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_nsB_fast(const sph_params& P, const FastSet& st, const FastWork& w, int n,
+                           int block, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (block <= 0 || block > 256) block = 256;
+  if (P.dim == 3)
+    hipLaunchKernelGGL(nsB_fast_kernel<3>, dim3(blocks_for(n, block)), dim3(block), 0, s, P, st.PR,
+                       st.VP, w.DV, w.keys, w.cellStart, w.cellEnd, w.RATE, n);
+  else
+    hipLaunchKernelGGL(nsB_fast_kernel<2>, dim3(blocks_for(n, block)), dim3(block), 0, s, P, st.PR,
+                       st.VP, w.DV, w.keys, w.cellStart, w.cellEnd, w.RATE, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pc_fast(const sph_params& P, bool predictor, const FastSet& st,
+                          const FastWork& w, const int* perm, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 g(blocks_for(n, 256)), b(256);
+  if (P.dim == 3) {
+    if (predictor)
+      hipLaunchKernelGGL((pc_fast_kernel<3, true>), g, b, 0, s, P, st, w, perm, n);
+    else
+      hipLaunchKernelGGL((pc_fast_kernel<3, false>), g, b, 0, s, P, st, w, perm, n);
+  } else {
+    if (predictor)
+      hipLaunchKernelGGL((pc_fast_kernel<2, true>), g, b, 0, s, P, st, w, perm, n);
+    else
+      hipLaunchKernelGGL((pc_fast_kernel<2, false>), g, b, 0, s, P, st, w, perm, n);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_upload_fast(const sph_params& P, const FastSet& st, const float* pos,
+                              const float* vel, const float* rho, const float* prs,
+                              const int* type, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(upload_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, pos, vel, rho,
+                     prs, type, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_download_fast(const FastSet& st, float* pos, float* vel, float* rho, float* prs,
+                                int* type, double* pos64, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(download_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, pos, vel, rho,
+                     prs, type, pos64, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_download_rates(const FastSet& st, const FastWork& w, float* drho, float* acc,
+                                 float* dofv, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(download_rates_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, w, drho, acc,
+                     dofv, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_stats(const sph_params& P, const FastSet& st, const FastWork& w,
+                             unsigned long long* out4, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (P.dim == 3)
+    hipLaunchKernelGGL(pair_stats_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st.PR, w.keys,
+                       w.cellStart, w.cellEnd, out4, n);
+  else
+    hipLaunchKernelGGL(pair_stats_kernel<2>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st.PR, w.keys,
+                       w.cellStart, w.cellEnd, out4, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_probe_fast(const sph_params& P, const FastSet& st, double band,
+                             unsigned long long* out2, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(probe_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, band, out2, n);
+  return hipGetLastError();
+}
+
+}  // namespace sph

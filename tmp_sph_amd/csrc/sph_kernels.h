@@ -1,0 +1,89 @@
+// Host-side launchers of the gfx950 kernels (internal C++ interface).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sph_internal.h"
+
+namespace sph {
+
+// ----------------------------------------------------------------- radix sort
+// Stable LSD radix sort of (key, value) int pairs on keys in [0, 2^bits).
+// Scratch: keys_alt/vals_alt of n ints, hist of radix_hist_words(n) words,
+// totals of 256 words.  The sorted pairs end in (keys, vals) or in
+// (keys_alt, vals_alt); *result_in_alt tells which.
+int radix_hist_words(int n);
+hipError_t radix_sort_pairs(int* keys, int* vals, int* keys_alt, int* vals_alt, unsigned* hist,
+                            unsigned* totals, int n, int bits, hipStream_t s, bool* result_in_alt);
+
+hipError_t fill_int(int* p, int value, long long n, hipStream_t s);
+
+// --------------------------------------------------- reference-surface kernels
+hipError_t launch_hash_aos(const sph_params& P, const float* pos, int* hash, int* idx, int n,
+                           hipStream_t s);
+hipError_t launch_reorder_aos(const sph_params& P, const float* oPos, const float* oVel,
+                              const float* oRho, const float* oP, const int* oType,
+                              const int* oZone, float* sPos, float* sVel, float* sRho, float* sP,
+                              int* sType, int* sZone, const int* hash, const int* idx,
+                              int* cellStart, int* cellEnd, int n, int* table_l,
+                              const int* table_r, hipStream_t s);
+hipError_t launch_ns_aos(const sph_params& P, const float* sPos, const float* sVel, float* sRho,
+                         float* sP, const int* sType, float* drho, float* acc,
+                         const int* cellStart, const int* cellEnd, const int* hash, float* dofv,
+                         int n, hipStream_t s);
+hipError_t launch_pc_aos(const sph_params& P, bool predictor, float* sPos, float* sVel,
+                         float* sRho, float* sP, const float* drho, const float* acc,
+                         const int* idx, float* tPos, float* tVel, float* tRho, int n,
+                         int* sType, hipStream_t s);
+
+// ------------------------------------------------------------- fast path
+// One sorted particle set.  PR = (x, y, z, rho signed: + fluid, - otherwise),
+// VP = (vx, vy, vz, p); X = FP64 positions (SoA); ID = original id; TY = type.
+struct FastSet {
+  float4* PR;
+  float4* VP;
+  double* X[3];
+  int* ID;
+  int* TY;
+};
+
+// Per-half-step work arrays shared by both particle sets.
+struct FastWork {
+  int* keys;
+  int* idx;
+  int* keys_alt;
+  int* idx_alt;
+  unsigned* hist;
+  unsigned* totals;
+  int* cellStart;
+  int* cellEnd;
+  float* DV;       // velocity divergence (acoustic damper)
+  float4* RATE;    // (drho/dt, ax, ay, az)
+  double* TX[3];   // positions at the start of the step (predictor slot order)
+  float4* TV;      // (vx, vy, vz, rho) at the start of the step
+};
+
+hipError_t launch_hash_fast(const sph_params& P, const FastSet& in, int* keys, int* idx, int n,
+                            hipStream_t s);
+hipError_t launch_cells_gather_fast(const sph_params& P, const FastSet& in, const FastSet& out,
+                                    const int* keys, const int* idx, int* cellStart,
+                                    int* cellEnd, int n, hipStream_t s);
+hipError_t launch_nsA_fast(const sph_params& P, const FastSet& st, const FastWork& w, int n,
+                           int block, hipStream_t s);
+hipError_t launch_nsB_fast(const sph_params& P, const FastSet& st, const FastWork& w, int n,
+                           int block, hipStream_t s);
+hipError_t launch_pc_fast(const sph_params& P, bool predictor, const FastSet& st,
+                          const FastWork& w, const int* perm, int n, hipStream_t s);
+hipError_t launch_upload_fast(const sph_params& P, const FastSet& st, const float* pos,
+                              const float* vel, const float* rho, const float* prs,
+                              const int* type, int n, hipStream_t s);
+hipError_t launch_download_fast(const FastSet& st, float* pos, float* vel, float* rho, float* prs,
+                                int* type, double* pos64, int n, hipStream_t s);
+hipError_t launch_download_rates(const FastSet& st, const FastWork& w, float* drho, float* acc,
+                                 float* dofv, int n, hipStream_t s);
+hipError_t launch_pair_stats(const sph_params& P, const FastSet& st, const FastWork& w,
+                             unsigned long long* out4, int n, hipStream_t s);
+hipError_t launch_probe_fast(const sph_params& P, const FastSet& st, double band,
+                             unsigned long long* out2, int n, hipStream_t s);
+
+}  // namespace sph

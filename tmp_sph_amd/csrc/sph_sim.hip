@@ -1,0 +1,468 @@
+// Simulation context: owns two sorted particle sets (ping-pong, as in
+// opt_src_v0/sph/sph_run.cu:269-309) and the per-half-step work arrays, and
+// runs full predictor-corrector steps on one HIP stream, eagerly or from a
+// captured hipGraph.  Nothing is allocated after sph_sim_create.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sph_kernels.h"
+
+using namespace sph;
+
+struct sph_sim {
+  sph_params P{};
+  int n = 0;
+  int bits = 1;
+  hipStream_t stream = nullptr;
+  FastSet set[2]{};
+  FastWork w{};
+  sph_tuning tun{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  long long steps = 0;
+  unsigned long long* probe = nullptr;
+  std::vector<void*> allocs;
+  // kernel timing (eager steps with tuning.flags bit 0)
+  std::vector<std::string> tnames;
+  std::vector<double> tsum_ms;
+  std::vector<long long> tcount;
+  struct Mark {
+    int slot;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> event_pool;
+  size_t event_next = 0;
+};
+
+#define SPH_HIP(call, where)                            \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return hip_status(e_, where); \
+  } while (0)
+
+namespace {
+
+template <class T>
+int dalloc(sph_sim* s, T** p, size_t count) {
+  void* q = nullptr;
+  SPH_HIP(hipMalloc(&q, (count ? count : 1) * sizeof(T)), "sph_sim_create: hipMalloc");
+  s->allocs.push_back(q);
+  *p = (T*)q;
+  return SPH_OK;
+}
+
+hipEvent_t next_event(sph_sim* s) {
+  if (s->event_next == s->event_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    s->event_pool.push_back(e);
+  }
+  return s->event_pool[s->event_next++];
+}
+
+int slot_of(sph_sim* s, const char* name) {
+  for (size_t k = 0; k < s->tnames.size(); ++k)
+    if (s->tnames[k] == name) return (int)k;
+  s->tnames.push_back(name);
+  s->tsum_ms.push_back(0.0);
+  s->tcount.push_back(0);
+  return (int)s->tnames.size() - 1;
+}
+
+// Runs `launch` and, in timing mode, brackets it with events on the stream.
+template <class F>
+hipError_t timed(sph_sim* s, bool timing, const char* name, F&& launch) {
+  if (!timing) return launch();
+  hipEvent_t a = next_event(s), b = next_event(s);
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipEventRecord(a, s->stream);
+  hipError_t e = launch();
+  hipEventRecord(b, s->stream);
+  s->marks.push_back({slot_of(s, name), a, b});
+  return e;
+}
+
+hipError_t half_step(sph_sim* s, int from, int to, bool predictor, bool timing) {
+  const sph_params& P = s->P;
+  const int n = s->n;
+  FastSet& in = s->set[from];
+  FastSet& out = s->set[to];
+  FastWork w = s->w;
+  hipError_t e;
+  e = timed(s, timing, "hash", [&] { return launch_hash_fast(P, in, w.keys, w.idx, n, s->stream); });
+  if (e != hipSuccess) return e;
+  bool alt = false;
+  e = timed(s, timing, "radix_sort", [&] {
+    return radix_sort_pairs(w.keys, w.idx, w.keys_alt, w.idx_alt, w.hist, w.totals, n, s->bits,
+                            s->stream, &alt);
+  });
+  if (e != hipSuccess) return e;
+  if (alt) {
+    w.keys = s->w.keys_alt;
+    w.idx = s->w.idx_alt;
+    w.keys_alt = s->w.keys;
+    w.idx_alt = s->w.idx;
+  }
+  e = timed(s, timing, "cells_gather", [&] {
+    return launch_cells_gather_fast(P, in, out, w.keys, w.idx, w.cellStart, w.cellEnd, n, s->stream);
+  });
+  if (e != hipSuccess) return e;
+  e = timed(s, timing, "nsA", [&] { return launch_nsA_fast(P, out, w, n, s->tun.nbr_block, s->stream); });
+  if (e != hipSuccess) return e;
+  e = timed(s, timing, "nsB", [&] { return launch_nsB_fast(P, out, w, n, s->tun.nbr_block, s->stream); });
+  if (e != hipSuccess) return e;
+  e = timed(s, timing, predictor ? "predictor" : "corrector",
+            [&] { return launch_pc_fast(P, predictor, out, w, w.idx, n, s->stream); });
+  return e;
+}
+
+hipError_t full_step(sph_sim* s, bool timing) {
+  hipError_t e = half_step(s, 0, 1, true, timing);
+  if (e != hipSuccess) return e;
+  return half_step(s, 1, 0, false, timing);
+}
+
+void release(sph_sim* s) {
+  if (s->exec) hipGraphExecDestroy(s->exec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
+  for (void* p : s->allocs) hipFree(p);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+}
+
+}  // namespace
+
+extern "C" int sph_sim_create(const sph_params* p, int n, const float* pos, const float* vel,
+                              const float* rho, const float* prs, const int* type, sph_sim** out) {
+  if (!out) return set_error(SPH_E_INVAL, "sph_sim_create: null output");
+  *out = nullptr;
+  if (!p || n <= 0 || !pos || !vel || !rho || !prs || !type)
+    return set_error(SPH_E_INVAL, "sph_sim_create: bad argument");
+  {
+    sph_params tmp = *p;
+    int rc = sph_set_params(&tmp);  // validates; also makes the reference surface usable
+    if (rc) return rc;
+  }
+  sph_sim* s = new sph_sim();
+  s->P = *p;
+  s->P.numParticles = n;
+  s->n = n;
+  s->bits = key_bits(p->numCells);
+  s->tun.nbr_block = 256;
+  s->tun.nbr_variant = 0;
+  int rc = SPH_OK;
+  hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    rc = hip_status(he, "hipStreamCreate");
+    release(s);
+    return rc;
+  }
+  const size_t N = (size_t)n;
+  const bool d3 = p->dim == 3;
+  for (int k = 0; k < 2 && !rc; ++k) {
+    FastSet& f = s->set[k];
+    rc = rc ? rc : dalloc(s, &f.PR, N);
+    rc = rc ? rc : dalloc(s, &f.VP, N);
+    rc = rc ? rc : dalloc(s, &f.X[0], N);
+    if (d3) rc = rc ? rc : dalloc(s, &f.X[1], N);
+    rc = rc ? rc : dalloc(s, &f.X[2], N);
+    rc = rc ? rc : dalloc(s, &f.ID, N);
+    rc = rc ? rc : dalloc(s, &f.TY, N);
+  }
+  FastWork& w = s->w;
+  rc = rc ? rc : dalloc(s, &w.keys, N);
+  rc = rc ? rc : dalloc(s, &w.idx, N);
+  rc = rc ? rc : dalloc(s, &w.keys_alt, N);
+  rc = rc ? rc : dalloc(s, &w.idx_alt, N);
+  rc = rc ? rc : dalloc(s, &w.hist, (size_t)radix_hist_words(n));
+  rc = rc ? rc : dalloc(s, &w.totals, 256);
+  rc = rc ? rc : dalloc(s, &w.cellStart, (size_t)p->numCells);
+  rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)p->numCells);
+  rc = rc ? rc : dalloc(s, &w.DV, N);
+  rc = rc ? rc : dalloc(s, &w.RATE, N);
+  rc = rc ? rc : dalloc(s, &w.TX[0], N);
+  if (d3) rc = rc ? rc : dalloc(s, &w.TX[1], N);
+  rc = rc ? rc : dalloc(s, &w.TX[2], N);
+  rc = rc ? rc : dalloc(s, &w.TV, N);
+  rc = rc ? rc : dalloc(s, &s->probe, 4);
+  if (rc) {
+    release(s);
+    return rc;
+  }
+  // stage the host arrays through device scratch (the sort buffers), widen on device
+  float* dpos = nullptr;
+  float* dvel = nullptr;
+  float* drho = nullptr;
+  float* dprs = nullptr;
+  int* dtype = nullptr;
+  he = hipMalloc(&dpos, N * 3 * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&dvel, N * 3 * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&drho, N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&dprs, N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&dtype, N * sizeof(int));
+  if (he == hipSuccess) he = hipMemcpy(dpos, pos, N * 3 * sizeof(float), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dvel, vel, N * 3 * sizeof(float), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(drho, rho, N * sizeof(float), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dprs, prs, N * sizeof(float), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dtype, type, N * sizeof(int), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = launch_upload_fast(s->P, s->set[0], dpos, dvel, drho, dprs, dtype, n, s->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
+  hipFree(dpos);
+  hipFree(dvel);
+  hipFree(drho);
+  hipFree(dprs);
+  hipFree(dtype);
+  if (he != hipSuccess) {
+    rc = hip_status(he, "sph_sim_create: upload");
+    release(s);
+    return rc;
+  }
+  *out = s;
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_destroy(sph_sim* s) {
+  if (!s) return SPH_OK;
+  hipStreamSynchronize(s->stream);
+  release(s);
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_get_tuning(sph_sim* s, sph_tuning* t) {
+  if (!s || !t) return set_error(SPH_E_INVAL, "null argument");
+  *t = s->tun;
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
+  if (!s || !t) return set_error(SPH_E_INVAL, "null argument");
+  if (t->nbr_block < 64 || t->nbr_block > 256 || (t->nbr_block % 64) != 0)
+    return set_error(SPH_E_INVAL, "nbr_block must be 64, 128, 192 or 256");
+  s->tun = *t;
+  if (s->exec) {  // a captured graph holds the old launch shapes
+    hipGraphExecDestroy(s->exec);
+    hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_step(sph_sim* s, int nsteps) {
+  if (!s || nsteps < 0) return set_error(SPH_E_INVAL, "sph_sim_step: bad argument");
+  const bool timing = (s->tun.flags & 1) != 0;
+  s->marks.clear();
+  s->event_next = 0;
+  for (int k = 0; k < nsteps; ++k) {
+    SPH_HIP(full_step(s, timing), "sph_sim_step");
+    ++s->steps;
+    if (timing) {
+      SPH_HIP(hipStreamSynchronize(s->stream), "sph_sim_step sync");
+      for (auto& m : s->marks) {
+        float ms = 0.0f;
+        SPH_HIP(hipEventElapsedTime(&ms, m.a, m.b), "hipEventElapsedTime");
+        s->tsum_ms[m.slot] += ms;
+        s->tcount[m.slot] += 1;
+      }
+      s->marks.clear();
+      s->event_next = 0;
+    }
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_capture(sph_sim* s) {
+  if (!s) return set_error(SPH_E_INVAL, "null sim");
+  if (s->exec) {
+    hipGraphExecDestroy(s->exec);
+    hipGraphDestroy(s->graph);
+    s->exec = nullptr;
+    s->graph = nullptr;
+  }
+  SPH_HIP(hipStreamSynchronize(s->stream), "capture sync");
+  SPH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  hipError_t e = full_step(s, false);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+  if (e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return hip_status(e, "capture step");
+  }
+  SPH_HIP(e2, "hipStreamEndCapture");
+  s->graph = g;
+  SPH_HIP(hipGraphInstantiate(&s->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_replay(sph_sim* s, int nsteps) {
+  if (!s || nsteps < 0) return set_error(SPH_E_INVAL, "sph_sim_replay: bad argument");
+  if (!s->exec) return set_error(SPH_E_STATE, "sph_sim_replay: no captured step");
+  for (int k = 0; k < nsteps; ++k) {
+    SPH_HIP(hipGraphLaunch(s->exec, s->stream), "hipGraphLaunch");
+    ++s->steps;
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_synchronize(sph_sim* s) {
+  if (!s) return set_error(SPH_E_INVAL, "null sim");
+  SPH_HIP(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_download(sph_sim* s, float* pos, float* vel, float* rho, float* p, int* type) {
+  if (!s) return set_error(SPH_E_INVAL, "null sim");
+  const size_t N = (size_t)s->n;
+  float *dpos = nullptr, *dvel = nullptr, *drho = nullptr, *dp = nullptr;
+  int* dtype = nullptr;
+  hipError_t e = hipSuccess;
+  if (pos) e = hipMalloc(&dpos, N * 3 * sizeof(float));
+  if (e == hipSuccess && vel) e = hipMalloc(&dvel, N * 3 * sizeof(float));
+  if (e == hipSuccess && rho) e = hipMalloc(&drho, N * sizeof(float));
+  if (e == hipSuccess && p) e = hipMalloc(&dp, N * sizeof(float));
+  if (e == hipSuccess && type) e = hipMalloc(&dtype, N * sizeof(int));
+  if (e == hipSuccess) e = launch_download_fast(s->set[0], dpos, dvel, drho, dp, dtype, nullptr, s->n, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess && pos) e = hipMemcpy(pos, dpos, N * 3 * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && vel) e = hipMemcpy(vel, dvel, N * 3 * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && rho) e = hipMemcpy(rho, drho, N * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && p) e = hipMemcpy(p, dp, N * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && type) e = hipMemcpy(type, dtype, N * sizeof(int), hipMemcpyDeviceToHost);
+  hipFree(dpos);
+  hipFree(dvel);
+  hipFree(drho);
+  hipFree(dp);
+  hipFree(dtype);
+  SPH_HIP(e, "sph_sim_download");
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_download_pos64(sph_sim* s, double* pos) {
+  if (!s || !pos) return set_error(SPH_E_INVAL, "null argument");
+  const size_t N = (size_t)s->n;
+  double* d = nullptr;
+  hipError_t e = hipMalloc(&d, N * 3 * sizeof(double));
+  if (e == hipSuccess) e = launch_download_fast(s->set[0], nullptr, nullptr, nullptr, nullptr, nullptr, d, s->n, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpy(pos, d, N * 3 * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(d);
+  SPH_HIP(e, "sph_sim_download_pos64");
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_probe(sph_sim* s, double band, double* front, double* height) {
+  if (!s || !front || !height) return set_error(SPH_E_INVAL, "null argument");
+  SPH_HIP(hipMemsetAsync(s->probe, 0, 2 * sizeof(unsigned long long), s->stream), "probe clear");
+  SPH_HIP(launch_probe_fast(s->P, s->set[0], band, s->probe, s->n, s->stream), "probe");
+  unsigned long long hv[2];
+  SPH_HIP(hipMemcpyAsync(hv, s->probe, sizeof(hv), hipMemcpyDeviceToHost, s->stream), "probe copy");
+  SPH_HIP(hipStreamSynchronize(s->stream), "probe sync");
+  double f, h;
+  std::memcpy(&f, &hv[0], 8);
+  std::memcpy(&h, &hv[1], 8);
+  *front = f;
+  *height = h;
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_kernel_times(sph_sim* s, char* names, int names_cap, double* us, int* count,
+                                    int cap) {
+  if (!s || !count) return set_error(SPH_E_INVAL, "null argument");
+  std::string all;
+  int k = 0;
+  for (size_t i = 0; i < s->tnames.size(); ++i) {
+    if (i) all += ";";
+    all += s->tnames[i];
+    if (us && k < cap) us[k++] = s->tcount[i] ? 1000.0 * s->tsum_ms[i] / (double)s->tcount[i] : 0.0;
+  }
+  *count = (int)s->tnames.size();
+  if (names && names_cap > 0) {
+    std::strncpy(names, all.c_str(), (size_t)names_cap - 1);
+    names[names_cap - 1] = '\0';
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, float* p, float* rho,
+                            int* keys, int* perm) {
+  if (!s) return set_error(SPH_E_INVAL, "null sim");
+  const sph_params& P = s->P;
+  const int n = s->n;
+  const size_t N = (size_t)n;
+  FastWork w = s->w;
+  SPH_HIP(launch_hash_fast(P, s->set[0], w.keys, w.idx, n, s->stream), "eval hash");
+  bool alt = false;
+  SPH_HIP(radix_sort_pairs(w.keys, w.idx, w.keys_alt, w.idx_alt, w.hist, w.totals, n, s->bits, s->stream, &alt),
+          "eval sort");
+  if (alt) {
+    w.keys = s->w.keys_alt;
+    w.idx = s->w.idx_alt;
+  }
+  SPH_HIP(launch_cells_gather_fast(P, s->set[0], s->set[1], w.keys, w.idx, w.cellStart, w.cellEnd, n, s->stream),
+          "eval gather");
+  SPH_HIP(launch_nsA_fast(P, s->set[1], w, n, s->tun.nbr_block, s->stream), "eval nsA");
+  SPH_HIP(launch_nsB_fast(P, s->set[1], w, n, s->tun.nbr_block, s->stream), "eval nsB");
+  float *ddr = nullptr, *dacc = nullptr, *ddv = nullptr, *dp = nullptr, *drh = nullptr;
+  hipError_t e = hipMalloc(&ddr, N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dacc, 3 * N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&ddv, N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dp, N * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&drh, N * sizeof(float));
+  if (e == hipSuccess) e = launch_download_rates(s->set[1], w, ddr, dacc, ddv, n, s->stream);
+  if (e == hipSuccess)
+    e = launch_download_fast(s->set[1], nullptr, nullptr, drh, dp, nullptr, nullptr, n, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess && drho) e = hipMemcpy(drho, ddr, N * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && acc) e = hipMemcpy(acc, dacc, 3 * N * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && dofv) e = hipMemcpy(dofv, ddv, N * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && p) e = hipMemcpy(p, dp, N * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && rho) e = hipMemcpy(rho, drh, N * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && keys) e = hipMemcpy(keys, w.keys, N * sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && perm) e = hipMemcpy(perm, w.idx, N * sizeof(int), hipMemcpyDeviceToHost);
+  hipFree(ddr);
+  hipFree(dacc);
+  hipFree(ddv);
+  hipFree(dp);
+  hipFree(drh);
+  SPH_HIP(e, "sph_sim_eval");
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA, long long* candB,
+                                  long long* hitsB) {
+  if (!s) return set_error(SPH_E_INVAL, "null sim");
+  const sph_params& P = s->P;
+  const int n = s->n;
+  FastWork w = s->w;
+  SPH_HIP(launch_hash_fast(P, s->set[0], w.keys, w.idx, n, s->stream), "stats hash");
+  bool alt = false;
+  SPH_HIP(radix_sort_pairs(w.keys, w.idx, w.keys_alt, w.idx_alt, w.hist, w.totals, n, s->bits, s->stream, &alt),
+          "stats sort");
+  if (alt) {
+    w.keys = s->w.keys_alt;
+    w.idx = s->w.idx_alt;
+  }
+  SPH_HIP(launch_cells_gather_fast(P, s->set[0], s->set[1], w.keys, w.idx, w.cellStart, w.cellEnd, n, s->stream),
+          "stats gather");
+  SPH_HIP(hipMemsetAsync(s->probe, 0, 4 * sizeof(unsigned long long), s->stream), "stats clear");
+  SPH_HIP(launch_pair_stats(P, s->set[1], w, s->probe, n, s->stream), "stats");
+  unsigned long long hv[4];
+  SPH_HIP(hipMemcpyAsync(hv, s->probe, sizeof(hv), hipMemcpyDeviceToHost, s->stream), "stats copy");
+  SPH_HIP(hipStreamSynchronize(s->stream), "stats sync");
+  if (candA) *candA = (long long)hv[0];
+  if (hitsA) *hitsA = (long long)hv[1];
+  if (candB) *candB = (long long)hv[2];
+  if (hitsB) *hitsB = (long long)hv[3];
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_steps_done(sph_sim* s, long long* steps) {
+  if (!s || !steps) return set_error(SPH_E_INVAL, "null argument");
+  *steps = s->steps;
+  return SPH_OK;
+}

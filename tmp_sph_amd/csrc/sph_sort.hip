@@ -1,0 +1,204 @@
+// Stable LSD radix sort of (cell hash, particle index) pairs for gfx950.
+//
+// Replaces thrust::sort_by_key in sortParticles (src/lib/mesh.cu:73-86),
+// which for int keys is a stable LSD radix sort.  Each 8-bit digit pass is
+// three launches with no inter-block waiting:
+//   upsweep   per-tile digit counts, written digit-major (hist[d][tile])
+//   scan      one block per digit turns its row into exclusive tile offsets
+//   downsweep per-tile stable ranks from wave64 ballots, scatter to
+//             digit base + tile offset + rank
+// Stability: a wave handles a contiguous run of its tile in 64-element
+// rounds; lanes with equal digits are ranked by lane order (ballot match),
+// rounds in order, waves in order, tiles in order.
+#include "sph_kernels.h"
+
+namespace sph {
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_IPT = 16;
+constexpr int RS_TILE = RS_THREADS * RS_IPT;
+
+static_assert(RS_THREADS == 256, "one thread per digit in the per-digit phases");
+
+// Lanes of the wave whose digit equals this lane's digit (among `active`).
+__device__ __forceinline__ unsigned long long match_digit(unsigned d, int nbits,
+                                                          unsigned long long active) {
+  unsigned long long m = active;
+  for (int b = 0; b < nbits; ++b) {
+    const int bit = (d >> b) & 1;
+    const unsigned long long bal = __ballot(bit);
+    m &= bit ? bal : ~bal;
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const int* __restrict__ keys, int n,
+                                                         int shift, int nbits,
+                                                         unsigned* __restrict__ hist,
+                                                         int numTiles) {
+  __shared__ unsigned cnt[256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  cnt[tid] = 0;
+  __syncthreads();
+  const unsigned mask = (1u << nbits) - 1u;
+  const long long base = (long long)blockIdx.x * RS_TILE + (long long)w * (64 * RS_IPT);
+  for (int r = 0; r < RS_IPT; ++r) {
+    const long long i = base + r * 64 + lane;
+    const bool valid = i < n;
+    const unsigned d = valid ? (((unsigned)keys[i]) >> shift) & mask : 0u;
+    const unsigned long long active = __ballot(valid);
+    if (active == 0ull) break;  // wave-uniform
+    const unsigned long long peers = match_digit(d, nbits, active);
+    const int leader = __ffsll((long long)peers) - 1;
+    if (valid && lane == leader) atomicAdd(&cnt[d], (unsigned)__popcll(peers));
+  }
+  __syncthreads();
+  hist[(size_t)tid * numTiles + blockIdx.x] = cnt[tid];
+}
+
+__global__ __launch_bounds__(256) void rs_scan_rows(unsigned* __restrict__ hist, int numTiles,
+                                                    unsigned* __restrict__ totals) {
+  __shared__ unsigned s[256];
+  unsigned* row = hist + (size_t)blockIdx.x * numTiles;
+  const int per = (numTiles + 255) / 256;
+  const int b = threadIdx.x * per;
+  const int e = min(b + per, numTiles);
+  unsigned sum = 0;
+  for (int t = b; t < e; ++t) sum += row[t];
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    unsigned v = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned run = threadIdx.x ? s[threadIdx.x - 1] : 0u;
+  for (int t = b; t < e; ++t) {
+    unsigned c = row[t];
+    row[t] = run;
+    run += c;
+  }
+  if (threadIdx.x == 255) totals[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
+    const int* __restrict__ keys_in, const int* __restrict__ vals_in, int* __restrict__ keys_out,
+    int* __restrict__ vals_out, int n, int shift, int nbits, const unsigned* __restrict__ hist,
+    const unsigned* __restrict__ totals, int numTiles) {
+  __shared__ unsigned wcnt[RS_WAVES][256];
+  __shared__ unsigned dbase[256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int k = tid; k < RS_WAVES * 256; k += RS_THREADS) (&wcnt[0][0])[k] = 0u;
+  // exclusive scan of the digit totals -> first output slot of each digit
+  const unsigned own = totals[tid];
+  dbase[tid] = own;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    unsigned v = tid >= off ? dbase[tid - off] : 0u;
+    __syncthreads();
+    dbase[tid] += v;
+    __syncthreads();
+  }
+  const unsigned myBase = dbase[tid] - own + hist[(size_t)tid * numTiles + blockIdx.x];
+  __syncthreads();
+  dbase[tid] = myBase;
+
+  const unsigned mask = (1u << nbits) - 1u;
+  const unsigned long long ltmask = (1ull << lane) - 1ull;
+  const long long base = (long long)blockIdx.x * RS_TILE + (long long)w * (64 * RS_IPT);
+  int kk[RS_IPT], vv[RS_IPT];
+  unsigned rk[RS_IPT];
+#pragma unroll
+  for (int r = 0; r < RS_IPT; ++r) {
+    const long long i = base + r * 64 + lane;
+    const bool valid = i < n;
+    const int key = valid ? keys_in[i] : 0;
+    const int val = valid ? vals_in[i] : 0;
+    const unsigned d = (((unsigned)key) >> shift) & mask;
+    const unsigned long long active = __ballot(valid);
+    const unsigned long long peers = match_digit(d, nbits, active);
+    const int leader = __ffsll((long long)peers) - 1;
+    const unsigned before = wcnt[w][d];
+    rk[r] = before + (unsigned)__popcll(peers & ltmask);
+    if (valid && lane == leader) wcnt[w][d] = before + (unsigned)__popcll(peers);
+    kk[r] = key;
+    vv[r] = val;
+  }
+  __syncthreads();
+  {
+    // per digit: tile offset of each wave's run
+    unsigned run = dbase[tid];
+    for (int ww = 0; ww < RS_WAVES; ++ww) {
+      unsigned c = wcnt[ww][tid];
+      wcnt[ww][tid] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_IPT; ++r) {
+    const long long i = base + r * 64 + lane;
+    if (i < n) {
+      const unsigned d = (((unsigned)kk[r]) >> shift) & mask;
+      const unsigned dst = wcnt[w][d] + rk[r];
+      keys_out[dst] = kk[r];
+      vals_out[dst] = vv[r];
+    }
+  }
+}
+
+__global__ void fill_int_kernel(int* __restrict__ p, int value, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = value;
+}
+
+}  // namespace
+
+int radix_hist_words(int n) {
+  int tiles = (n + RS_TILE - 1) / RS_TILE;
+  if (tiles < 1) tiles = 1;
+  return 256 * tiles;
+}
+
+hipError_t fill_int(int* p, int value, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, value, n);
+  return hipGetLastError();
+}
+
+hipError_t radix_sort_pairs(int* keys, int* vals, int* keys_alt, int* vals_alt, unsigned* hist,
+                            unsigned* totals, int n, int bits, hipStream_t s,
+                            bool* result_in_alt) {
+  *result_in_alt = false;
+  if (n <= 1) return hipSuccess;
+  const int numTiles = (n + RS_TILE - 1) / RS_TILE;
+  int* kin = keys;
+  int* vin = vals;
+  int* kout = keys_alt;
+  int* vout = vals_alt;
+  bool in_alt = false;
+  for (int shift = 0; shift < bits; shift += 8) {
+    const int nb = bits - shift < 8 ? bits - shift : 8;
+    hipLaunchKernelGGL(rs_upsweep, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, n, shift, nb,
+                       hist, numTiles);
+    hipLaunchKernelGGL(rs_scan_rows, dim3(256), dim3(256), 0, s, hist, numTiles, totals);
+    hipLaunchKernelGGL(rs_downsweep, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, vin, kout, vout,
+                       n, shift, nb, hist, totals, numTiles);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int* t = kin; kin = kout; kout = t;
+    t = vin; vin = vout; vout = t;
+    in_alt = !in_alt;
+  }
+  *result_in_alt = in_alt;
+  return hipSuccess;
+}
+
+}  // namespace sph
