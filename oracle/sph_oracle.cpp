@@ -194,7 +194,12 @@ void kernelB(const sph_params& P, const R* sPos, const R* sVel, const R* sRho, c
         const R t1 = dens0 * dvdx * fr;
         const R t2 = (dens0 - dens1) * rr * fr * delta * h * cs / (rr + eta * eta);
         dd += (t1 + t2) * m / dens1;
-        md += (std::fabs((double)t1) + std::fabs((double)t2)) * (double)m / (double)dens1;
+        // the diffusion term takes the difference of two FP32 densities near
+        // rho0, so a one-ulp change of either input (e.g. a wall density from
+        // kernel A) moves it by |d t2 / d rho| * ulp: part of the scale
+        const double ulp_rho = 2.0 * std::ldexp(std::max(std::fabs((double)dens0), std::fabs((double)dens1)), -23);
+        const double t2_sens = std::fabs((double)rr * fr * delta * h * cs / (rr + eta * eta)) * ulp_rho;
+        md += (std::fabs((double)t1) + std::fabs((double)t2) + t2_sens) * (double)m / (double)dens1;
         R vtmp = -(pres0 + pres1) * m * fr / (dens0 * dens1);
         double vmag = std::fabs((double)vtmp);
         if (ti == 1 && sType[j] == 1) {

@@ -61,5 +61,8 @@ def rel_err(got, want, scale):
     """max |got - want| / max(|want|, scale), elementwise scale allowed."""
     got = np.asarray(got, np.float64)
     want = np.asarray(want, np.float64)
-    den = np.maximum(np.abs(want), scale)
-    return float(np.max(np.abs(got - want) / den)) if got.size else 0.0
+    if not got.size:
+        return 0.0
+    diff = np.abs(got - want)
+    den = np.maximum(np.maximum(np.abs(want), scale), 1e-300)
+    return float(np.max(np.where(diff == 0, 0.0, diff / den)))

@@ -65,6 +65,21 @@ struct RowRange {
   int s, e;
 };
 
+// Candidates loaded per batch in the neighbour loops (independent loads in flight).
+constexpr int UNROLL = 4;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2.  Neighbouring target tiles share most of
+// their candidates, so remap the hardware block id so that every XCD works on
+// one contiguous run of tiles instead of every eighth tile.
+__device__ __forceinline__ int xcd_tile() {
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+
 // Sorted index ranges of the stencil rows of cell `hash`, in the reference
 // order (z outer, y inner; the x-adjacent cells of a row are contiguous).
 template <int DIM>
@@ -115,7 +130,7 @@ __global__ __launch_bounds__(256) void nsA_fast_kernel(sph_params P, float4* PR,
                                                        const int* __restrict__ keys,
                                                        const int* __restrict__ cellStart,
                                                        const int* __restrict__ cellEnd, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 pi = PR[i];
   const float4 vi = VP[i];
@@ -128,14 +143,20 @@ __global__ __launch_bounds__(256) void nsA_fast_kernel(sph_params P, float4* PR,
   const float m = P.particleMass;
   float sp = 0.0f, sw = 0.0f, dv = 0.0f;
   for (int r = 0; r < nr; ++r) {
-    const int e = rows[r].e;
-    for (int j = rows[r].s; j < e; ++j) {
-      const float4 pj = PR[j];
+    const int s = rows[r].s, e = rows[r].e;
+    for (int j0 = s; j0 < e; j0 += UNROLL) {
+    float4 pjs[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) pjs[u] = PR[min(j0 + u, e - 1)];  // loads issued together
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      const int j = j0 + u;
+      const float4 pj = pjs[u];
       const float dx = pi.x - pj.x;
       const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
       const float dz = pi.z - pj.z;
       const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
-      if (rr <= P.rr_cut && j != i && pj.w > 0.0f) {
+      if (j < e && rr <= P.rr_cut && j != i && pj.w > 0.0f) {
         const float q = fast_sqrt(rr) * inv_h;
         const float t = 1.0f - 0.5f * q;
         const float t3 = t * t * t;
@@ -150,6 +171,7 @@ __global__ __launch_bounds__(256) void nsA_fast_kernel(sph_params P, float4* PR,
           dv -= cfr * t3 * dvdx * m * fast_rcp(pj.w);
         }
       }
+    }
     }
   }
   DV[i] = dv;
@@ -172,7 +194,7 @@ __global__ __launch_bounds__(256) void nsB_fast_kernel(sph_params P, const float
                                                        const int* __restrict__ cellStart,
                                                        const int* __restrict__ cellEnd,
                                                        float4* __restrict__ RATE, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 pi = PR[i];
   if (!(pi.w > 0.0f)) {
@@ -194,14 +216,20 @@ __global__ __launch_bounds__(256) void nsB_fast_kernel(sph_params P, const float
   const float pm_i = m * inv_rho_i;
   float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
   for (int r = 0; r < nr; ++r) {
-    const int e = rows[r].e;
-    for (int j = rows[r].s; j < e; ++j) {
-      const float4 pj = PR[j];
+    const int s = rows[r].s, e = rows[r].e;
+    for (int j0 = s; j0 < e; j0 += UNROLL) {
+    float4 pjs[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) pjs[u] = PR[min(j0 + u, e - 1)];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      const int j = j0 + u;
+      const float4 pj = pjs[u];
       const float dx = pi.x - pj.x;
       const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
       const float dz = pi.z - pj.z;
       const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
-      if (rr <= P.rr_cut && j != i) {
+      if (j < e && rr <= P.rr_cut && j != i) {
         const float4 vj = VP[j];
         const bool fj = pj.w > 0.0f;
         const float rho_j = fabsf(pj.w);
@@ -221,6 +249,7 @@ __global__ __launch_bounds__(256) void nsB_fast_kernel(sph_params P, const float
         if (DIM == 3) ay += vt * dy;
         az += vt * dz;
       }
+    }
     }
   }
   RATE[i] = make_float4(drho, ax, ay, az + P.gravity);

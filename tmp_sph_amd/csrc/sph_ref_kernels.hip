@@ -98,6 +98,8 @@ __global__ void nsA_aos_kernel(sph_params P, const float* __restrict__ sPos,
                                const int* __restrict__ sType, const int* __restrict__ cellStart,
                                const int* __restrict__ cellEnd, const int* __restrict__ hash,
                                float* __restrict__ dofv, int n) {
+  // no FMA contraction: every operation rounds once, in the reference's order
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float px = sPos[3 * i], py = sPos[3 * i + 1], pz = sPos[3 * i + 2];
@@ -145,6 +147,7 @@ __global__ void nsB_aos_kernel(sph_params P, const float* __restrict__ sPos,
                                const float* __restrict__ dofv, const int* __restrict__ cellStart,
                                const int* __restrict__ cellEnd, const int* __restrict__ hash,
                                float* __restrict__ drho, float* __restrict__ acc, int n) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float px = sPos[3 * i], py = sPos[3 * i + 1], pz = sPos[3 * i + 2];
