@@ -27,16 +27,15 @@ HBM_PEAK_GBS = 8000.0
 FP32_VALU_PEAK_TFLOPS = 157.3
 
 
-def _configs():
-    from tmp_sph_amd import sph
-    return {
-        "c3": ("3-D dam break, 1.0x0.5x0.5 m column in 3.2x1.0x1.0 m tank, dx=0.005 m",
-               sph.dambreak_cfg(dim=3, dx=0.005, tank=(3.2, 1.0, 1.0), column=(1.0, 0.5, 0.5))),
-        "c2": ("2-D dam break, 1.0x0.5 m column in 4.0x1.0 m tank, dx=0.001 m",
-               sph.dambreak_cfg(dim=2, dx=0.001)),
-        "c1": ("2-D dam break, 1.0x0.5 m column in 4.0x1.0 m tank, dx=0.01 m",
-               sph.dambreak_cfg(dim=2, dx=0.01)),
-    }
+def _workload(name: str):
+    """(description, list of cases)."""
+    from tmp_sph_amd import cases, sph
+    if name == "c4":
+        cs = cases.sweep_cases()
+        return (f"batched sweep: {len(cs)} uneven 2-D dam breaks (H~U[0.2,0.6] m, W/H~U[0.5,2], "
+                f"dx in {{0.002,0.004}} m, 4.0x1.0 m tank) in one graph per step", cs)
+    desc, cfg = cases.named(name)
+    return desc, [sph.make_case(cfg)]
 
 
 def _algorithmic_bytes(name: str, dim: int, n: int, nf: int) -> float:
@@ -71,7 +70,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=10, help="eager steps with per-kernel HIP events")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -83,15 +83,21 @@ def main() -> int:
         return 2
 
     from tmp_sph_amd import sph
-    desc, cfg = _configs()[args.config]
     t0 = time.time()
-    case = sph.make_case(cfg)
+    desc, cases = _workload(args.config)
+    case = cases[0]
     prm = case.params
-    print(f"# case {args.config}: N={case.n} fluid={case.n_fluid} dt={prm.timeStep:.4g} "
+    n_total = sum(c.n for c in cases)
+    n_fluid = sum(c.n_fluid for c in cases)
+    print(f"# workload {args.config}: {len(cases)} case(s), N={n_total} fluid={n_fluid} "
           f"(generated in {time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
-    sim = sph.Simulation(case)
+    sim = sph.Simulation(cases if len(cases) > 1 else case)
+    tuned = None
     if args.nbr_block:
         sim.set_tuning(nbr_block=args.nbr_block)
+    elif not args.no_autotune:
+        tuned, tuned_us = sim.autotune(reps=3)
+        print(f"# autotune: {tuned} ({tuned_us:.1f} us for kernels A+B)", file=sys.stderr, flush=True)
     sim.capture()
     sim.replay(args.warmup)
     sim.synchronize()
@@ -100,7 +106,7 @@ def main() -> int:
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
-    value = case.n_fluid * args.steps / elapsed
+    value = n_fluid * args.steps / elapsed
     print(f"# timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
 
     # per-kernel device time (HIP events on the simulation stream, eager steps)
@@ -113,7 +119,7 @@ def main() -> int:
     sim.close()
     dom = max(("nsA", "nsB"), key=lambda k: kt.get(k, 0.0))
     us = kt[dom]
-    byts = _algorithmic_bytes(dom, prm.dim, case.n, case.n_fluid)
+    byts = _algorithmic_bytes(dom, prm.dim, n_total, n_fluid)
     achieved = byts / (us * 1e-6) / 1e9
     # counted flops: 9 per candidate + 55 per in-support pair (kernel B), 9 + 25 (kernel A)
     flops = (9.0 * stats["candB"] + 55.0 * stats["hitsB"]) if dom == "nsB" else \
@@ -136,9 +142,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32 (f64 positions)",
         "data": "synthetic dam-break lattice at rest, hydrostatic pressure",
-        "config": {"workload": desc, "particles_total": case.n, "particles_fluid": case.n_fluid,
-                   "dt": prm.timeStep, "kernel_h_over_dx": 1.3, "eos": "tait7" if prm.eos == 1 else "linear",
-                   "parallelism": "single GPU"},
+        "config": {"workload": desc, "instances": len(cases), "particles_total": n_total,
+                   "particles_fluid": n_fluid, "dt": prm.timeStep if len(cases) == 1 else "per instance",
+                   "kernel_h_over_dx": 1.3, "eos": "tait7" if prm.eos == 1 else "linear",
+                   "parallelism": "single GPU", "tuning": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": us,
                      "algorithmic_bytes_per_launch": byts,
@@ -148,7 +155,7 @@ def main() -> int:
         "pair_stats": stats,
         "probe_after_run": {"front_m": front, "column_height_m": height},
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and len(cases) == 1:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)
     return 0

@@ -184,18 +184,36 @@ int sph_PC_correction(float* sortedPos, float* sortedVel, float* sortedDensity,
 /* ------------------------------------------------ simulation context (fast path) */
 typedef struct sph_sim sph_sim;
 
-/* Launch-shape knobs of the fast path, filled with defaults by sph_sim_create. */
+/* Launch-shape knobs of the fast path, filled with defaults by sph_sim_create
+   and chosen per workload by sph_sim_autotune.  None changes the results. */
 typedef struct sph_tuning {
-    int nbr_block;       /* threads per block of the neighbour-loop kernels */
-    int nbr_variant;     /* neighbour-loop kernel variant (0 = per-target rows) */
-    int sort_ipt;        /* reserved */
-    int flags;           /* reserved */
+    int nbr_block;       /* threads per block of the neighbour-loop kernels: 64, 128, 192, 256 */
+    int nbr_unroll;      /* candidate loads issued together: 1, 2, 4 or 8 */
+    int xcd_remap;       /* 1: each XCD gets a contiguous run of target tiles */
+    int flags;           /* bit 0: per-kernel HIP event timing in sph_sim_step */
 } sph_tuning;
 
 /* Creates a context on the current device from host arrays in the layout of
    sph_case_dambreak (AoS float[3N] pos/vel).  Positions are widened to FP64. */
 int sph_sim_create(const sph_params* p, int n, const float* pos, const float* vel,
                    const float* rho, const float* prs, const int* type, sph_sim** out);
+/* Batched mode (no reference counterpart): K independent instances of equal
+   dim -- e.g. dam breaks of different sizes, resolutions and time steps --
+   stepped together, one launch per phase for all of them and one hipGraph per
+   step.  params[k] and counts[k] describe instance k; the host arrays hold
+   the instances back to back.  Every instance's results are bit-identical to
+   running it alone in its own context.  Types must lie in 0..255. */
+int sph_sim_create_batch(const sph_params* params, int K, const int* counts, const float* pos,
+                         const float* vel, const float* rho, const float* prs, const int* type,
+                         sph_sim** out);
+int sph_sim_num_instances(sph_sim* s, int* K);
+/* Per-instance surge front and column height (arrays of K doubles). */
+int sph_sim_probe_batch(sph_sim* s, double band, double* front, double* height);
+/* Times the neighbour kernels of the current state over the candidate launch
+   shapes (block size, unroll, XCD remap), `reps` evaluations each, and keeps
+   the fastest; the state is not advanced.  *best receives the choice and
+   *best_us its kernel A + B time in microseconds (either may be null). */
+int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* best_us);
 int sph_sim_destroy(sph_sim* s);
 int sph_sim_get_tuning(sph_sim* s, sph_tuning* t);
 int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t);

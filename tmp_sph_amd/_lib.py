@@ -96,7 +96,10 @@ class DamBreakCfg(C.Structure):
 
 
 class SphTuning(C.Structure):
-    _fields_ = [("nbr_block", C.c_int), ("nbr_variant", C.c_int), ("sort_ipt", C.c_int), ("flags", C.c_int)]
+    _fields_ = [("nbr_block", C.c_int), ("nbr_unroll", C.c_int), ("xcd_remap", C.c_int), ("flags", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 P = C.c_void_p
@@ -145,6 +148,10 @@ _PROTOS = {
     "sph_sim_steps_done": [P, C.POINTER(C.c_longlong)],
     "sph_sim_eval": [P, P, P, P, P, P, P, P],
     "sph_sim_pair_stats": [P] + [C.POINTER(C.c_longlong)] * 4,
+    "sph_sim_create_batch": [C.POINTER(SphParams), I, IP, P, P, P, P, P, C.POINTER(C.c_void_p)],
+    "sph_sim_num_instances": [P, IP],
+    "sph_sim_probe_batch": [P, C.c_double, DP, DP],
+    "sph_sim_autotune": [P, I, C.POINTER(SphTuning), DP],
 }
 
 EXPORTS = ["sph_last_error", *_PROTOS]

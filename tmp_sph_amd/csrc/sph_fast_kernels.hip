@@ -1,10 +1,19 @@
 // Fast-path kernels of the delta-SPH step for gfx950.
 //
+// One launch covers every dam-break instance of a context (K >= 1): the
+// particles of all instances are sorted together by a global cell key
+// (instance cell offset + local cell hash), so each instance keeps its own
+// contiguous sorted range and its own neighbour order, and the per-particle
+// work reads its instance's parameters.  A wave selects parameters with a
+// waterfall loop over the distinct instances among its lanes, so they stay
+// wave-uniform (SGPRs); waves inside one instance run the loop body once.
+//
 // Layout (one particle set, sorted by cell):
 //   PR[i] = (x, y, z, +rho | -rho)  float4; the sign of w marks fluid (+)
 //   VP[i] = (vx, vy, vz, p)         float4
 //   X[k][i]                         FP64 positions, SoA; PR.xyz == (float)X
-//   ID[i], TY[i]                    original id and reference type
+//   ID[i]                           original (concatenated) particle index
+//   TY[i]                           instance << 8 | reference type
 // A neighbour candidate costs one 16-byte load (PR); VP and dofv are loaded
 // only for pairs inside the support.  Positions are FP64 only in the
 // integrator; hashing and the neighbour loops use the FP32 copy.
@@ -25,12 +34,44 @@ inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) /
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-__global__ void hash_fast_kernel(sph_params P, const float4* __restrict__ PR, int* __restrict__ keys,
-                                 int* __restrict__ idx, int n) {
+__device__ __forceinline__ int inst_of(int ty) { return ty >> 8; }
+__device__ __forceinline__ int type_of(int ty) { return ty & 0xff; }
+
+// Calls body(P, cell_offset) once for this lane with its instance's
+// parameters.  Each pass serves the lanes that share the first active lane's
+// instance, so P is wave-uniform inside body.
+template <class F>
+__device__ __forceinline__ void with_instance(const InstTable& T, int inst, F&& body) {
+  for (;;) {
+    const int u = __builtin_amdgcn_readfirstlane(inst);
+    if (inst == u) {
+      body(T.P[u], T.cell_off[u]);
+      break;
+    }
+  }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2.  Neighbouring target tiles share most of
+// their candidates, so the block id is remapped to give every XCD one
+// contiguous run of tiles instead of every eighth tile.
+__device__ __forceinline__ int tile_id(bool remap) {
+  const int b = blockIdx.x;
+  if (!remap) return b;
+  const int nb = gridDim.x;
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+
+__global__ void hash_fast_kernel(InstTable T, const float4* __restrict__ PR, const int* __restrict__ TY,
+                                 int* __restrict__ keys, int* __restrict__ idx, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 p = PR[i];
-  keys[i] = cell_hash(P, p.x, p.y, p.z);
+  with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
+    keys[i] = coff + cell_hash(P, p.x, p.y, p.z);
+  });
   idx[i] = i;
 }
 
@@ -65,30 +106,17 @@ struct RowRange {
   int s, e;
 };
 
-// Candidates loaded per batch in the neighbour loops (independent loads in flight).
-constexpr int UNROLL = 4;
-
-// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8), each with its own L2.  Neighbouring target tiles share most of
-// their candidates, so remap the hardware block id so that every XCD works on
-// one contiguous run of tiles instead of every eighth tile.
-__device__ __forceinline__ int xcd_tile() {
-  const int nb = gridDim.x;
-  const int b = blockIdx.x;
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, k = b >> 3;
-  return x * q + min(x, r) + k;
-}
-
-// Sorted index ranges of the stencil rows of cell `hash`, in the reference
-// order (z outer, y inner; the x-adjacent cells of a row are contiguous).
+// Sorted index ranges of the stencil rows of the cell with global key `key`,
+// in the reference order (z outer, y inner; the x-adjacent cells of a row are
+// contiguous in sorted order, so each row is one range).
 template <int DIM>
-__device__ __forceinline__ int row_ranges(const sph_params& P, int hash,
-                                          const int* __restrict__ cellStart,
-                                          const int* __restrict__ cellEnd,
-                                          RowRange (&rows)[DIM == 3 ? 9 : 3]) {
+__device__ __forceinline__ void row_ranges(const sph_params& P, int coff, int key,
+                                           const int* __restrict__ cellStart,
+                                           const int* __restrict__ cellEnd,
+                                           RowRange (&rows)[DIM == 3 ? 9 : 3]) {
   const int X = P.gridxdim, Y = P.gridydim, Z = P.gridzdim;
   const int XY = X * Y;
+  const int hash = key - coff;
   const int cz = hash / XY;
   const int rem = hash - cz * XY;
   const int cy = rem / X;
@@ -103,7 +131,7 @@ __device__ __forceinline__ int row_ranges(const sph_params& P, int hash,
       const int z = cz + dz, y = cy + dy;
       int s = 0, e = 0;
       if (z >= 0 && z < Z && y >= 0 && y < Y) {
-        const int rowBase = z * XY + y * X;
+        const int rowBase = coff + z * XY + y * X;
         s = -1;
         for (int c = rowBase + x0; c <= rowBase + x1; ++c) {
           const int st = cellStart[c];
@@ -119,82 +147,87 @@ __device__ __forceinline__ int row_ranges(const sph_params& P, int hash,
       ++nr;
     }
   }
-  return nr;
 }
 
 // Kernel A: wall pressure by Shepard extrapolation with the hydrostatic
 // correction, and the velocity divergence of fluid particles.
-template <int DIM>
-__global__ __launch_bounds__(256) void nsA_fast_kernel(sph_params P, float4* PR, float4* VP,
-                                                       float* __restrict__ DV,
+template <int DIM, int UNROLL>
+__global__ __launch_bounds__(256) void nsA_fast_kernel(InstTable T, float4* PR, float4* VP,
+                                                       const int* __restrict__ TY, float* __restrict__ DV,
                                                        const int* __restrict__ keys,
                                                        const int* __restrict__ cellStart,
-                                                       const int* __restrict__ cellEnd, int n) {
-  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+                                                       const int* __restrict__ cellEnd, int n, int remap) {
+  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 pi = PR[i];
   const float4 vi = VP[i];
   const bool fi = pi.w > 0.0f;
-  RowRange rows[DIM == 3 ? 9 : 3];
-  const int nr = row_ranges<DIM>(P, keys[i], cellStart, cellEnd, rows);
-  const float inv_h = 1.0f / P.h;
-  const float cfr = -5.0f * P.adh / (P.h * P.h);
-  const float mg = 0.0f - P.gravity;
-  const float m = P.particleMass;
-  float sp = 0.0f, sw = 0.0f, dv = 0.0f;
-  for (int r = 0; r < nr; ++r) {
-    const int s = rows[r].s, e = rows[r].e;
-    for (int j0 = s; j0 < e; j0 += UNROLL) {
-    float4 pjs[UNROLL];
+  const int key = keys[i];
+  with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
+    RowRange rows[DIM == 3 ? 9 : 3];
+    row_ranges<DIM>(P, coff, key, cellStart, cellEnd, rows);
+    const float inv_h = 1.0f / P.h;
+    const float cfr = -5.0f * P.adh / (P.h * P.h);
+    const float mg = 0.0f - P.gravity;
+    const float m = P.particleMass;
+    const float cut = P.rr_cut;
+    float sp = 0.0f, sw = 0.0f, dv = 0.0f;
+#pragma unroll 1
+    for (int r = 0; r < (DIM == 3 ? 9 : 3); ++r) {
+      const int s = rows[r].s, e = rows[r].e;
+      for (int j0 = s; j0 < e; j0 += UNROLL) {
+        float4 pjs[UNROLL];
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) pjs[u] = PR[min(j0 + u, e - 1)];  // loads issued together
+        for (int u = 0; u < UNROLL; ++u) pjs[u] = PR[min(j0 + u, e - 1)];  // loads issued together
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      const int j = j0 + u;
-      const float4 pj = pjs[u];
-      const float dx = pi.x - pj.x;
-      const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
-      const float dz = pi.z - pj.z;
-      const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
-      if (j < e && rr <= P.rr_cut && j != i && pj.w > 0.0f) {
-        const float q = fast_sqrt(rr) * inv_h;
-        const float t = 1.0f - 0.5f * q;
-        const float t3 = t * t * t;
-        const float4 vj = VP[j];
-        if (!fi) {
-          const float w = t3 * t * ((2.0f * q + 1.0f) * P.adh);
-          sp += (vj.w - pj.w * (mg * dz)) * w;
-          sw += w;
-        } else {
-          const float dvdx = DIM == 3 ? (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz
-                                      : (vi.x - vj.x) * dx + (vi.z - vj.z) * dz;
-          dv -= cfr * t3 * dvdx * m * fast_rcp(pj.w);
+        for (int u = 0; u < UNROLL; ++u) {
+          const int j = j0 + u;
+          const float4 pj = pjs[u];
+          const float dx = pi.x - pj.x;
+          const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
+          const float dz = pi.z - pj.z;
+          const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
+          if (j < e && rr <= cut && j != i && pj.w > 0.0f) {
+            const float q = fast_sqrt(rr) * inv_h;
+            const float t = 1.0f - 0.5f * q;
+            const float t3 = t * t * t;
+            const float4 vj = VP[j];
+            if (!fi) {
+              const float w = t3 * t * ((2.0f * q + 1.0f) * P.adh);
+              sp += (vj.w - pj.w * (mg * dz)) * w;
+              sw += w;
+            } else {
+              const float dvdx = DIM == 3 ? (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz
+                                          : (vi.x - vj.x) * dx + (vi.z - vj.z) * dz;
+              dv -= cfr * t3 * dvdx * m * fast_rcp(pj.w);
+            }
+          }
         }
       }
     }
+    DV[i] = dv;
+    if (!fi) {
+      float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
+      if (pw < 0.0f) pw = 0.0f;
+      reinterpret_cast<float*>(VP)[4 * i + 3] = pw;
+      reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
     }
-  }
-  DV[i] = dv;
-  if (!fi) {
-    float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
-    if (pw < 0.0f) pw = 0.0f;
-    reinterpret_cast<float*>(VP)[4 * i + 3] = pw;
-    reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
-  }
+  });
 }
 
 // Kernel B: continuity with delta-SPH diffusion and momentum with pressure,
 // artificial viscosity, acoustic damper and gravity.  Non-fluid targets are
 // skipped: the integrator never reads their rates.
-template <int DIM>
-__global__ __launch_bounds__(256) void nsB_fast_kernel(sph_params P, const float4* __restrict__ PR,
+template <int DIM, int UNROLL>
+__global__ __launch_bounds__(256) void nsB_fast_kernel(InstTable T, const float4* __restrict__ PR,
                                                        const float4* __restrict__ VP,
+                                                       const int* __restrict__ TY,
                                                        const float* __restrict__ DV,
                                                        const int* __restrict__ keys,
                                                        const int* __restrict__ cellStart,
                                                        const int* __restrict__ cellEnd,
-                                                       float4* __restrict__ RATE, int n) {
-  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+                                                       float4* __restrict__ RATE, int n, int remap) {
+  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 pi = PR[i];
   if (!(pi.w > 0.0f)) {
@@ -203,56 +236,64 @@ __global__ __launch_bounds__(256) void nsB_fast_kernel(sph_params P, const float
   }
   const float4 vi = VP[i];
   const float rho_i = pi.w;
-  const float dofv_i = P.damper * DV[i];
-  RowRange rows[DIM == 3 ? 9 : 3];
-  const int nr = row_ranges<DIM>(P, keys[i], cellStart, cellEnd, rows);
-  const float inv_h = 1.0f / P.h;
-  const float cfr = -5.0f * P.adh / (P.h * P.h);
-  const float m = P.particleMass;
-  const float dhc = P.delta * P.h * P.cs;
-  const float eta2 = P.eta * P.eta;
-  const float inv_rho_i = 1.0f / rho_i;
-  const float avc = P.h * P.cs * P.restDensity * m * inv_rho_i;
-  const float pm_i = m * inv_rho_i;
-  float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (int r = 0; r < nr; ++r) {
-    const int s = rows[r].s, e = rows[r].e;
-    for (int j0 = s; j0 < e; j0 += UNROLL) {
-    float4 pjs[UNROLL];
+  const float dvi = DV[i];
+  const int key = keys[i];
+  with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
+    RowRange rows[DIM == 3 ? 9 : 3];
+    row_ranges<DIM>(P, coff, key, cellStart, cellEnd, rows);
+    const float inv_h = 1.0f / P.h;
+    const float cfr = -5.0f * P.adh / (P.h * P.h);
+    const float m = P.particleMass;
+    const float dhc = P.delta * P.h * P.cs;
+    const float eta2 = P.eta * P.eta;
+    const float cut = P.rr_cut;
+    const float damper = P.damper;
+    const float afa = P.afa;
+    const float dofv_i = damper * dvi;
+    const float inv_rho_i = 1.0f / rho_i;
+    const float avc = P.h * P.cs * P.restDensity * m * inv_rho_i;
+    const float pm_i = m * inv_rho_i;
+    float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll 1
+    for (int r = 0; r < (DIM == 3 ? 9 : 3); ++r) {
+      const int s = rows[r].s, e = rows[r].e;
+      for (int j0 = s; j0 < e; j0 += UNROLL) {
+        float4 pjs[UNROLL];
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) pjs[u] = PR[min(j0 + u, e - 1)];
+        for (int u = 0; u < UNROLL; ++u) pjs[u] = PR[min(j0 + u, e - 1)];
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      const int j = j0 + u;
-      const float4 pj = pjs[u];
-      const float dx = pi.x - pj.x;
-      const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
-      const float dz = pi.z - pj.z;
-      const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
-      if (j < e && rr <= P.rr_cut && j != i) {
-        const float4 vj = VP[j];
-        const bool fj = pj.w > 0.0f;
-        const float rho_j = fabsf(pj.w);
-        const float q = fast_sqrt(rr) * inv_h;
-        const float t = 1.0f - 0.5f * q;
-        const float fr = cfr * (t * t * t);
-        const float inv_rho_j = fast_rcp(rho_j);
-        const float inv_re = fast_rcp(rr + eta2);
-        const float dvdx = DIM == 3 ? (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz
-                                    : (vi.x - vj.x) * dx + (vi.z - vj.z) * dz;
-        const float frj = fr * inv_rho_j;
-        drho += frj * m * (rho_i * dvdx + (rho_i - rho_j) * rr * dhc * inv_re);
-        float vt = -(vi.w + vj.w) * pm_i;
-        if (fj) vt += avc * (dofv_i + P.damper * DV[j] + P.afa * dvdx * inv_re);
-        vt *= frj;
-        ax += vt * dx;
-        if (DIM == 3) ay += vt * dy;
-        az += vt * dz;
+        for (int u = 0; u < UNROLL; ++u) {
+          const int j = j0 + u;
+          const float4 pj = pjs[u];
+          const float dx = pi.x - pj.x;
+          const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
+          const float dz = pi.z - pj.z;
+          const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
+          if (j < e && rr <= cut && j != i) {
+            const float4 vj = VP[j];
+            const bool fj = pj.w > 0.0f;
+            const float rho_j = fabsf(pj.w);
+            const float q = fast_sqrt(rr) * inv_h;
+            const float t = 1.0f - 0.5f * q;
+            const float fr = cfr * (t * t * t);
+            const float inv_rho_j = fast_rcp(rho_j);
+            const float inv_re = fast_rcp(rr + eta2);
+            const float dvdx = DIM == 3 ? (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz
+                                        : (vi.x - vj.x) * dx + (vi.z - vj.z) * dz;
+            const float frj = fr * inv_rho_j;
+            drho += frj * m * (rho_i * dvdx + (rho_i - rho_j) * rr * dhc * inv_re);
+            float vt = -(vi.w + vj.w) * pm_i;
+            if (fj) vt += avc * (dofv_i + damper * DV[j] + afa * dvdx * inv_re);
+            vt *= frj;
+            ax += vt * dx;
+            if (DIM == 3) ay += vt * dy;
+            az += vt * dz;
+          }
+        }
       }
     }
-    }
-  }
-  RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
+    RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
+  });
 }
 
 __device__ __forceinline__ bool outside64(const sph_params& P, double x, double y, double z) {
@@ -264,90 +305,94 @@ __device__ __forceinline__ bool outside64(const sph_params& P, double x, double 
 // positions.  The predictor stores the step-start state at its own sorted
 // slot; the corrector finds it through perm (the half-step-2 sort order).
 template <int DIM, bool PRED>
-__global__ void pc_fast_kernel(sph_params P, FastSet st, FastWork w, const int* __restrict__ perm,
-                               int n) {
+__global__ void pc_fast_kernel(InstTable T, FastSet st, FastWork w, const int* __restrict__ perm, int n) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 pr = st.PR[i];
   if (!(pr.w > 0.0f)) return;  // fluid <=> type 1 <=> positive signed density
-  const double x0 = st.X[0][i];
-  const double y0 = DIM == 3 ? st.X[1][i] : 0.0;
-  const double z0 = st.X[2][i];
-  if (outside64(P, x0, y0, z0)) {
-    st.X[0][i] = P.park_pos[0];
-    if (DIM == 3) st.X[1][i] = P.park_pos[1];
-    st.X[2][i] = P.park_pos[2];
-    const float rs = P.park_type == 1 ? P.restDensity : -P.restDensity;
-    st.PR[i] = make_float4(P.park_pos[0], DIM == 3 ? P.park_pos[1] : 0.0f, P.park_pos[2], rs);
-    st.VP[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    st.TY[i] = P.park_type;
-    return;
-  }
-  const float4 vp = st.VP[i];
-  const float4 rate = w.RATE[i];
-  const float dt = P.timeStep;
-  const double hdt = 0.5 * (double)dt;
-  double bx, by, bz;
-  float4 bv;  // (vx, vy, vz, rho) at step start
-  if (PRED) {
-    bx = x0; by = y0; bz = z0;
-    bv = make_float4(vp.x, vp.y, vp.z, pr.w);
-    w.TX[0][i] = bx;
-    if (DIM == 3) w.TX[1][i] = by;
-    w.TX[2][i] = bz;
-    w.TV[i] = bv;
-  } else {
-    const int r = perm[i];
-    bx = w.TX[0][r];
-    by = DIM == 3 ? w.TX[1][r] : 0.0;
-    bz = w.TX[2][r];
-    bv = w.TV[r];
-  }
-  float rho = (float)((double)bv.w + (double)(rate.x * dt) / 2.0);
-  if (rho < P.restDensity) rho = P.restDensity;
-  double nx = bx + (double)vp.x * hdt;
-  double ny = DIM == 3 ? by + (double)vp.y * hdt : 0.0;
-  double nz = bz + (double)vp.z * hdt;
-  float vx = (float)((double)bv.x + (double)(rate.y * dt) / 2.0);
-  float vy = DIM == 3 ? (float)((double)bv.y + (double)(rate.z * dt) / 2.0) : 0.0f;
-  float vz = (float)((double)bv.z + (double)(rate.w * dt) / 2.0);
-  if (!PRED) {
-    rho = 2 * rho - bv.w;
+  const int ty = st.TY[i];
+  with_instance(T, inst_of(ty), [&](const sph_params& P, int) {
+    const double x0 = st.X[0][i];
+    const double y0 = DIM == 3 ? st.X[1][i] : 0.0;
+    const double z0 = st.X[2][i];
+    if (outside64(P, x0, y0, z0)) {
+      st.X[0][i] = P.park_pos[0];
+      if (DIM == 3) st.X[1][i] = P.park_pos[1];
+      st.X[2][i] = P.park_pos[2];
+      const float rs = P.park_type == 1 ? P.restDensity : -P.restDensity;
+      st.PR[i] = make_float4(P.park_pos[0], DIM == 3 ? P.park_pos[1] : 0.0f, P.park_pos[2], rs);
+      st.VP[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      st.TY[i] = (inst_of(ty) << 8) | (P.park_type & 0xff);
+      return;
+    }
+    const float4 vp = st.VP[i];
+    const float4 rate = w.RATE[i];
+    const float dt = P.timeStep;
+    const double hdt = 0.5 * (double)dt;
+    double bx, by, bz;
+    float4 bv;  // (vx, vy, vz, rho) at step start
+    if (PRED) {
+      bx = x0;
+      by = y0;
+      bz = z0;
+      bv = make_float4(vp.x, vp.y, vp.z, pr.w);
+      w.TX[0][i] = bx;
+      if (DIM == 3) w.TX[1][i] = by;
+      w.TX[2][i] = bz;
+      w.TV[i] = bv;
+    } else {
+      const int r = perm[i];
+      bx = w.TX[0][r];
+      by = DIM == 3 ? w.TX[1][r] : 0.0;
+      bz = w.TX[2][r];
+      bv = w.TV[r];
+    }
+    float rho = (float)((double)bv.w + (double)(rate.x * dt) / 2.0);
     if (rho < P.restDensity) rho = P.restDensity;
-    nx = 2.0 * nx - bx;
-    ny = DIM == 3 ? 2.0 * ny - by : 0.0;
-    nz = 2.0 * nz - bz;
-    vx = 2 * vx - bv.x;
-    vy = DIM == 3 ? 2 * vy - bv.y : 0.0f;
-    vz = 2 * vz - bv.z;
-  }
-  st.X[0][i] = nx;
-  if (DIM == 3) st.X[1][i] = ny;
-  st.X[2][i] = nz;
-  st.PR[i] = make_float4((float)nx, (float)ny, (float)nz, rho);
-  st.VP[i] = make_float4(vx, vy, vz, eos_pressure(P, rho));
+    double nx = bx + (double)vp.x * hdt;
+    double ny = DIM == 3 ? by + (double)vp.y * hdt : 0.0;
+    double nz = bz + (double)vp.z * hdt;
+    float vx = (float)((double)bv.x + (double)(rate.y * dt) / 2.0);
+    float vy = DIM == 3 ? (float)((double)bv.y + (double)(rate.z * dt) / 2.0) : 0.0f;
+    float vz = (float)((double)bv.z + (double)(rate.w * dt) / 2.0);
+    if (!PRED) {
+      rho = 2 * rho - bv.w;
+      if (rho < P.restDensity) rho = P.restDensity;
+      nx = 2.0 * nx - bx;
+      ny = DIM == 3 ? 2.0 * ny - by : 0.0;
+      nz = 2.0 * nz - bz;
+      vx = 2 * vx - bv.x;
+      vy = DIM == 3 ? 2 * vy - bv.y : 0.0f;
+      vz = 2 * vz - bv.z;
+    }
+    st.X[0][i] = nx;
+    if (DIM == 3) st.X[1][i] = ny;
+    st.X[2][i] = nz;
+    st.PR[i] = make_float4((float)nx, (float)ny, (float)nz, rho);
+    st.VP[i] = make_float4(vx, vy, vz, eos_pressure(P, rho));
+  });
 }
 
-__global__ void upload_kernel(sph_params P, FastSet st, const float* __restrict__ pos,
-                              const float* __restrict__ vel, const float* __restrict__ rho,
-                              const float* __restrict__ prs, const int* __restrict__ type, int n) {
+__global__ void upload_kernel(int dim, FastSet st, const float* __restrict__ pos, const float* __restrict__ vel,
+                              const float* __restrict__ rho, const float* __restrict__ prs,
+                              const int* __restrict__ type, const int* __restrict__ inst, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float x = pos[3 * i], y = P.dim == 3 ? pos[3 * i + 1] : 0.0f, z = pos[3 * i + 2];
+  const float x = pos[3 * i], y = dim == 3 ? pos[3 * i + 1] : 0.0f, z = pos[3 * i + 2];
   const int t = type[i];
   st.X[0][i] = (double)x;
   if (st.X[1]) st.X[1][i] = (double)y;
   st.X[2][i] = (double)z;
   st.PR[i] = make_float4(x, y, z, t == 1 ? fabsf(rho[i]) : -fabsf(rho[i]));
-  st.VP[i] = make_float4(vel[3 * i], P.dim == 3 ? vel[3 * i + 1] : 0.0f, vel[3 * i + 2], prs[i]);
+  st.VP[i] = make_float4(vel[3 * i], dim == 3 ? vel[3 * i + 1] : 0.0f, vel[3 * i + 2], prs[i]);
   st.ID[i] = i;
-  st.TY[i] = t;
+  st.TY[i] = (inst[i] << 8) | (t & 0xff);
 }
 
 __global__ void download_kernel(FastSet st, float* __restrict__ pos, float* __restrict__ vel,
-                                float* __restrict__ rho, float* __restrict__ prs,
-                                int* __restrict__ type, double* __restrict__ pos64, int n) {
+                                float* __restrict__ rho, float* __restrict__ prs, int* __restrict__ type,
+                                double* __restrict__ pos64, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int o = st.ID[i];
@@ -371,34 +416,37 @@ __global__ void download_kernel(FastSet st, float* __restrict__ pos, float* __re
   }
   if (rho) rho[o] = fabsf(pr.w);
   if (prs) prs[o] = vp.w;
-  if (type) type[o] = st.TY[i];
+  if (type) type[o] = type_of(st.TY[i]);
 }
 
 // Candidate / in-support pair counts of kernels A and B (statistics only).
 template <int DIM>
-__global__ void pair_stats_kernel(sph_params P, const float4* __restrict__ PR, const int* __restrict__ keys,
-                                  const int* __restrict__ cellStart, const int* __restrict__ cellEnd,
-                                  unsigned long long* __restrict__ out, int n) {
+__global__ void pair_stats_kernel(InstTable T, const float4* __restrict__ PR, const int* __restrict__ TY,
+                                  const int* __restrict__ keys, const int* __restrict__ cellStart,
+                                  const int* __restrict__ cellEnd, unsigned long long* __restrict__ out,
+                                  int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long c = 0, hA = 0, hB = 0;
   bool fi = false;
   if (i < n) {
     const float4 pi = PR[i];
     fi = pi.w > 0.0f;
-    RowRange rows[DIM == 3 ? 9 : 3];
-    const int nr = row_ranges<DIM>(P, keys[i], cellStart, cellEnd, rows);
-    for (int r = 0; r < nr; ++r)
-      for (int j = rows[r].s; j < rows[r].e; ++j) {
-        if (j == i) continue;
-        ++c;
-        const float4 pj = PR[j];
-        const float dx = pi.x - pj.x, dy = DIM == 3 ? pi.y - pj.y : 0.0f, dz = pi.z - pj.z;
-        const float rr = dx * dx + dy * dy + dz * dz;
-        if (rr <= P.rr_cut) {
-          ++hB;
-          if (pj.w > 0.0f) ++hA;
+    with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
+      RowRange rows[DIM == 3 ? 9 : 3];
+      row_ranges<DIM>(P, coff, keys[i], cellStart, cellEnd, rows);
+      for (int r = 0; r < (DIM == 3 ? 9 : 3); ++r)
+        for (int j = rows[r].s; j < rows[r].e; ++j) {
+          if (j == i) continue;
+          ++c;
+          const float4 pj = PR[j];
+          const float dx = pi.x - pj.x, dy = DIM == 3 ? pi.y - pj.y : 0.0f, dz = pi.z - pj.z;
+          const float rr = dx * dx + dy * dy + dz * dz;
+          if (rr <= P.rr_cut) {
+            ++hB;
+            if (pj.w > 0.0f) ++hA;
+          }
         }
-      }
+    });
   }
   unsigned long long v[4] = {c, hA, fi ? c : 0ull, fi ? hB : 0ull};
   for (int k = 0; k < 4; ++k) {
@@ -408,8 +456,8 @@ __global__ void pair_stats_kernel(sph_params P, const float4* __restrict__ PR, c
   }
 }
 
-__global__ void download_rates_kernel(FastSet st, FastWork w, float* __restrict__ drho,
-                                      float* __restrict__ acc, float* __restrict__ dofv, int n) {
+__global__ void download_rates_kernel(FastSet st, FastWork w, float* __restrict__ drho, float* __restrict__ acc,
+                                      float* __restrict__ dofv, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int o = st.ID[i];
@@ -424,139 +472,133 @@ __global__ void download_rates_kernel(FastSet st, FastWork w, float* __restrict_
 }
 
 // Surge front (max x of in-domain fluid) and column height (max z of fluid
-// with x <= band).  Both are >= 0 inside the domain, so the IEEE bit
-// patterns order like the values and an integer atomicMax reduces them.
-__global__ void probe_kernel(sph_params P, FastSet st, double band,
-                             unsigned long long* __restrict__ out, int n) {
+// with x <= band) per instance.  Both are >= 0 inside the domain, so the IEEE
+// bit patterns order like the values and an integer atomicMax reduces them.
+__global__ void probe_kernel(InstTable T, FastSet st, double band, unsigned long long* __restrict__ out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double front = 0.0, height = 0.0;
-  if (i < n && st.TY[i] == 1) {
-    const double x = st.X[0][i], z = st.X[2][i];
-    const double y = st.X[1] ? st.X[1][i] : 0.0;
-    if (!outside64(P, x, y, z)) {
-      front = x;
-      if (x <= band) height = z;
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    front = fmax(front, __shfl_xor(front, off));
-    height = fmax(height, __shfl_xor(height, off));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(&out[0], (unsigned long long)__double_as_longlong(front));
-    atomicMax(&out[1], (unsigned long long)__double_as_longlong(height));
-  }
+  if (i >= n) return;
+  const int ty = st.TY[i];
+  if (type_of(ty) != 1) return;
+  const int k = inst_of(ty);
+  const double x = st.X[0][i], z = st.X[2][i];
+  const double y = st.X[1] ? st.X[1][i] : 0.0;
+  const sph_params& P = T.P[k];
+  if (outside64(P, x, y, z)) return;
+  atomicMax(&out[2 * k], (unsigned long long)__double_as_longlong(x));
+  if (x <= band) atomicMax(&out[2 * k + 1], (unsigned long long)__double_as_longlong(z));
+}
+
+template <class K2, class K3, class... A>
+hipError_t launch_dim(int dim, K2 k2, K3 k3, dim3 g, dim3 b, hipStream_t s, A... args) {
+  if (dim == 3)
+    hipLaunchKernelGGL(k3, g, b, 0, s, args...);
+  else
+    hipLaunchKernelGGL(k2, g, b, 0, s, args...);
+  return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_hash_fast(const sph_params& P, const FastSet& in, int* keys, int* idx, int n,
-                            hipStream_t s) {
+hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(hash_fast_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, in.PR, keys,
-                     idx, n);
+  hipLaunchKernelGGL(hash_fast_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, in.PR, in.TY, keys, idx, n);
   return hipGetLastError();
 }
 
-hipError_t launch_cells_gather_fast(const sph_params& P, const FastSet& in, const FastSet& out,
-                                    const int* keys, const int* idx, int* cellStart,
-                                    int* cellEnd, int n, hipStream_t s) {
-  hipError_t e = fill_int(cellStart, -1, P.numCells, s);
+hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
+                                    const int* idx, int* cellStart, int* cellEnd, int n, hipStream_t s) {
+  hipError_t e = fill_int(cellStart, -1, T.total_cells, s);
   if (e != hipSuccess || n <= 0) return e;
-  if (P.dim == 3)
-    hipLaunchKernelGGL(cells_gather_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, out,
-                       keys, idx, cellStart, cellEnd, n);
-  else
-    hipLaunchKernelGGL(cells_gather_kernel<2>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, out,
-                       keys, idx, cellStart, cellEnd, n);
-  return hipGetLastError();
+  return launch_dim(T.dim, cells_gather_kernel<2>, cells_gather_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), s,
+                    in, out, keys, idx, cellStart, cellEnd, n);
 }
 
-hipError_t launch_nsA_fast(const sph_params& P, const FastSet& st, const FastWork& w, int n,
-                           int block, hipStream_t s) {
+template <int U>
+static hipError_t nsA_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
+                        hipStream_t s) {
+  return launch_dim(T.dim, nsA_fast_kernel<2, U>, nsA_fast_kernel<3, U>, dim3(blocks_for(n, block)), dim3(block),
+                    s, T, st.PR, st.VP, (const int*)st.TY, w.DV, (const int*)w.keys, (const int*)w.cellStart,
+                    (const int*)w.cellEnd, n, remap);
+}
+
+template <int U>
+static hipError_t nsB_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
+                        hipStream_t s) {
+  return launch_dim(T.dim, nsB_fast_kernel<2, U>, nsB_fast_kernel<3, U>, dim3(blocks_for(n, block)), dim3(block),
+                    s, T, (const float4*)st.PR, (const float4*)st.VP, (const int*)st.TY, (const float*)w.DV,
+                    (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap);
+}
+
+static int norm_block(int b) { return (b >= 64 && b <= 256 && b % 64 == 0) ? b : 256; }
+
+hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (block <= 0 || block > 256) block = 256;
-  if (P.dim == 3)
-    hipLaunchKernelGGL(nsA_fast_kernel<3>, dim3(blocks_for(n, block)), dim3(block), 0, s, P, st.PR,
-                       st.VP, w.DV, w.keys, w.cellStart, w.cellEnd, n);
-  else
-    hipLaunchKernelGGL(nsA_fast_kernel<2>, dim3(blocks_for(n, block)), dim3(block), 0, s, P, st.PR,
-                       st.VP, w.DV, w.keys, w.cellStart, w.cellEnd, n);
-  return hipGetLastError();
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0;
+  switch (tu.nbr_unroll) {
+    case 1: return nsA_u<1>(T, st, w, n, b, rm, s);
+    case 2: return nsA_u<2>(T, st, w, n, b, rm, s);
+    case 8: return nsA_u<8>(T, st, w, n, b, rm, s);
+    default: return nsA_u<4>(T, st, w, n, b, rm, s);
+  }
 }
 
-hipError_t launch_nsB_fast(const sph_params& P, const FastSet& st, const FastWork& w, int n,
-                           int block, hipStream_t s) {
+hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (block <= 0 || block > 256) block = 256;
-  if (P.dim == 3)
-    hipLaunchKernelGGL(nsB_fast_kernel<3>, dim3(blocks_for(n, block)), dim3(block), 0, s, P, st.PR,
-                       st.VP, w.DV, w.keys, w.cellStart, w.cellEnd, w.RATE, n);
-  else
-    hipLaunchKernelGGL(nsB_fast_kernel<2>, dim3(blocks_for(n, block)), dim3(block), 0, s, P, st.PR,
-                       st.VP, w.DV, w.keys, w.cellStart, w.cellEnd, w.RATE, n);
-  return hipGetLastError();
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0;
+  switch (tu.nbr_unroll) {
+    case 1: return nsB_u<1>(T, st, w, n, b, rm, s);
+    case 2: return nsB_u<2>(T, st, w, n, b, rm, s);
+    case 8: return nsB_u<8>(T, st, w, n, b, rm, s);
+    default: return nsB_u<4>(T, st, w, n, b, rm, s);
+  }
 }
 
-hipError_t launch_pc_fast(const sph_params& P, bool predictor, const FastSet& st,
-                          const FastWork& w, const int* perm, int n, hipStream_t s) {
+hipError_t launch_pc_fast(const InstTable& T, bool predictor, const FastSet& st, const FastWork& w,
+                          const int* perm, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g(blocks_for(n, 256)), b(256);
-  if (P.dim == 3) {
-    if (predictor)
-      hipLaunchKernelGGL((pc_fast_kernel<3, true>), g, b, 0, s, P, st, w, perm, n);
-    else
-      hipLaunchKernelGGL((pc_fast_kernel<3, false>), g, b, 0, s, P, st, w, perm, n);
-  } else {
-    if (predictor)
-      hipLaunchKernelGGL((pc_fast_kernel<2, true>), g, b, 0, s, P, st, w, perm, n);
-    else
-      hipLaunchKernelGGL((pc_fast_kernel<2, false>), g, b, 0, s, P, st, w, perm, n);
-  }
+  if (predictor)
+    return launch_dim(T.dim, pc_fast_kernel<2, true>, pc_fast_kernel<3, true>, g, b, s, T, st, w, perm, n);
+  return launch_dim(T.dim, pc_fast_kernel<2, false>, pc_fast_kernel<3, false>, g, b, s, T, st, w, perm, n);
+}
+
+hipError_t launch_upload_fast(int dim, const FastSet& st, const float* pos, const float* vel, const float* rho,
+                              const float* prs, const int* type, const int* inst, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(upload_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, dim, st, pos, vel, rho, prs, type,
+                     inst, n);
   return hipGetLastError();
 }
 
-hipError_t launch_upload_fast(const sph_params& P, const FastSet& st, const float* pos,
-                              const float* vel, const float* rho, const float* prs,
-                              const int* type, int n, hipStream_t s) {
+hipError_t launch_download_fast(const FastSet& st, float* pos, float* vel, float* rho, float* prs, int* type,
+                                double* pos64, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(upload_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, pos, vel, rho,
-                     prs, type, n);
+  hipLaunchKernelGGL(download_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, pos, vel, rho, prs, type,
+                     pos64, n);
   return hipGetLastError();
 }
 
-hipError_t launch_download_fast(const FastSet& st, float* pos, float* vel, float* rho, float* prs,
-                                int* type, double* pos64, int n, hipStream_t s) {
+hipError_t launch_download_rates(const FastSet& st, const FastWork& w, float* drho, float* acc, float* dofv, int n,
+                                 hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(download_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, pos, vel, rho,
-                     prs, type, pos64, n);
+  hipLaunchKernelGGL(download_rates_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, w, drho, acc, dofv, n);
   return hipGetLastError();
 }
 
-hipError_t launch_download_rates(const FastSet& st, const FastWork& w, float* drho, float* acc,
-                                 float* dofv, int n, hipStream_t s) {
+hipError_t launch_pair_stats(const InstTable& T, const FastSet& st, const FastWork& w, unsigned long long* out4,
+                             int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(download_rates_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, w, drho, acc,
-                     dofv, n);
-  return hipGetLastError();
+  return launch_dim(T.dim, pair_stats_kernel<2>, pair_stats_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), s, T,
+                    (const float4*)st.PR, (const int*)st.TY, (const int*)w.keys, (const int*)w.cellStart,
+                    (const int*)w.cellEnd, out4, n);
 }
 
-hipError_t launch_pair_stats(const sph_params& P, const FastSet& st, const FastWork& w,
-                             unsigned long long* out4, int n, hipStream_t s) {
+hipError_t launch_probe_fast(const InstTable& T, const FastSet& st, double band, unsigned long long* out, int n,
+                             hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  if (P.dim == 3)
-    hipLaunchKernelGGL(pair_stats_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st.PR, w.keys,
-                       w.cellStart, w.cellEnd, out4, n);
-  else
-    hipLaunchKernelGGL(pair_stats_kernel<2>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st.PR, w.keys,
-                       w.cellStart, w.cellEnd, out4, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_probe_fast(const sph_params& P, const FastSet& st, double band,
-                             unsigned long long* out2, int n, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(probe_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, st, band, out2, n);
+  hipLaunchKernelGGL(probe_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, st, band, out, n);
   return hipGetLastError();
 }
 

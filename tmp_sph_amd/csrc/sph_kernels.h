@@ -63,27 +63,34 @@ struct FastWork {
   float4* TV;      // (vx, vy, vz, rho) at the start of the step
 };
 
-hipError_t launch_hash_fast(const sph_params& P, const FastSet& in, int* keys, int* idx, int n,
-                            hipStream_t s);
-hipError_t launch_cells_gather_fast(const sph_params& P, const FastSet& in, const FastSet& out,
-                                    const int* keys, const int* idx, int* cellStart,
-                                    int* cellEnd, int n, hipStream_t s);
-hipError_t launch_nsA_fast(const sph_params& P, const FastSet& st, const FastWork& w, int n,
-                           int block, hipStream_t s);
-hipError_t launch_nsB_fast(const sph_params& P, const FastSet& st, const FastWork& w, int n,
-                           int block, hipStream_t s);
-hipError_t launch_pc_fast(const sph_params& P, bool predictor, const FastSet& st,
-                          const FastWork& w, const int* perm, int n, hipStream_t s);
-hipError_t launch_upload_fast(const sph_params& P, const FastSet& st, const float* pos,
-                              const float* vel, const float* rho, const float* prs,
-                              const int* type, int n, hipStream_t s);
-hipError_t launch_download_fast(const FastSet& st, float* pos, float* vel, float* rho, float* prs,
-                                int* type, double* pos64, int n, hipStream_t s);
-hipError_t launch_download_rates(const FastSet& st, const FastWork& w, float* drho, float* acc,
-                                 float* dofv, int n, hipStream_t s);
-hipError_t launch_pair_stats(const sph_params& P, const FastSet& st, const FastWork& w,
-                             unsigned long long* out4, int n, hipStream_t s);
-hipError_t launch_probe_fast(const sph_params& P, const FastSet& st, double band,
-                             unsigned long long* out2, int n, hipStream_t s);
+// Device table of the instances of one context (K >= 1).  Instance k owns the
+// global cells [cell_off[k], cell_off[k] + P[k].numCells); all share `dim`.
+struct InstTable {
+  const sph_params* P;
+  const int* cell_off;
+  int K;
+  int dim;
+  int total_cells;
+};
+
+hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s);
+hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
+                                    const int* idx, int* cellStart, int* cellEnd, int n, hipStream_t s);
+hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                           hipStream_t s);
+hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                           hipStream_t s);
+hipError_t launch_pc_fast(const InstTable& T, bool predictor, const FastSet& st, const FastWork& w,
+                          const int* perm, int n, hipStream_t s);
+hipError_t launch_upload_fast(int dim, const FastSet& st, const float* pos, const float* vel, const float* rho,
+                              const float* prs, const int* type, const int* inst, int n, hipStream_t s);
+hipError_t launch_download_fast(const FastSet& st, float* pos, float* vel, float* rho, float* prs, int* type,
+                                double* pos64, int n, hipStream_t s);
+hipError_t launch_download_rates(const FastSet& st, const FastWork& w, float* drho, float* acc, float* dofv, int n,
+                                 hipStream_t s);
+hipError_t launch_pair_stats(const InstTable& T, const FastSet& st, const FastWork& w, unsigned long long* out4,
+                             int n, hipStream_t s);
+hipError_t launch_probe_fast(const InstTable& T, const FastSet& st, double band, unsigned long long* out, int n,
+                             hipStream_t s);
 
 }  // namespace sph

@@ -1,9 +1,11 @@
 // Simulation context: owns two sorted particle sets (ping-pong, as in
-// opt_src_v0/sph/sph_run.cu:269-309) and the per-half-step work arrays, and
-// runs full predictor-corrector steps on one HIP stream, eagerly or from a
-// captured hipGraph.  Nothing is allocated after sph_sim_create.
+// opt_src_v0/sph/sph_run.cu:269-309) and the per-half-step work arrays for K
+// dam-break instances (K = 1 for a single case), and runs whole
+// predictor-corrector steps on one HIP stream, eagerly or from a captured
+// hipGraph.  Nothing is allocated by a step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -13,7 +15,11 @@
 using namespace sph;
 
 struct sph_sim {
-  sph_params P{};
+  std::vector<sph_params> P;  // per instance (host copy)
+  std::vector<int> cell_off;  // per instance
+  InstTable T{};
+  int K = 1;
+  int dim = 3;
   int n = 0;
   int bits = 1;
   hipStream_t stream = nullptr;
@@ -23,7 +29,7 @@ struct sph_sim {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   long long steps = 0;
-  unsigned long long* probe = nullptr;
+  unsigned long long* probe = nullptr;  // 2K words (also 4 for pair statistics)
   std::vector<void*> allocs;
   // kernel timing (eager steps with tuning.flags bit 0)
   std::vector<std::string> tnames;
@@ -86,19 +92,17 @@ hipError_t timed(sph_sim* s, bool timing, const char* name, F&& launch) {
   return e;
 }
 
-hipError_t half_step(sph_sim* s, int from, int to, bool predictor, bool timing) {
-  const sph_params& P = s->P;
+// hash -> sort -> cell ranges + gather (set `from` into set `to`); returns the
+// work arrays with keys/idx pointing at the sorted result.
+hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) {
   const int n = s->n;
-  FastSet& in = s->set[from];
-  FastSet& out = s->set[to];
   FastWork w = s->w;
   hipError_t e;
-  e = timed(s, timing, "hash", [&] { return launch_hash_fast(P, in, w.keys, w.idx, n, s->stream); });
+  e = timed(s, timing, "hash", [&] { return launch_hash_fast(s->T, s->set[from], w.keys, w.idx, n, s->stream); });
   if (e != hipSuccess) return e;
   bool alt = false;
   e = timed(s, timing, "radix_sort", [&] {
-    return radix_sort_pairs(w.keys, w.idx, w.keys_alt, w.idx_alt, w.hist, w.totals, n, s->bits,
-                            s->stream, &alt);
+    return radix_sort_pairs(w.keys, w.idx, w.keys_alt, w.idx_alt, w.hist, w.totals, n, s->bits, s->stream, &alt);
   });
   if (e != hipSuccess) return e;
   if (alt) {
@@ -108,16 +112,27 @@ hipError_t half_step(sph_sim* s, int from, int to, bool predictor, bool timing) 
     w.idx_alt = s->w.idx;
   }
   e = timed(s, timing, "cells_gather", [&] {
-    return launch_cells_gather_fast(P, in, out, w.keys, w.idx, w.cellStart, w.cellEnd, n, s->stream);
+    return launch_cells_gather_fast(s->T, s->set[from], s->set[to], w.keys, w.idx, w.cellStart, w.cellEnd, n,
+                                     s->stream);
   });
-  if (e != hipSuccess) return e;
-  e = timed(s, timing, "nsA", [&] { return launch_nsA_fast(P, out, w, n, s->tun.nbr_block, s->stream); });
-  if (e != hipSuccess) return e;
-  e = timed(s, timing, "nsB", [&] { return launch_nsB_fast(P, out, w, n, s->tun.nbr_block, s->stream); });
-  if (e != hipSuccess) return e;
-  e = timed(s, timing, predictor ? "predictor" : "corrector",
-            [&] { return launch_pc_fast(P, predictor, out, w, w.idx, n, s->stream); });
+  *out = w;
   return e;
+}
+
+hipError_t rates_phase(sph_sim* s, int set, const FastWork& w, bool timing) {
+  hipError_t e = timed(s, timing, "nsA", [&] { return launch_nsA_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
+  if (e != hipSuccess) return e;
+  return timed(s, timing, "nsB", [&] { return launch_nsB_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
+}
+
+hipError_t half_step(sph_sim* s, int from, int to, bool predictor, bool timing) {
+  FastWork w;
+  hipError_t e = sort_phase(s, from, to, timing, &w);
+  if (e != hipSuccess) return e;
+  e = rates_phase(s, to, w, timing);
+  if (e != hipSuccess) return e;
+  return timed(s, timing, predictor ? "predictor" : "corrector",
+               [&] { return launch_pc_fast(s->T, predictor, s->set[to], w, w.idx, s->n, s->stream); });
 }
 
 hipError_t full_step(sph_sim* s, bool timing) {
@@ -126,35 +141,71 @@ hipError_t full_step(sph_sim* s, bool timing) {
   return half_step(s, 1, 0, false, timing);
 }
 
-void release(sph_sim* s) {
+void drop_graph(sph_sim* s) {
   if (s->exec) hipGraphExecDestroy(s->exec);
   if (s->graph) hipGraphDestroy(s->graph);
+  s->exec = nullptr;
+  s->graph = nullptr;
+}
+
+void release(sph_sim* s) {
+  drop_graph(s);
   for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
   for (void* p : s->allocs) hipFree(p);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
 }
 
+int check_instance(const sph_params& p, int dim) {
+  if (p.dim != dim) return set_error(SPH_E_INVAL, "batched instances must share dim");
+  return SPH_OK;
+}
+
 }  // namespace
 
-extern "C" int sph_sim_create(const sph_params* p, int n, const float* pos, const float* vel,
-                              const float* rho, const float* prs, const int* type, sph_sim** out) {
+extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* counts, const float* pos,
+                                    const float* vel, const float* rho, const float* prs, const int* type,
+                                    sph_sim** out) {
   if (!out) return set_error(SPH_E_INVAL, "sph_sim_create: null output");
   *out = nullptr;
-  if (!p || n <= 0 || !pos || !vel || !rho || !prs || !type)
+  if (!params || K <= 0 || K > (1 << 23) || !counts || !pos || !vel || !rho || !prs || !type)
     return set_error(SPH_E_INVAL, "sph_sim_create: bad argument");
-  {
-    sph_params tmp = *p;
-    int rc = sph_set_params(&tmp);  // validates; also makes the reference surface usable
+  long long ntot = 0, ctot = 0;
+  for (int k = 0; k < K; ++k) {
+    sph_params tmp = params[k];
+    int rc = sph_set_params(&tmp);  // validates each instance
     if (rc) return rc;
+    rc = check_instance(params[k], params[0].dim);
+    if (rc) return rc;
+    if (counts[k] <= 0) return set_error(SPH_E_INVAL, "sph_sim_create: every instance needs particles");
+    ntot += counts[k];
+    ctot += params[k].numCells;
   }
+  if (ntot > 0x7fffffffLL || ctot > (1LL << 30))
+    return set_error(SPH_E_INVAL, "sph_sim_create: too many particles or cells");
+  const size_t N = (size_t)ntot;
+  std::vector<int> inst(N);
+  for (int k = 0, o = 0; k < K; ++k)
+    for (int j = 0; j < counts[k]; ++j) inst[o++] = k;
+  for (size_t i = 0; i < N; ++i)
+    if (type[i] < 0 || type[i] > 255) return set_error(SPH_E_INVAL, "sph_sim_create: types must lie in 0..255");
+
   sph_sim* s = new sph_sim();
-  s->P = *p;
-  s->P.numParticles = n;
-  s->n = n;
-  s->bits = key_bits(p->numCells);
+  s->K = K;
+  s->dim = params[0].dim;
+  s->n = (int)ntot;
+  s->P.assign(params, params + K);
+  s->cell_off.resize(K);
+  for (int k = 0, c = 0; k < K; ++k) {
+    s->cell_off[k] = c;
+    s->P[k].numParticles = counts[k];
+    c += params[k].numCells;
+  }
+  s->bits = key_bits((int)ctot);
   s->tun.nbr_block = 256;
-  s->tun.nbr_variant = 0;
+  s->tun.nbr_unroll = 4;
+  s->tun.xcd_remap = 1;
+  s->tun.flags = 0;
   int rc = SPH_OK;
   hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (he != hipSuccess) {
@@ -162,8 +213,7 @@ extern "C" int sph_sim_create(const sph_params* p, int n, const float* pos, cons
     release(s);
     return rc;
   }
-  const size_t N = (size_t)n;
-  const bool d3 = p->dim == 3;
+  const bool d3 = s->dim == 3;
   for (int k = 0; k < 2 && !rc; ++k) {
     FastSet& f = s->set[k];
     rc = rc ? rc : dalloc(s, &f.PR, N);
@@ -179,44 +229,55 @@ extern "C" int sph_sim_create(const sph_params* p, int n, const float* pos, cons
   rc = rc ? rc : dalloc(s, &w.idx, N);
   rc = rc ? rc : dalloc(s, &w.keys_alt, N);
   rc = rc ? rc : dalloc(s, &w.idx_alt, N);
-  rc = rc ? rc : dalloc(s, &w.hist, (size_t)radix_hist_words(n));
+  rc = rc ? rc : dalloc(s, &w.hist, (size_t)radix_hist_words(s->n));
   rc = rc ? rc : dalloc(s, &w.totals, 256);
-  rc = rc ? rc : dalloc(s, &w.cellStart, (size_t)p->numCells);
-  rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)p->numCells);
+  rc = rc ? rc : dalloc(s, &w.cellStart, (size_t)ctot);
+  rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)ctot);
   rc = rc ? rc : dalloc(s, &w.DV, N);
   rc = rc ? rc : dalloc(s, &w.RATE, N);
   rc = rc ? rc : dalloc(s, &w.TX[0], N);
   if (d3) rc = rc ? rc : dalloc(s, &w.TX[1], N);
   rc = rc ? rc : dalloc(s, &w.TX[2], N);
   rc = rc ? rc : dalloc(s, &w.TV, N);
-  rc = rc ? rc : dalloc(s, &s->probe, 4);
+  rc = rc ? rc : dalloc(s, &s->probe, (size_t)std::max(2 * K, 4));
+  sph_params* dP = nullptr;
+  int* dC = nullptr;
+  rc = rc ? rc : dalloc(s, &dP, (size_t)K);
+  rc = rc ? rc : dalloc(s, &dC, (size_t)K);
   if (rc) {
     release(s);
     return rc;
   }
-  // stage the host arrays through device scratch (the sort buffers), widen on device
-  float* dpos = nullptr;
-  float* dvel = nullptr;
-  float* drho = nullptr;
-  float* dprs = nullptr;
-  int* dtype = nullptr;
-  he = hipMalloc(&dpos, N * 3 * sizeof(float));
+  s->T.P = dP;
+  s->T.cell_off = dC;
+  s->T.K = K;
+  s->T.dim = s->dim;
+  s->T.total_cells = (int)ctot;
+  // stage the host arrays in temporary device buffers, widen on device
+  float *dpos = nullptr, *dvel = nullptr, *drho = nullptr, *dprs = nullptr;
+  int *dtype = nullptr, *dinst = nullptr;
+  he = hipMemcpy(dP, s->P.data(), sizeof(sph_params) * K, hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(dC, s->cell_off.data(), sizeof(int) * K, hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMalloc(&dpos, N * 3 * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&dvel, N * 3 * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&drho, N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&dprs, N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&dtype, N * sizeof(int));
+  if (he == hipSuccess) he = hipMalloc(&dinst, N * sizeof(int));
   if (he == hipSuccess) he = hipMemcpy(dpos, pos, N * 3 * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(dvel, vel, N * 3 * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(drho, rho, N * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(dprs, prs, N * sizeof(float), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(dtype, type, N * sizeof(int), hipMemcpyHostToDevice);
-  if (he == hipSuccess) he = launch_upload_fast(s->P, s->set[0], dpos, dvel, drho, dprs, dtype, n, s->stream);
+  if (he == hipSuccess) he = hipMemcpy(dinst, inst.data(), N * sizeof(int), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = launch_upload_fast(s->dim, s->set[0], dpos, dvel, drho, dprs, dtype, dinst, s->n, s->stream);
   if (he == hipSuccess) he = hipStreamSynchronize(s->stream);
   hipFree(dpos);
   hipFree(dvel);
   hipFree(drho);
   hipFree(dprs);
   hipFree(dtype);
+  hipFree(dinst);
   if (he != hipSuccess) {
     rc = hip_status(he, "sph_sim_create: upload");
     release(s);
@@ -226,10 +287,24 @@ extern "C" int sph_sim_create(const sph_params* p, int n, const float* pos, cons
   return SPH_OK;
 }
 
+extern "C" int sph_sim_create(const sph_params* p, int n, const float* pos, const float* vel, const float* rho,
+                              const float* prs, const int* type, sph_sim** out) {
+  if (!out) return set_error(SPH_E_INVAL, "sph_sim_create: null output");
+  *out = nullptr;
+  if (!p || n <= 0) return set_error(SPH_E_INVAL, "sph_sim_create: bad argument");
+  return sph_sim_create_batch(p, 1, &n, pos, vel, rho, prs, type, out);
+}
+
 extern "C" int sph_sim_destroy(sph_sim* s) {
   if (!s) return SPH_OK;
   hipStreamSynchronize(s->stream);
   release(s);
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_num_instances(sph_sim* s, int* K) {
+  if (!s || !K) return set_error(SPH_E_INVAL, "null argument");
+  *K = s->K;
   return SPH_OK;
 }
 
@@ -243,13 +318,15 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
   if (!s || !t) return set_error(SPH_E_INVAL, "null argument");
   if (t->nbr_block < 64 || t->nbr_block > 256 || (t->nbr_block % 64) != 0)
     return set_error(SPH_E_INVAL, "nbr_block must be 64, 128, 192 or 256");
+  if (t->nbr_unroll != 1 && t->nbr_unroll != 2 && t->nbr_unroll != 4 && t->nbr_unroll != 8)
+    return set_error(SPH_E_INVAL, "nbr_unroll must be 1, 2, 4 or 8");
+  const bool shape_changed = t->nbr_block != s->tun.nbr_block || t->nbr_unroll != s->tun.nbr_unroll ||
+                             (t->xcd_remap != 0) != (s->tun.xcd_remap != 0);
   s->tun = *t;
-  if (s->exec) {  // a captured graph holds the old launch shapes
-    hipGraphExecDestroy(s->exec);
-    hipGraphDestroy(s->graph);
-    s->exec = nullptr;
-    s->graph = nullptr;
-  }
+  s->tun.xcd_remap = t->xcd_remap ? 1 : 0;
+  if (shape_changed) drop_graph(s);  // a captured graph holds the old launch shapes
+  std::fill(s->tsum_ms.begin(), s->tsum_ms.end(), 0.0);
+  std::fill(s->tcount.begin(), s->tcount.end(), 0);
   return SPH_OK;
 }
 
@@ -278,12 +355,7 @@ extern "C" int sph_sim_step(sph_sim* s, int nsteps) {
 
 extern "C" int sph_sim_capture(sph_sim* s) {
   if (!s) return set_error(SPH_E_INVAL, "null sim");
-  if (s->exec) {
-    hipGraphExecDestroy(s->exec);
-    hipGraphDestroy(s->graph);
-    s->exec = nullptr;
-    s->graph = nullptr;
-  }
+  drop_graph(s);
   SPH_HIP(hipStreamSynchronize(s->stream), "capture sync");
   SPH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   hipError_t e = full_step(s, false);
@@ -355,23 +427,33 @@ extern "C" int sph_sim_download_pos64(sph_sim* s, double* pos) {
   return SPH_OK;
 }
 
-extern "C" int sph_sim_probe(sph_sim* s, double band, double* front, double* height) {
+extern "C" int sph_sim_probe_batch(sph_sim* s, double band, double* front, double* height) {
   if (!s || !front || !height) return set_error(SPH_E_INVAL, "null argument");
-  SPH_HIP(hipMemsetAsync(s->probe, 0, 2 * sizeof(unsigned long long), s->stream), "probe clear");
-  SPH_HIP(launch_probe_fast(s->P, s->set[0], band, s->probe, s->n, s->stream), "probe");
-  unsigned long long hv[2];
-  SPH_HIP(hipMemcpyAsync(hv, s->probe, sizeof(hv), hipMemcpyDeviceToHost, s->stream), "probe copy");
+  const size_t words = 2 * (size_t)s->K;
+  SPH_HIP(hipMemsetAsync(s->probe, 0, words * sizeof(unsigned long long), s->stream), "probe clear");
+  SPH_HIP(launch_probe_fast(s->T, s->set[0], band, s->probe, s->n, s->stream), "probe");
+  std::vector<unsigned long long> hv(words);
+  SPH_HIP(hipMemcpyAsync(hv.data(), s->probe, words * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream),
+          "probe copy");
   SPH_HIP(hipStreamSynchronize(s->stream), "probe sync");
-  double f, h;
-  std::memcpy(&f, &hv[0], 8);
-  std::memcpy(&h, &hv[1], 8);
-  *front = f;
-  *height = h;
+  for (int k = 0; k < s->K; ++k) {
+    std::memcpy(&front[k], &hv[2 * k], 8);
+    std::memcpy(&height[k], &hv[2 * k + 1], 8);
+  }
   return SPH_OK;
 }
 
-extern "C" int sph_sim_kernel_times(sph_sim* s, char* names, int names_cap, double* us, int* count,
-                                    int cap) {
+extern "C" int sph_sim_probe(sph_sim* s, double band, double* front, double* height) {
+  if (!s || !front || !height) return set_error(SPH_E_INVAL, "null argument");
+  std::vector<double> f(s->K), h(s->K);
+  int rc = sph_sim_probe_batch(s, band, f.data(), h.data());
+  if (rc) return rc;
+  *front = *std::max_element(f.begin(), f.end());
+  *height = *std::max_element(h.begin(), h.end());
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_kernel_times(sph_sim* s, char* names, int names_cap, double* us, int* count, int cap) {
   if (!s || !count) return set_error(SPH_E_INVAL, "null argument");
   std::string all;
   int k = 0;
@@ -388,25 +470,14 @@ extern "C" int sph_sim_kernel_times(sph_sim* s, char* names, int names_cap, doub
   return SPH_OK;
 }
 
-extern "C" int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, float* p, float* rho,
-                            int* keys, int* perm) {
+extern "C" int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, float* p, float* rho, int* keys,
+                            int* perm) {
   if (!s) return set_error(SPH_E_INVAL, "null sim");
-  const sph_params& P = s->P;
   const int n = s->n;
   const size_t N = (size_t)n;
-  FastWork w = s->w;
-  SPH_HIP(launch_hash_fast(P, s->set[0], w.keys, w.idx, n, s->stream), "eval hash");
-  bool alt = false;
-  SPH_HIP(radix_sort_pairs(w.keys, w.idx, w.keys_alt, w.idx_alt, w.hist, w.totals, n, s->bits, s->stream, &alt),
-          "eval sort");
-  if (alt) {
-    w.keys = s->w.keys_alt;
-    w.idx = s->w.idx_alt;
-  }
-  SPH_HIP(launch_cells_gather_fast(P, s->set[0], s->set[1], w.keys, w.idx, w.cellStart, w.cellEnd, n, s->stream),
-          "eval gather");
-  SPH_HIP(launch_nsA_fast(P, s->set[1], w, n, s->tun.nbr_block, s->stream), "eval nsA");
-  SPH_HIP(launch_nsB_fast(P, s->set[1], w, n, s->tun.nbr_block, s->stream), "eval nsB");
+  FastWork w;
+  SPH_HIP(sort_phase(s, 0, 1, false, &w), "eval sort");
+  SPH_HIP(rates_phase(s, 1, w, false), "eval rates");
   float *ddr = nullptr, *dacc = nullptr, *ddv = nullptr, *dp = nullptr, *drh = nullptr;
   hipError_t e = hipMalloc(&ddr, N * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&dacc, 3 * N * sizeof(float));
@@ -414,8 +485,7 @@ extern "C" int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, fl
   if (e == hipSuccess) e = hipMalloc(&dp, N * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&drh, N * sizeof(float));
   if (e == hipSuccess) e = launch_download_rates(s->set[1], w, ddr, dacc, ddv, n, s->stream);
-  if (e == hipSuccess)
-    e = launch_download_fast(s->set[1], nullptr, nullptr, drh, dp, nullptr, nullptr, n, s->stream);
+  if (e == hipSuccess) e = launch_download_fast(s->set[1], nullptr, nullptr, drh, dp, nullptr, nullptr, n, s->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e == hipSuccess && drho) e = hipMemcpy(drho, ddr, N * sizeof(float), hipMemcpyDeviceToHost);
   if (e == hipSuccess && acc) e = hipMemcpy(acc, dacc, 3 * N * sizeof(float), hipMemcpyDeviceToHost);
@@ -436,21 +506,10 @@ extern "C" int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, fl
 extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA, long long* candB,
                                   long long* hitsB) {
   if (!s) return set_error(SPH_E_INVAL, "null sim");
-  const sph_params& P = s->P;
-  const int n = s->n;
-  FastWork w = s->w;
-  SPH_HIP(launch_hash_fast(P, s->set[0], w.keys, w.idx, n, s->stream), "stats hash");
-  bool alt = false;
-  SPH_HIP(radix_sort_pairs(w.keys, w.idx, w.keys_alt, w.idx_alt, w.hist, w.totals, n, s->bits, s->stream, &alt),
-          "stats sort");
-  if (alt) {
-    w.keys = s->w.keys_alt;
-    w.idx = s->w.idx_alt;
-  }
-  SPH_HIP(launch_cells_gather_fast(P, s->set[0], s->set[1], w.keys, w.idx, w.cellStart, w.cellEnd, n, s->stream),
-          "stats gather");
+  FastWork w;
+  SPH_HIP(sort_phase(s, 0, 1, false, &w), "stats sort");
   SPH_HIP(hipMemsetAsync(s->probe, 0, 4 * sizeof(unsigned long long), s->stream), "stats clear");
-  SPH_HIP(launch_pair_stats(P, s->set[1], w, s->probe, n, s->stream), "stats");
+  SPH_HIP(launch_pair_stats(s->T, s->set[1], w, s->probe, s->n, s->stream), "stats");
   unsigned long long hv[4];
   SPH_HIP(hipMemcpyAsync(hv, s->probe, sizeof(hv), hipMemcpyDeviceToHost, s->stream), "stats copy");
   SPH_HIP(hipStreamSynchronize(s->stream), "stats sync");
@@ -458,6 +517,60 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
   if (hitsA) *hitsA = (long long)hv[1];
   if (candB) *candB = (long long)hv[2];
   if (hitsB) *hitsB = (long long)hv[3];
+  return SPH_OK;
+}
+
+// Autotuner: times kernels A and B on the live state for every launch shape
+// (block 64/128/256 x unroll 1/2/4/8 x XCD remap on/off) with HIP events on
+// the context's stream and keeps the fastest.  The neighbour phase is
+// re-evaluated from the same sorted set each time, so the state is unchanged
+// and every candidate sees identical work.
+extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* best_us) {
+  if (!s) return set_error(SPH_E_INVAL, "null sim");
+  if (reps <= 0) reps = 3;
+  FastWork w;
+  SPH_HIP(sort_phase(s, 0, 1, false, &w), "autotune sort");
+  // kernel A rewrites wall p/rho of set 1 in place from fluid values only, so
+  // repeating it is idempotent
+  hipEvent_t a, b;
+  SPH_HIP(hipEventCreate(&a), "hipEventCreate");
+  SPH_HIP(hipEventCreate(&b), "hipEventCreate");
+  const sph_tuning saved = s->tun;
+  sph_tuning pick = saved;
+  double pick_us = 1e300;
+  const int blocks[] = {64, 128, 256};
+  const int unrolls[] = {1, 2, 4, 8};
+  hipError_t err = hipSuccess;
+  for (int remap = 0; remap < 2 && err == hipSuccess; ++remap)
+    for (int bl : blocks)
+      for (int un : unrolls) {
+        if (err != hipSuccess) break;
+        sph_tuning t = saved;
+        t.nbr_block = bl;
+        t.nbr_unroll = un;
+        t.xcd_remap = remap;
+        s->tun = t;
+        err = rates_phase(s, 1, w, false);  // warm-up
+        if (err == hipSuccess) err = hipEventRecord(a, s->stream);
+        for (int r = 0; r < reps && err == hipSuccess; ++r) err = rates_phase(s, 1, w, false);
+        if (err == hipSuccess) err = hipEventRecord(b, s->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(b);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+        const double us = 1000.0 * ms / reps;
+        if (err == hipSuccess && us < pick_us) {
+          pick_us = us;
+          pick = t;
+        }
+      }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  s->tun = saved;
+  SPH_HIP(err, "sph_sim_autotune");
+  int rc = sph_sim_set_tuning(s, &pick);
+  if (rc) return rc;
+  if (best) *best = s->tun;
+  if (best_us) *best_us = pick_us;
   return SPH_OK;
 }
 

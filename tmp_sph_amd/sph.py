@@ -226,18 +226,32 @@ def _hp(a: np.ndarray | None) -> C.c_void_p:
 
 
 class Simulation:
-    """A dam-break simulation on the current device (sorted SoA state, FP64 positions)."""
+    """Dam-break simulation(s) on the current device (sorted SoA state, FP64 positions).
 
-    def __init__(self, case: Case):
+    ``Simulation(case)`` runs one case; ``Simulation([case0, case1, ...])``
+    runs the cases as one batch (one launch per phase for all of them), each
+    with its own parameters and time step.  Arrays returned by :meth:`state`
+    hold the cases back to back; :attr:`offsets` gives their start indices.
+    """
+
+    def __init__(self, cases: "Case | list[Case]"):
         load()
-        self.params = case.params.copy()
-        self.n = case.n
-        self.n_fluid = case.n_fluid
+        batch = list(cases) if isinstance(cases, (list, tuple)) else [cases]
+        if not batch:
+            raise ValueError("no cases")
+        self.cases_params = [c.params.copy() for c in batch]
+        self.params = self.cases_params[0]
+        counts = np.array([c.n for c in batch], np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.n = int(counts.sum())
+        self.n_fluid = int(sum(c.n_fluid for c in batch))
+        self.K = len(batch)
+        cat = lambda f, d: np.ascontiguousarray(np.concatenate([getattr(c, f) for c in batch]), d)  # noqa: E731
+        prm = (SphParams * self.K)(*self.cases_params)
         h = C.c_void_p()
-        call("sph_sim_create", C.byref(self.params), self.n, _hp(np.ascontiguousarray(case.pos, np.float32)),
-             _hp(np.ascontiguousarray(case.vel, np.float32)), _hp(np.ascontiguousarray(case.rho, np.float32)),
-             _hp(np.ascontiguousarray(case.p, np.float32)), _hp(np.ascontiguousarray(case.type, np.int32)),
-             C.byref(h))
+        call("sph_sim_create_batch", prm, self.K, counts.ctypes.data_as(C.POINTER(C.c_int)),
+             _hp(cat("pos", np.float32)), _hp(cat("vel", np.float32)), _hp(cat("rho", np.float32)),
+             _hp(cat("p", np.float32)), _hp(cat("type", np.int32)), C.byref(h))
         self._h = h
 
     def close(self) -> None:
@@ -297,9 +311,24 @@ class Simulation:
         return out
 
     def probe(self, band: float) -> tuple[float, float]:
+        """Surge front (max x of in-domain fluid) and column height (max z of
+        fluid with x <= band); the maxima over all instances of a batch."""
         f, h = C.c_double(), C.c_double()
         call("sph_sim_probe", self._h, C.c_double(band), C.byref(f), C.byref(h))
         return f.value, h.value
+
+    def probe_batch(self, band: float) -> tuple[np.ndarray, np.ndarray]:
+        f = np.zeros(self.K)
+        h = np.zeros(self.K)
+        call("sph_sim_probe_batch", self._h, C.c_double(band), _hp(f), _hp(h))
+        return f, h
+
+    def autotune(self, reps: int = 3) -> tuple[dict, float]:
+        """Picks the fastest neighbour-kernel launch shape for the current state."""
+        t = SphTuning()
+        us = C.c_double()
+        call("sph_sim_autotune", self._h, int(reps), C.byref(t), C.byref(us))
+        return t.as_dict(), us.value
 
     def eval_rates(self) -> dict:
         n = self.n
