@@ -254,6 +254,64 @@ int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, float* p, flo
 int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA, long long* candB,
                        long long* hitsB);
 
+/* --------------------------------------------- learned surrogate (no reference counterpart)
+   A message-passing particle network (encoder, `layers` processor blocks of
+   edge and node MLPs with residual updates, decoder; every MLP has two hidden
+   layers of width 128) run and trained with BF16 MFMA kernels that accumulate
+   in FP32.  Nodes are the particles, edges the pairs within the SPH support
+   radius 2h.  Node inputs: the last `history` velocities (x, z), distances to
+   the left, right and bottom walls (clipped at 2h), the fluid flag and the
+   density deviation; outputs: the velocity and density change over k SPH
+   steps, normalised by dv_scale and drho_scale.  2-D (x-z plane) only. */
+typedef struct sph_gnn sph_gnn;
+
+typedef struct sph_gnn_cfg {
+    int layers;          /* message-passing blocks (10) */
+    int history;         /* velocity history length (5) */
+    int max_nodes;       /* capacity */
+    int max_edges;       /* capacity */
+    int train;           /* 1: keep per-layer activations for backward */
+    unsigned seed;       /* weight initialisation */
+    float v_scale;       /* velocity normalisation */
+    float dv_scale;      /* output velocity-change normalisation */
+    float drho_scale;    /* output density-change normalisation */
+    float rho_scale;     /* input density deviation scale ((rho/rho0 - 1) * rho_scale) */
+} sph_gnn_cfg;
+
+int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out);
+int sph_gnn_destroy(sph_gnn* g);
+int sph_gnn_num_params(sph_gnn* g, long long* count);
+/* FP32 master parameters (flat, in the layout of sph_gnn_param_layout) */
+int sph_gnn_get_params(sph_gnn* g, float* host);
+int sph_gnn_set_params(sph_gnn* g, const float* host);
+int sph_gnn_get_grads(sph_gnn* g, float* host);
+/* Layout of the flat parameter vector: for every linear layer (in the order
+   node encoder, edge encoder, [edge MLP, node MLP] x layers, decoder; three
+   layers per MLP) the five ints {weight offset, N, K, bias offset, 0};
+   W is N x K row-major, b has N entries.  *nlayers receives the count. */
+int sph_gnn_param_layout(sph_gnn* g, int* layout5, int cap, int* nlayers);
+/* Loads one particle state (host arrays, original order): pos AoS[3n],
+   hist[history][n][3] newest first, rho[n], type[n]; target[n][3] (normalised
+   output) may be null.  Builds the radius graph on the device. */
+int sph_gnn_load_state(sph_gnn* g, const sph_params* p, int n, const float* pos, const float* hist,
+                       const float* rho, const int* type, const float* target);
+/* Graph of the loaded state: edge count and, if the pointers are not null,
+   CSR offsets[n+1], senders[e] and receivers[e]. */
+int sph_gnn_graph(sph_gnn* g, int* edges, int* offsets, int* senders, int* receivers);
+/* Node and edge input features (128 wide, FP32) of the loaded state. */
+int sph_gnn_features(sph_gnn* g, float* node_feat, float* edge_feat);
+/* Forward pass; out[n][3] (normalised outputs) may be null; *loss gets the
+   mean squared error over fluid nodes when a target was loaded. */
+int sph_gnn_forward(sph_gnn* g, float* out, double* loss);
+/* Backward pass of the last forward (needs train = 1); accumulates into the gradients. */
+int sph_gnn_backward(sph_gnn* g);
+int sph_gnn_zero_grad(sph_gnn* g);
+/* Adam step (beta 0.9 / 0.999, eps 1e-8) with global-norm clipping at clip (<= 0: none). */
+int sph_gnn_adam(sph_gnn* g, float lr, float clip);
+/* One MFMA tile D(16x16) = A(16x32) B(32x16) in BF16 (host arrays as raw
+   uint16 BF16 bits), for checking the operand layout. */
+int sph_gnn_mfma_probe(const unsigned short* A, const unsigned short* B, float* D);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,7 @@ import subprocess
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SRC = HERE / "sph_oracle.cpp"
+SRCS = [HERE / "sph_oracle.cpp", HERE / "gnn_oracle.cpp"]
 LIB = HERE / "liboracle_sph.so"
 
 FLAGS = ["-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
@@ -24,7 +24,7 @@ FLAGS = ["-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math", "-fop
 
 def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = ["g++", *FLAGS, str(SRC), "-o", str(tmp)]
+    cmd = ["g++", *FLAGS, *map(str, SRCS), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

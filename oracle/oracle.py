@@ -46,6 +46,7 @@ def load() -> C.CDLL:
         "orc_eos_density64": ([sp, C.c_double], C.c_double),
         "orc_run": ([sp, I, I] + [P] * 6 + [I, C.c_double, I, P, P, P], I),
         "orc_eval": ([sp, I, I] + [P] * 15, None),
+        "orc_gnn_loss": ([I, P, P, I, I, I] + [P] * 7 + [I, P], C.c_double),
     }
     for name, (args, res) in protos.items():
         f = getattr(lib, name)
@@ -154,6 +155,21 @@ def run(prm, case, nsteps: int, mode: int = 1, band: float = 0.0, probe_every: i
                        int(nsteps), C.c_double(band), int(probe_every), _hp(front), _hp(height), C.byref(secs))
     state = {"pos": pos, "pos64": pos64, "vel": vel, "rho": rho, "p": p, "type": typ}
     return state, front[:k], height[:k], secs.value
+
+
+def gnn_loss(layers: int, params: np.ndarray, layout: np.ndarray, Xn, Xe, off, send, recv, target, typ,
+             bf16: bool = False, grad: bool = True):
+    """Loss (and FP64 gradient) of the message-passing network on a given graph."""
+    params = np.ascontiguousarray(params, np.float32)
+    layout = np.ascontiguousarray(layout, np.int32)
+    G = np.zeros(params.size, np.float64) if grad else None
+    n, ne = Xn.shape[0], Xe.shape[0]
+    loss = load().orc_gnn_loss(int(layers), _hp(params), _hp(layout), layout.size // 5, n, ne,
+                               _hp(np.ascontiguousarray(Xn, np.float32)), _hp(np.ascontiguousarray(Xe, np.float32)),
+                               _hp(np.ascontiguousarray(off, np.int32)), _hp(np.ascontiguousarray(send, np.int32)),
+                               _hp(np.ascontiguousarray(recv, np.int32)), _hp(np.ascontiguousarray(target, np.float32)),
+                               _hp(np.ascontiguousarray(typ, np.int32)), 1 if bf16 else 0, _hp(G))
+    return loss, G
 
 
 def eval_rates(prm, case, mode: int = 1) -> dict:

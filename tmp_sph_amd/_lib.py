@@ -152,6 +152,21 @@ _PROTOS = {
     "sph_sim_num_instances": [P, IP],
     "sph_sim_probe_batch": [P, C.c_double, DP, DP],
     "sph_sim_autotune": [P, I, C.POINTER(SphTuning), DP],
+    "sph_gnn_create": [P, C.POINTER(C.c_void_p)],
+    "sph_gnn_destroy": [P],
+    "sph_gnn_num_params": [P, C.POINTER(C.c_longlong)],
+    "sph_gnn_get_params": [P, P],
+    "sph_gnn_set_params": [P, P],
+    "sph_gnn_get_grads": [P, P],
+    "sph_gnn_param_layout": [P, P, I, IP],
+    "sph_gnn_load_state": [P, C.POINTER(SphParams), I, P, P, P, P, P],
+    "sph_gnn_graph": [P, IP, P, P, P],
+    "sph_gnn_features": [P, P, P],
+    "sph_gnn_forward": [P, P, DP],
+    "sph_gnn_backward": [P],
+    "sph_gnn_zero_grad": [P],
+    "sph_gnn_adam": [P, C.c_float, C.c_float],
+    "sph_gnn_mfma_probe": [P, P, P],
 }
 
 EXPORTS = ["sph_last_error", *_PROTOS]

@@ -1,0 +1,860 @@
+// Message-passing particle surrogate: model, graph construction, forward,
+// backward and Adam, all on the device with the kernels of gnn_kernels.hip.
+//
+// Graph: nodes are particles in their original order; edges join every pair
+// closer than the SPH support (the pair set of the reference's kernels, built
+// from the same cell hash + radix sort); edges are stored in CSR by receiver
+// with both directions present, and rev[e] is the reverse edge of e, so the
+// sender-side gradient is a segment sum too (no atomics, deterministic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gnn_kernels.h"
+#include "sph_kernels.h"
+
+using namespace sph;
+using namespace sph::gnn;
+
+namespace {
+
+constexpr int W = 128;  // latent / hidden width
+
+struct Lin {
+  int N, K;
+  size_t w, b;  // offsets into the FP32 parameter vector
+  size_t wb;    // offset into the packed BF16 weight buffers
+};
+
+struct Mlp {
+  Lin l[3];
+  int kin;
+};
+
+inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace
+
+struct sph_gnn {
+  sph_gnn_cfg cfg{};
+  hipStream_t stream = nullptr;
+  int L = 10;
+  std::vector<void*> allocs;
+  // parameters
+  std::vector<Lin> lins;
+  Mlp enc_n{}, enc_e{}, dec{};
+  std::vector<Mlp> pe, pn;
+  size_t nparam = 0, nbf = 0;
+  float *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr;
+  bf16 *Wb = nullptr, *Wt = nullptr;
+  int adam_t = 0;
+  bool packed = false;
+  // loaded state / graph
+  sph_params prm{};
+  int n = 0, ne = 0, nfluid = 0;
+  bool has_target = false;
+  int *off = nullptr, *recv = nullptr, *send = nullptr, *rev = nullptr;
+  int *keys = nullptr, *idx = nullptr, *keys2 = nullptr, *idx2 = nullptr, *inv = nullptr, *deg = nullptr;
+  unsigned *hist_sort = nullptr, *totals = nullptr;
+  int *cellStart = nullptr, *cellEnd = nullptr;
+  int cells_cap = 0;
+  int* scan_blocks = nullptr;
+  float *pos = nullptr, *vhist = nullptr, *rho = nullptr, *target = nullptr;
+  int* type = nullptr;
+  // activations
+  float *Xn = nullptr, *Xe = nullptr;
+  float *V = nullptr, *E = nullptr, *Agg = nullptr, *Y = nullptr;
+  bf16* H = nullptr;  // hidden activations of every MLP call (training) or one call (inference)
+  // backward scratch
+  float *dY = nullptr, *dV = nullptr, *dE = nullptr, *dX = nullptr, *dZf[2] = {nullptr, nullptr};
+  bf16 *dZb[2] = {nullptr, nullptr}, *Tz = nullptr, *Tx = nullptr;
+  float *part = nullptr, *colpart = nullptr, *scal = nullptr;
+  int split_cap = 64;
+};
+
+#define SPH_HIP(call, where)                            \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return hip_status(e_, where); \
+  } while (0)
+#define HIPCHK(call)                      \
+  do {                                    \
+    hipError_t e_ = (call);               \
+    if (e_ != hipSuccess) return e_;      \
+  } while (0)
+
+namespace {
+
+template <class T>
+int galloc(sph_gnn* g, T** p, size_t count) {
+  void* q = nullptr;
+  SPH_HIP(hipMalloc(&q, (count ? count : 1) * sizeof(T)), "sph_gnn_create: hipMalloc");
+  g->allocs.push_back(q);
+  *p = (T*)q;
+  return SPH_OK;
+}
+
+// ------------------------------------------------------------ graph kernels
+__global__ void cells_only_kernel(const int* __restrict__ keys, int* __restrict__ cs, int* __restrict__ ce, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int h = keys[i];
+  if (i == 0 || h != keys[i - 1]) {
+    cs[h] = i;
+    if (i > 0) ce[keys[i - 1]] = i;
+  }
+  if (i == n - 1) ce[h] = n;
+}
+
+__global__ void invert_kernel(const int* __restrict__ idx, int* __restrict__ inv, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) inv[idx[s]] = s;
+}
+
+// Visits the candidate sorted slots j of sorted slot s (2-D: 3 stencil rows).
+template <class F>
+__device__ __forceinline__ void for_pairs2d(const sph_params& P, int key, const int* __restrict__ cs,
+                                            const int* __restrict__ ce, F&& f) {
+  const int X = P.gridxdim, Z = P.gridzdim;
+  const int cz = key / X, cx = key - cz * X;
+  const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < X - 1 ? cx + 1 : X - 1;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= Z) continue;
+    int s = -1, e = -1;
+    for (int c = z * X + x0; c <= z * X + x1; ++c) {
+      const int st = cs[c];
+      if (st != -1) {
+        if (s < 0) s = st;
+        e = ce[c];
+      }
+    }
+    for (int j = (s < 0 ? 0 : s); j < (s < 0 ? 0 : e); ++j) f(j);
+  }
+}
+
+__global__ void degree_kernel(sph_params P, const float* __restrict__ pos, const int* __restrict__ keys,
+                              const int* __restrict__ idx, const int* __restrict__ cs, const int* __restrict__ ce,
+                              int* __restrict__ deg, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int p = idx[s];
+  const float px = pos[3 * p], pz = pos[3 * p + 2];
+  int d = 0;
+  for_pairs2d(P, keys[s], cs, ce, [&](int j) {
+    if (j == s) return;
+    const int q = idx[j];
+    const float dx = px - pos[3 * q], dz = pz - pos[3 * q + 2];
+    if (dx * dx + dz * dz <= P.rr_cut) ++d;
+  });
+  deg[p] = d;
+}
+
+__global__ void fill_edges_kernel(sph_params P, const float* __restrict__ pos, const int* __restrict__ keys,
+                                  const int* __restrict__ idx, const int* __restrict__ cs, const int* __restrict__ ce,
+                                  const int* __restrict__ off, int* __restrict__ send, int* __restrict__ recv, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int p = idx[s];
+  const float px = pos[3 * p], pz = pos[3 * p + 2];
+  int e = off[p];
+  for_pairs2d(P, keys[s], cs, ce, [&](int j) {
+    if (j == s) return;
+    const int q = idx[j];
+    const float dx = px - pos[3 * q], dz = pz - pos[3 * q + 2];
+    if (dx * dx + dz * dz <= P.rr_cut) {
+      send[e] = q;
+      recv[e] = p;
+      ++e;
+    }
+  });
+}
+
+// rev[e]: the edge q <- p for e = p <- q; senders of a segment ascend in
+// sorted slot, so a binary search on inv[] finds it.
+__global__ void reverse_kernel(const int* __restrict__ off, const int* __restrict__ send, const int* __restrict__ recv,
+                               const int* __restrict__ inv, int* __restrict__ rev, int ne) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int p = recv[e], q = send[e];
+  const int target = inv[p];
+  int lo = off[q], hi = off[q + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (inv[send[mid]] < target)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  rev[e] = lo;
+}
+
+// exclusive scan of deg[0..n) into off[0..n], 1024 elements per block
+__global__ void scan_block_kernel(const int* __restrict__ in, int* __restrict__ out, int* __restrict__ bsum, int n) {
+  __shared__ int s[1024];
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const int v = i < n ? in[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
+}
+
+__global__ void scan_sums_kernel(int* __restrict__ bsum, int nb, int* __restrict__ total_out) {
+  // one thread: nb is small (n / 1024)
+  if (threadIdx.x != 0) return;
+  int run = 0;
+  for (int b = 0; b < nb; ++b) {
+    const int v = bsum[b];
+    bsum[b] = run;
+    run += v;
+  }
+  *total_out = run;
+}
+
+__global__ void scan_add_kernel(int* __restrict__ out, const int* __restrict__ bsum, int n) {
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) out[i] += bsum[blockIdx.x];
+}
+
+// ------------------------------------------------------------ features
+__global__ void node_features_kernel(sph_params P, sph_gnn_cfg cfg, const float* __restrict__ pos,
+                                     const float* __restrict__ vhist, const float* __restrict__ rho,
+                                     const int* __restrict__ type, float* __restrict__ Xn, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float* x = Xn + (size_t)p * W;
+  const float inv_v = 1.0f / cfg.v_scale;
+  for (int t = 0; t < cfg.history; ++t) {
+    x[2 * t] = vhist[((size_t)t * n + p) * 3] * inv_v;
+    x[2 * t + 1] = vhist[((size_t)t * n + p) * 3 + 2] * inv_v;
+  }
+  const int b = 2 * cfg.history;
+  const float R = P.kh;
+  const float px = pos[3 * p], pz = pos[3 * p + 2];
+  x[b] = fminf(fmaxf((px - (float)P.domain_min[0]) / R, 0.0f), 1.0f);
+  x[b + 1] = fminf(fmaxf(((float)P.domain_max[0] - px) / R, 0.0f), 1.0f);
+  x[b + 2] = fminf(fmaxf((pz - (float)P.domain_min[2]) / R, 0.0f), 1.0f);
+  x[b + 3] = type[p] == 1 ? 1.0f : 0.0f;
+  x[b + 4] = (rho[p] / P.restDensity - 1.0f) * cfg.rho_scale;
+}
+
+__global__ void edge_features_kernel(sph_params P, const float* __restrict__ pos, const int* __restrict__ send,
+                                     const int* __restrict__ recv, float* __restrict__ Xe, int ne) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int p = recv[e], q = send[e];
+  const float R = P.kh;
+  const float dx = (pos[3 * p] - pos[3 * q]) / R, dz = (pos[3 * p + 2] - pos[3 * q + 2]) / R;
+  float* x = Xe + (size_t)e * W;
+  x[0] = dx;
+  x[1] = dz;
+  x[2] = sqrtf(dx * dx + dz * dz);
+}
+
+// loss over fluid nodes: mean of (y - t)^2 over 3 channels; dY = gradient
+__global__ void loss_kernel(const float* __restrict__ Y, const float* __restrict__ T, const int* __restrict__ type,
+                            float inv_count, float* __restrict__ dY, double* __restrict__ loss, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (p < n) {
+    float* d = dY + (size_t)p * W;
+    const bool fl = type[p] == 1;
+    for (int c = 0; c < 3; ++c) {
+      const float r = fl ? Y[(size_t)p * W + c] - T[(size_t)p * 3 + c] : 0.0f;
+      d[c] = 2.0f * r * inv_count;
+      l += (double)r * r;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, l * inv_count);
+}
+
+__global__ void gather_out_kernel(const float* __restrict__ Y, float* __restrict__ out, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int c = 0; c < 3; ++c) out[(size_t)p * 3 + c] = Y[(size_t)p * W + c];
+}
+
+inline unsigned nblk(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// ------------------------------------------------------------ model helpers
+ASeg segf(const float* p, int ld, int width, const int* rows = nullptr) { return ASeg{p, rows, ld, width, 1}; }
+ASeg segb(const bf16* p, int ld, int width, const int* rows = nullptr) { return ASeg{p, rows, ld, width, 0}; }
+AOperand op1(ASeg a) {
+  AOperand o{};
+  o.seg[0] = a;
+  o.nseg = 1;
+  return o;
+}
+AOperand op2(ASeg a, ASeg b) {
+  AOperand o = op1(a);
+  o.seg[1] = b;
+  o.nseg = 2;
+  return o;
+}
+AOperand op3(ASeg a, ASeg b, ASeg c) {
+  AOperand o = op2(a, b);
+  o.seg[2] = c;
+  o.nseg = 3;
+  return o;
+}
+
+Mlp make_mlp(sph_gnn* g, int kin) {
+  Mlp m{};
+  m.kin = kin;
+  const int ks[3] = {kin, W, W};
+  for (int i = 0; i < 3; ++i) {
+    Lin l{};
+    l.N = W;
+    l.K = ks[i];
+    l.w = g->nparam;
+    g->nparam += (size_t)l.N * l.K;
+    l.b = g->nparam;
+    g->nparam += (size_t)l.N;
+    l.wb = g->nbf;
+    g->nbf += (size_t)l.N * l.K;
+    m.l[i] = l;
+    g->lins.push_back(l);
+  }
+  return m;
+}
+
+hipError_t pack_all(sph_gnn* g) {
+  for (const Lin& l : g->lins) HIPCHK(pack_weights(g->P + l.w, l.N, l.K, g->Wb + l.wb, g->Wt + l.wb, g->stream));
+  g->packed = true;
+  return hipSuccess;
+}
+
+// out (f32, M x 128) = MLP(in) (+ resid); h1, h2 receive the hidden activations
+hipError_t mlp_forward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, bf16* h1, bf16* h2, float* out,
+                       const float* resid) {
+  GemmArgs a{};
+  a.A = in;
+  a.B = g->Wb + m.l[0].wb;
+  a.bias = g->P + m.l[0].b;
+  a.out_bf16 = h1;
+  a.M = M;
+  a.N = W;
+  a.K = m.kin;
+  a.ldo = W;
+  a.relu = 1;
+  a.ksplit = 1;
+  HIPCHK(gemm(a, g->stream));
+  a.A = op1(segb(h1, W, W));
+  a.B = g->Wb + m.l[1].wb;
+  a.bias = g->P + m.l[1].b;
+  a.out_bf16 = h2;
+  a.K = W;
+  HIPCHK(gemm(a, g->stream));
+  a.A = op1(segb(h2, W, W));
+  a.B = g->Wb + m.l[2].wb;
+  a.bias = g->P + m.l[2].b;
+  a.out_bf16 = nullptr;
+  a.out_f32 = out;
+  a.resid = resid;
+  a.relu = 0;
+  return gemm(a, g->stream);
+}
+
+// dW += dZ^T X and db += colsum(dZ) for one linear layer; dZ is M x 128
+// (FP32 for the bias sum, BF16 for the GEMM), X the layer input (width K).
+hipError_t weight_grad(sph_gnn* g, const float* dZf, const bf16* dZb, const AOperand& X, int M, const Lin& l) {
+  const int Mp = (int)round_up((size_t)M, 32);
+  // transposes with the reduction (row) dimension padded with zeros to Mp
+  HIPCHK(transpose_to_bf16(op1(segb(dZb, W, W)), M, W, g->Tz, Mp, g->stream));
+  HIPCHK(transpose_to_bf16(X, M, l.K, g->Tx, Mp, g->stream));
+  int ks = Mp / 2048;
+  if (ks < 1) ks = 1;
+  if (ks > g->split_cap) ks = g->split_cap;
+  GemmArgs a{};
+  a.A = op1(segb(g->Tz, Mp, Mp));
+  a.B = g->Tx;
+  a.out_f32 = g->part;
+  a.M = W;
+  a.N = l.K;
+  a.K = Mp;
+  a.ldo = l.K;
+  a.ksplit = ks;
+  HIPCHK(gemm(a, g->stream));
+  if (ks == 1) {
+    HIPCHK(axpy(g->part, g->G + l.w, (long long)W * l.K, 1.0f, g->stream));
+  } else {
+    HIPCHK(reduce_partials(g->part, ks, W * l.K, g->G + l.w, 1, g->stream));
+  }
+  return column_sum(dZf, M, W, W, g->G + l.b, g->colpart, 1, g->stream);
+}
+
+// Backward of one MLP call: dOut (f32, M x 128) -> parameter gradients and,
+// when dX is not null, dX (f32, M x kin) = gradient w.r.t. the input.
+hipError_t mlp_backward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, const bf16* h1, const bf16* h2,
+                        const float* dOut, float* dX) {
+  // the FP32 upstream gradient, rounded to BF16 for the transpose
+  HIPCHK(f32_to_bf16(dOut, g->dZb[1], (long long)M * W, g->stream));
+  HIPCHK(weight_grad(g, dOut, g->dZb[1], op1(segb(h2, W, W)), M, m.l[2]));
+  GemmArgs a{};
+  a.A = op1(segf(dOut, W, W));
+  a.B = g->Wt + m.l[2].wb;
+  a.mask = h2;
+  a.out_f32 = g->dZf[0];
+  a.out_bf16 = g->dZb[0];
+  a.M = M;
+  a.N = W;
+  a.K = W;
+  a.ldo = W;
+  a.ksplit = 1;
+  HIPCHK(gemm(a, g->stream));  // dZ2
+  HIPCHK(weight_grad(g, g->dZf[0], g->dZb[0], op1(segb(h1, W, W)), M, m.l[1]));
+  a.A = op1(segb(g->dZb[0], W, W));
+  a.B = g->Wt + m.l[1].wb;
+  a.mask = h1;
+  a.out_f32 = g->dZf[1];
+  a.out_bf16 = g->dZb[1];
+  HIPCHK(gemm(a, g->stream));  // dZ1
+  HIPCHK(weight_grad(g, g->dZf[1], g->dZb[1], in, M, m.l[0]));
+  if (!dX) return hipSuccess;
+  GemmArgs b{};
+  b.A = op1(segb(g->dZb[1], W, W));
+  b.B = g->Wt + m.l[0].wb;
+  b.out_f32 = dX;
+  b.M = M;
+  b.N = m.kin;
+  b.K = W;
+  b.ldo = m.kin;
+  b.ksplit = 1;
+  return gemm(b, g->stream);
+}
+
+// hidden-activation slots: call c of the forward pass uses H + c * slot
+size_t h_slot(const sph_gnn* g) { return (size_t)2 * W * (size_t)std::max(g->cfg.max_nodes, g->cfg.max_edges); }
+
+bf16* h1_of(sph_gnn* g, int call) { return g->H + (g->cfg.train ? (size_t)call : 0) * h_slot(g); }
+bf16* h2_of(sph_gnn* g, int call) { return h1_of(g, call) + (size_t)W * std::max(g->cfg.max_nodes, g->cfg.max_edges); }
+
+float* V_of(sph_gnn* g, int l) { return g->V + (g->cfg.train ? (size_t)l : 0) * (size_t)g->cfg.max_nodes * W; }
+float* E_of(sph_gnn* g, int l) { return g->E + (g->cfg.train ? (size_t)l : 0) * (size_t)g->cfg.max_edges * W; }
+float* A_of(sph_gnn* g, int l) { return g->Agg + (g->cfg.train ? (size_t)l : 0) * (size_t)g->cfg.max_nodes * W; }
+
+// Forward pass over the loaded graph.  With train = 0 the latents are updated
+// in place and only one slot of hidden activations is used.
+hipError_t forward(sph_gnn* g) {
+  const int n = g->n, ne = g->ne, L = g->L;
+  const bool tr = g->cfg.train != 0;
+  int call = 0;
+  HIPCHK(mlp_forward(g, g->enc_n, op1(segf(g->Xn, W, W)), n, h1_of(g, call), h2_of(g, call), V_of(g, 0), nullptr));
+  ++call;
+  HIPCHK(mlp_forward(g, g->enc_e, op1(segf(g->Xe, W, W)), ne, h1_of(g, call), h2_of(g, call), E_of(g, 0), nullptr));
+  ++call;
+  for (int l = 0; l < L; ++l) {
+    float* Vl = V_of(g, l);
+    float* El = E_of(g, l);
+    float* El1 = E_of(g, tr ? l + 1 : 0);
+    float* Vl1 = V_of(g, tr ? l + 1 : 0);
+    float* Al = A_of(g, l);
+    AOperand ein = op3(segf(El, W, W), segf(Vl, W, W, g->recv), segf(Vl, W, W, g->send));
+    HIPCHK(mlp_forward(g, g->pe[l], ein, ne, h1_of(g, call), h2_of(g, call), El1, El));
+    ++call;
+    HIPCHK(segment_sum(El1, W, g->off, nullptr, n, Al, W, 0, g->stream));
+    AOperand nin = op2(segf(Vl, W, W), segf(Al, W, W));
+    HIPCHK(mlp_forward(g, g->pn[l], nin, n, h1_of(g, call), h2_of(g, call), Vl1, Vl));
+    ++call;
+  }
+  return mlp_forward(g, g->dec, op1(segf(V_of(g, tr ? L : 0), W, W)), n, h1_of(g, call), h2_of(g, call), g->Y,
+                     nullptr);
+}
+
+hipError_t backward(sph_gnn* g) {
+  const int n = g->n, ne = g->ne, L = g->L;
+  int call = 2 + 2 * L;  // decoder call index
+  // decoder: dV_L = dMLP(dY)
+  HIPCHK(mlp_backward(g, g->dec, op1(segf(V_of(g, L), W, W)), n, h1_of(g, call), h2_of(g, call), g->dY, g->dV));
+  HIPCHK(fill_f32(g->dE, 0.0f, (long long)ne * W, g->stream));
+  for (int l = L - 1; l >= 0; --l) {
+    float* Vl = V_of(g, l);
+    float* El = E_of(g, l);
+    float* Al = A_of(g, l);
+    // node update: V_{l+1} = V_l + MLP_n([V_l, A_l]); dV holds dV_{l+1}
+    --call;
+    HIPCHK(mlp_backward(g, g->pn[l], op2(segf(Vl, W, W), segf(Al, W, W)), n, h1_of(g, call), h2_of(g, call), g->dV,
+                        g->dX));
+    // dV_l = dV_{l+1} + dX[:, 0:128];  dA_l = dX[:, 128:256] -> dE_{l+1} += dA_l[recv]
+    HIPCHK(broadcast_to_edges(g->dX + W, 2 * W, g->recv, ne, g->dE, W, 1, g->stream));
+    HIPCHK(hipMemcpy2DAsync(g->dZf[0], W * sizeof(float), g->dX, 2 * W * sizeof(float), W * sizeof(float), n,
+                            hipMemcpyDeviceToDevice, g->stream));
+    HIPCHK(axpy(g->dZf[0], g->dV, (long long)n * W, 1.0f, g->stream));
+    // edge update: E_{l+1} = E_l + MLP_e([E_l, V_l[recv], V_l[send]]); dE holds dE_{l+1}
+    --call;
+    AOperand ein = op3(segf(El, W, W), segf(Vl, W, W, g->recv), segf(Vl, W, W, g->send));
+    HIPCHK(mlp_backward(g, g->pe[l], ein, ne, h1_of(g, call), h2_of(g, call), g->dE, g->dX));
+    // dV_l += segsum(dX[:, 128:256]) + segsum over the reverse edges of dX[:, 256:384]
+    HIPCHK(segment_sum(g->dX + W, 3 * W, g->off, nullptr, n, g->dV, W, 1, g->stream));
+    HIPCHK(segment_sum(g->dX + 2 * W, 3 * W, g->off, g->rev, n, g->dV, W, 1, g->stream));
+    // dE_l = dE_{l+1} + dX[:, 0:128]
+    HIPCHK(hipMemcpy2DAsync(g->dZf[0], W * sizeof(float), g->dX, 3 * W * sizeof(float), W * sizeof(float), ne,
+                            hipMemcpyDeviceToDevice, g->stream));
+    HIPCHK(axpy(g->dZf[0], g->dE, (long long)ne * W, 1.0f, g->stream));
+  }
+  // encoders (inputs are features: no input gradient)
+  --call;
+  HIPCHK(mlp_backward(g, g->enc_e, op1(segf(g->Xe, W, W)), ne, h1_of(g, call), h2_of(g, call), g->dE, nullptr));
+  --call;
+  return mlp_backward(g, g->enc_n, op1(segf(g->Xn, W, W)), n, h1_of(g, call), h2_of(g, call), g->dV, nullptr);
+}
+
+void release(sph_gnn* g) {
+  for (void* p : g->allocs) hipFree(p);
+  if (g->stream) hipStreamDestroy(g->stream);
+  delete g;
+}
+
+// deterministic uniform in [-1, 1) from a 64-bit LCG
+struct Lcg {
+  unsigned long long s;
+  float next() {
+    s = s * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (float)((double)(s >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
+  }
+};
+
+}  // namespace
+
+extern "C" int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out) {
+  if (!out) return set_error(SPH_E_INVAL, "sph_gnn_create: null output");
+  *out = nullptr;
+  if (!cfg || cfg->layers < 1 || cfg->layers > 64 || cfg->history < 1 || 2 * cfg->history + 5 > W ||
+      cfg->max_nodes <= 0 || cfg->max_edges <= 0 || !(cfg->v_scale > 0) || !(cfg->dv_scale > 0) ||
+      !(cfg->drho_scale > 0))
+    return set_error(SPH_E_INVAL, "sph_gnn_create: bad configuration");
+  sph_gnn* g = new sph_gnn();
+  g->cfg = *cfg;
+  g->L = cfg->layers;
+  g->enc_n = make_mlp(g, W);
+  g->enc_e = make_mlp(g, W);
+  for (int l = 0; l < g->L; ++l) {
+    g->pe.push_back(make_mlp(g, 3 * W));
+    g->pn.push_back(make_mlp(g, 2 * W));
+  }
+  g->dec = make_mlp(g, W);
+  int rc = SPH_OK;
+  hipError_t he = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    rc = hip_status(he, "hipStreamCreate");
+    release(g);
+    return rc;
+  }
+  const size_t Nn = cfg->max_nodes, Ne = cfg->max_edges;
+  const size_t Mx = std::max(Nn, Ne);
+  const size_t Mp = round_up(Mx, 32);
+  const bool tr = cfg->train != 0;
+  const int calls = tr ? 3 + 2 * g->L : 1;
+  rc = rc ? rc : galloc(g, &g->P, g->nparam);
+  rc = rc ? rc : galloc(g, &g->G, g->nparam);
+  rc = rc ? rc : galloc(g, &g->M1, g->nparam);
+  rc = rc ? rc : galloc(g, &g->M2, g->nparam);
+  rc = rc ? rc : galloc(g, &g->Wb, g->nbf);
+  rc = rc ? rc : galloc(g, &g->Wt, g->nbf);
+  rc = rc ? rc : galloc(g, &g->off, Nn + 1);
+  rc = rc ? rc : galloc(g, &g->recv, Ne);
+  rc = rc ? rc : galloc(g, &g->send, Ne);
+  rc = rc ? rc : galloc(g, &g->rev, Ne);
+  rc = rc ? rc : galloc(g, &g->keys, Nn);
+  rc = rc ? rc : galloc(g, &g->idx, Nn);
+  rc = rc ? rc : galloc(g, &g->keys2, Nn);
+  rc = rc ? rc : galloc(g, &g->idx2, Nn);
+  rc = rc ? rc : galloc(g, &g->inv, Nn);
+  rc = rc ? rc : galloc(g, &g->deg, Nn + 1);
+  rc = rc ? rc : galloc(g, &g->hist_sort, (size_t)radix_hist_words((int)Nn));
+  rc = rc ? rc : galloc(g, &g->totals, 256);
+  rc = rc ? rc : galloc(g, &g->scan_blocks, Nn / 1024 + 2);
+  rc = rc ? rc : galloc(g, &g->pos, 3 * Nn);
+  rc = rc ? rc : galloc(g, &g->vhist, (size_t)cfg->history * 3 * Nn);
+  rc = rc ? rc : galloc(g, &g->rho, Nn);
+  rc = rc ? rc : galloc(g, &g->type, Nn);
+  rc = rc ? rc : galloc(g, &g->target, 3 * Nn);
+  rc = rc ? rc : galloc(g, &g->Xn, Nn * W);
+  rc = rc ? rc : galloc(g, &g->Xe, Ne * W);
+  rc = rc ? rc : galloc(g, &g->V, (tr ? (size_t)g->L + 1 : 1) * Nn * W);
+  rc = rc ? rc : galloc(g, &g->E, (tr ? (size_t)g->L + 1 : 1) * Ne * W);
+  rc = rc ? rc : galloc(g, &g->Agg, (tr ? (size_t)g->L : 1) * Nn * W);
+  rc = rc ? rc : galloc(g, &g->Y, Nn * W);
+  rc = rc ? rc : galloc(g, &g->H, (size_t)calls * h_slot(g));
+  rc = rc ? rc : galloc(g, &g->scal, 64);
+  if (tr) {
+    rc = rc ? rc : galloc(g, &g->dY, Nn * W);
+    rc = rc ? rc : galloc(g, &g->dV, Nn * W);
+    rc = rc ? rc : galloc(g, &g->dE, Ne * W);
+    rc = rc ? rc : galloc(g, &g->dX, Mx * 3 * W);
+    rc = rc ? rc : galloc(g, &g->dZf[0], Mx * W);
+    rc = rc ? rc : galloc(g, &g->dZf[1], Mx * W);
+    rc = rc ? rc : galloc(g, &g->dZb[0], Mx * W);
+    rc = rc ? rc : galloc(g, &g->dZb[1], Mx * W);
+    rc = rc ? rc : galloc(g, &g->Tz, (size_t)W * Mp);
+    rc = rc ? rc : galloc(g, &g->Tx, (size_t)3 * W * Mp);
+    rc = rc ? rc : galloc(g, &g->part, (size_t)g->split_cap * W * 3 * W);
+    rc = rc ? rc : galloc(g, &g->colpart, (size_t)256 * W);
+  }
+  if (rc) {
+    release(g);
+    return rc;
+  }
+  // initialisation: U(-a, a) with a = sqrt(6 / K) for ReLU layers, a tenth of
+  // that for the last layer of each MLP; zero biases; decoder rows >= 3 zero
+  std::vector<float> host(g->nparam, 0.0f);
+  Lcg rng{0x9E3779B97F4A7C15ULL ^ (unsigned long long)cfg->seed};
+  for (size_t li = 0; li < g->lins.size(); ++li) {
+    const Lin& l = g->lins[li];
+    const bool last = (li % 3) == 2;
+    const float a = std::sqrt(6.0f / (float)l.K) * (last ? 0.1f : 1.0f);
+    for (int r = 0; r < l.N; ++r)
+      for (int k = 0; k < l.K; ++k) host[l.w + (size_t)r * l.K + k] = a * rng.next();
+  }
+  {
+    const Lin& l = g->dec.l[2];
+    for (int r = 3; r < l.N; ++r)
+      for (int k = 0; k < l.K; ++k) host[l.w + (size_t)r * l.K + k] = 0.0f;
+  }
+  he = hipMemcpy(g->P, host.data(), g->nparam * sizeof(float), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemset(g->G, 0, g->nparam * sizeof(float));
+  if (he == hipSuccess) he = hipMemset(g->M1, 0, g->nparam * sizeof(float));
+  if (he == hipSuccess) he = hipMemset(g->M2, 0, g->nparam * sizeof(float));
+  if (he == hipSuccess) he = pack_all(g);
+  if (he == hipSuccess) he = hipStreamSynchronize(g->stream);
+  if (he != hipSuccess) {
+    rc = hip_status(he, "sph_gnn_create: init");
+    release(g);
+    return rc;
+  }
+  *out = g;
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_destroy(sph_gnn* g) {
+  if (!g) return SPH_OK;
+  hipStreamSynchronize(g->stream);
+  release(g);
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_num_params(sph_gnn* g, long long* count) {
+  if (!g || !count) return set_error(SPH_E_INVAL, "null argument");
+  *count = (long long)g->nparam;
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_param_layout(sph_gnn* g, int* layout5, int cap, int* nlayers) {
+  if (!g || !nlayers) return set_error(SPH_E_INVAL, "null argument");
+  *nlayers = (int)g->lins.size();
+  if (layout5)
+    for (int i = 0; i < *nlayers && i < cap; ++i) {
+      const Lin& l = g->lins[i];
+      layout5[5 * i] = (int)l.w;
+      layout5[5 * i + 1] = l.N;
+      layout5[5 * i + 2] = l.K;
+      layout5[5 * i + 3] = (int)l.b;
+      layout5[5 * i + 4] = 0;
+    }
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_get_params(sph_gnn* g, float* host) {
+  if (!g || !host) return set_error(SPH_E_INVAL, "null argument");
+  SPH_HIP(hipStreamSynchronize(g->stream), "sync");
+  SPH_HIP(hipMemcpy(host, g->P, g->nparam * sizeof(float), hipMemcpyDeviceToHost), "get params");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_set_params(sph_gnn* g, const float* host) {
+  if (!g || !host) return set_error(SPH_E_INVAL, "null argument");
+  SPH_HIP(hipStreamSynchronize(g->stream), "sync");
+  SPH_HIP(hipMemcpy(g->P, host, g->nparam * sizeof(float), hipMemcpyHostToDevice), "set params");
+  SPH_HIP(pack_all(g), "pack");
+  SPH_HIP(hipStreamSynchronize(g->stream), "sync");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_get_grads(sph_gnn* g, float* host) {
+  if (!g || !host) return set_error(SPH_E_INVAL, "null argument");
+  SPH_HIP(hipStreamSynchronize(g->stream), "sync");
+  SPH_HIP(hipMemcpy(host, g->G, g->nparam * sizeof(float), hipMemcpyDeviceToHost), "get grads");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_load_state(sph_gnn* g, const sph_params* p, int n, const float* pos, const float* hist,
+                                  const float* rho, const int* type, const float* target) {
+  if (!g || !p || n <= 0 || !pos || !hist || !rho || !type) return set_error(SPH_E_INVAL, "sph_gnn_load_state: bad argument");
+  if (p->dim != 2) return set_error(SPH_E_INVAL, "sph_gnn_load_state: the surrogate is 2-D");
+  if (n > g->cfg.max_nodes) return set_error(SPH_E_INVAL, "sph_gnn_load_state: more nodes than max_nodes");
+  hipStream_t s = g->stream;
+  g->prm = *p;
+  g->n = n;
+  const size_t N = (size_t)n;
+  SPH_HIP(hipMemcpyAsync(g->pos, pos, 3 * N * sizeof(float), hipMemcpyHostToDevice, s), "upload");
+  SPH_HIP(hipMemcpyAsync(g->vhist, hist, (size_t)g->cfg.history * 3 * N * sizeof(float), hipMemcpyHostToDevice, s),
+          "upload");
+  SPH_HIP(hipMemcpyAsync(g->rho, rho, N * sizeof(float), hipMemcpyHostToDevice, s), "upload");
+  SPH_HIP(hipMemcpyAsync(g->type, type, N * sizeof(int), hipMemcpyHostToDevice, s), "upload");
+  g->has_target = target != nullptr;
+  if (target) SPH_HIP(hipMemcpyAsync(g->target, target, 3 * N * sizeof(float), hipMemcpyHostToDevice, s), "upload");
+  // graph: hash, sort, cells, degrees, scan, edges, reverse edges
+  if (p->numCells > g->cells_cap) {
+    if (g->cellStart) {
+      hipFree(g->cellStart);
+      hipFree(g->cellEnd);
+      g->allocs.erase(std::remove(g->allocs.begin(), g->allocs.end(), (void*)g->cellStart), g->allocs.end());
+      g->allocs.erase(std::remove(g->allocs.begin(), g->allocs.end(), (void*)g->cellEnd), g->allocs.end());
+    }
+    int rc = galloc(g, &g->cellStart, (size_t)p->numCells);
+    if (rc) return rc;
+    rc = galloc(g, &g->cellEnd, (size_t)p->numCells);
+    if (rc) return rc;
+    g->cells_cap = p->numCells;
+  }
+  SPH_HIP(launch_hash_aos(*p, g->pos, g->keys, g->idx, n, s), "graph hash");
+  bool alt = false;
+  SPH_HIP(radix_sort_pairs(g->keys, g->idx, g->keys2, g->idx2, g->hist_sort, g->totals, n, key_bits(p->numCells), s,
+                           &alt),
+          "graph sort");
+  int* keys = alt ? g->keys2 : g->keys;
+  int* idx = alt ? g->idx2 : g->idx;
+  SPH_HIP(fill_int(g->cellStart, -1, p->numCells, s), "graph cells");
+  hipLaunchKernelGGL(cells_only_kernel, dim3(nblk(n)), dim3(256), 0, s, keys, g->cellStart, g->cellEnd, n);
+  hipLaunchKernelGGL(invert_kernel, dim3(nblk(n)), dim3(256), 0, s, idx, g->inv, n);
+  hipLaunchKernelGGL(degree_kernel, dim3(nblk(n)), dim3(256), 0, s, *p, g->pos, keys, idx, g->cellStart, g->cellEnd,
+                     g->deg, n);
+  const int nb = (n + 1023) / 1024;
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(1024), 0, s, g->deg, g->off, g->scan_blocks, n);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, s, g->scan_blocks, nb, g->off + n);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(1024), 0, s, g->off, g->scan_blocks, n);
+  SPH_HIP(hipGetLastError(), "graph scan");
+  int ne = 0;
+  SPH_HIP(hipMemcpyAsync(&ne, g->off + n, sizeof(int), hipMemcpyDeviceToHost, s), "graph edges");
+  SPH_HIP(hipStreamSynchronize(s), "graph sync");
+  if (ne > g->cfg.max_edges) return set_error(SPH_E_INVAL, "sph_gnn_load_state: more edges than max_edges");
+  g->ne = ne;
+  hipLaunchKernelGGL(fill_edges_kernel, dim3(nblk(n)), dim3(256), 0, s, *p, g->pos, keys, idx, g->cellStart,
+                     g->cellEnd, g->off, g->send, g->recv, n);
+  if (ne > 0)
+    hipLaunchKernelGGL(reverse_kernel, dim3(nblk(ne)), dim3(256), 0, s, g->off, g->send, g->recv, g->inv, g->rev, ne);
+  // features
+  SPH_HIP(fill_f32(g->Xn, 0.0f, (long long)N * W, s), "features");
+  SPH_HIP(fill_f32(g->Xe, 0.0f, (long long)ne * W, s), "features");
+  hipLaunchKernelGGL(node_features_kernel, dim3(nblk(n)), dim3(256), 0, s, *p, g->cfg, g->pos, g->vhist, g->rho,
+                     g->type, g->Xn, n);
+  if (ne > 0)
+    hipLaunchKernelGGL(edge_features_kernel, dim3(nblk(ne)), dim3(256), 0, s, *p, g->pos, g->send, g->recv, g->Xe, ne);
+  SPH_HIP(hipGetLastError(), "features");
+  std::vector<int> ty(type, type + n);
+  g->nfluid = 0;
+  for (int t : ty) g->nfluid += (t == 1);
+  SPH_HIP(hipStreamSynchronize(s), "load sync");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_graph(sph_gnn* g, int* edges, int* offsets, int* senders, int* receivers) {
+  if (!g || !edges) return set_error(SPH_E_INVAL, "null argument");
+  *edges = g->ne;
+  if (offsets) SPH_HIP(hipMemcpy(offsets, g->off, ((size_t)g->n + 1) * sizeof(int), hipMemcpyDeviceToHost), "graph");
+  if (senders) SPH_HIP(hipMemcpy(senders, g->send, (size_t)g->ne * sizeof(int), hipMemcpyDeviceToHost), "graph");
+  if (receivers) SPH_HIP(hipMemcpy(receivers, g->recv, (size_t)g->ne * sizeof(int), hipMemcpyDeviceToHost), "graph");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_features(sph_gnn* g, float* node_feat, float* edge_feat) {
+  if (!g) return set_error(SPH_E_INVAL, "null argument");
+  if (node_feat) SPH_HIP(hipMemcpy(node_feat, g->Xn, (size_t)g->n * W * sizeof(float), hipMemcpyDeviceToHost), "features");
+  if (edge_feat) SPH_HIP(hipMemcpy(edge_feat, g->Xe, (size_t)g->ne * W * sizeof(float), hipMemcpyDeviceToHost), "features");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_forward(sph_gnn* g, float* out, double* loss) {
+  if (!g || g->n <= 0) return set_error(SPH_E_STATE, "sph_gnn_forward: no state loaded");
+  hipStream_t s = g->stream;
+  SPH_HIP(forward(g), "gnn forward");
+  if (g->has_target && g->cfg.train) {
+    SPH_HIP(fill_f32(g->dY, 0.0f, (long long)g->n * W, s), "loss");
+    SPH_HIP(hipMemsetAsync(g->scal, 0, sizeof(double), s), "loss");
+    const float inv = g->nfluid > 0 ? 1.0f / (3.0f * (float)g->nfluid) : 0.0f;
+    hipLaunchKernelGGL(loss_kernel, dim3(nblk(g->n)), dim3(256), 0, s, g->Y, g->target, g->type, inv, g->dY,
+                       (double*)g->scal, g->n);
+    SPH_HIP(hipGetLastError(), "loss");
+  }
+  if (out) {
+    float* d = nullptr;
+    SPH_HIP(hipMalloc(&d, (size_t)g->n * 3 * sizeof(float)), "out");
+    hipLaunchKernelGGL(gather_out_kernel, dim3(nblk(g->n)), dim3(256), 0, s, g->Y, d, g->n);
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)g->n * 3 * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(d);
+    SPH_HIP(e, "out");
+  }
+  if (loss) {
+    double l = 0.0;
+    if (g->has_target && g->cfg.train)
+      SPH_HIP(hipMemcpyAsync(&l, g->scal, sizeof(double), hipMemcpyDeviceToHost, s), "loss");
+    SPH_HIP(hipStreamSynchronize(s), "loss sync");
+    *loss = l;
+  }
+  SPH_HIP(hipStreamSynchronize(s), "forward sync");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_backward(sph_gnn* g) {
+  if (!g || !g->cfg.train || !g->has_target) return set_error(SPH_E_STATE, "sph_gnn_backward: needs train=1 and a target");
+  SPH_HIP(backward(g), "gnn backward");
+  SPH_HIP(hipStreamSynchronize(g->stream), "backward sync");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_zero_grad(sph_gnn* g) {
+  if (!g) return set_error(SPH_E_INVAL, "null argument");
+  SPH_HIP(fill_f32(g->G, 0.0f, (long long)g->nparam, g->stream), "zero grad");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_adam(sph_gnn* g, float lr, float clip) {
+  if (!g) return set_error(SPH_E_INVAL, "null argument");
+  hipStream_t s = g->stream;
+  float scale = 1.0f;
+  if (clip > 0.0f) {
+    SPH_HIP(hipMemsetAsync(g->scal + 2, 0, sizeof(float), s), "clip");
+    SPH_HIP(sum_squares(g->G, (long long)g->nparam, g->scal + 2, s), "clip");
+    float sq = 0.0f;
+    SPH_HIP(hipMemcpyAsync(&sq, g->scal + 2, sizeof(float), hipMemcpyDeviceToHost, s), "clip");
+    SPH_HIP(hipStreamSynchronize(s), "clip");
+    const float norm = std::sqrt(sq);
+    if (norm > clip) scale = clip / norm;
+  }
+  ++g->adam_t;
+  SPH_HIP(adam(g->P, g->G, g->M1, g->M2, (long long)g->nparam, lr, 0.9f, 0.999f, 1e-8f, g->adam_t, scale, s), "adam");
+  SPH_HIP(pack_all(g), "pack");
+  SPH_HIP(hipStreamSynchronize(s), "adam sync");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_mfma_probe(const unsigned short* A, const unsigned short* B, float* D) {
+  if (!A || !B || !D) return set_error(SPH_E_INVAL, "null argument");
+  bf16 *dA = nullptr, *dB = nullptr;
+  float* dD = nullptr;
+  hipError_t e = hipMalloc(&dA, 16 * 32 * sizeof(bf16));
+  if (e == hipSuccess) e = hipMalloc(&dB, 32 * 16 * sizeof(bf16));
+  if (e == hipSuccess) e = hipMalloc(&dD, 16 * 16 * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(dA, A, 16 * 32 * sizeof(bf16), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dB, B, 32 * 16 * sizeof(bf16), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = mfma_tile(dA, dB, dD, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(D, dD, 16 * 16 * sizeof(float), hipMemcpyDeviceToHost);
+  hipFree(dA);
+  hipFree(dB);
+  hipFree(dD);
+  SPH_HIP(e, "sph_gnn_mfma_probe");
+  return SPH_OK;
+}

@@ -1,0 +1,184 @@
+"""Learned surrogate: a message-passing particle network (GNS-style) trained on
+trajectories of the plain solver and run with the library's BF16 MFMA kernels.
+
+* :class:`Gnn` -- ctypes wrapper of ``sph_gnn_*`` (forward, backward, Adam on
+  the device).
+* :func:`gnn_layout` -- the flat parameter layout (also returned by the library).
+* :func:`make_dataset` / :func:`train` -- snapshots every k steps of plain 2-D
+  dam-break runs, inputs = velocity history + wall distances + type + density,
+  targets = velocity and density change over the next k steps.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import SphParams, call, load
+from .sph import Case, Simulation
+
+W = 128
+
+
+class GnnCfg(C.Structure):
+    _fields_ = [("layers", C.c_int), ("history", C.c_int), ("max_nodes", C.c_int), ("max_edges", C.c_int),
+                ("train", C.c_int), ("seed", C.c_uint), ("v_scale", C.c_float), ("dv_scale", C.c_float),
+                ("drho_scale", C.c_float), ("rho_scale", C.c_float)]
+
+
+def gnn_layout(layers: int) -> np.ndarray:
+    """Flat parameter layout: 5 ints per linear layer (w offset, N, K, b offset, 0)."""
+    kins = [W, W] + [k for _ in range(layers) for k in (3 * W, 2 * W)] + [W]
+    out, off = [], 0
+    for kin in kins:
+        for K in (kin, W, W):
+            out.append((off, W, K, off + W * K, 0))
+            off += W * K + W
+    return np.asarray(out, np.int32).reshape(-1)
+
+
+def _hp(a):
+    return C.c_void_p(0) if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Gnn:
+    """The surrogate network on the current device."""
+
+    def __init__(self, layers=10, history=5, max_nodes=60_000, max_edges=1_500_000, train=True, seed=1,
+                 v_scale=1.0, dv_scale=0.1, drho_scale=1.0, rho_scale=100.0):
+        load()
+        self.cfg = GnnCfg(layers, history, max_nodes, max_edges, 1 if train else 0, seed, v_scale, dv_scale,
+                          drho_scale, rho_scale)
+        h = C.c_void_p()
+        call("sph_gnn_create", C.byref(self.cfg), C.byref(h))
+        self._h = h
+        n = C.c_longlong()
+        call("sph_gnn_num_params", self._h, C.byref(n))
+        self.nparams = n.value
+        nl = C.c_int()
+        lay = np.zeros(5 * (3 * (2 * layers + 3)), np.int32)
+        call("sph_gnn_param_layout", self._h, _hp(lay), lay.size // 5, C.byref(nl))
+        self.layout = lay[: 5 * nl.value]
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib._lib.sph_gnn_destroy(self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, np.float32)
+        call("sph_gnn_get_params", self._h, _hp(out))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        call("sph_gnn_set_params", self._h, _hp(p))
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.nparams, np.float32)
+        call("sph_gnn_get_grads", self._h, _hp(out))
+        return out
+
+    def load(self, params: SphParams, pos, hist, rho, typ, target=None) -> None:
+        n = pos.shape[0]
+        call("sph_gnn_load_state", self._h, C.byref(params), n, _hp(np.ascontiguousarray(pos, np.float32)),
+             _hp(np.ascontiguousarray(hist, np.float32)), _hp(np.ascontiguousarray(rho, np.float32)),
+             _hp(np.ascontiguousarray(typ, np.int32)),
+             _hp(None if target is None else np.ascontiguousarray(target, np.float32)))
+        self.n = n
+
+    def graph(self):
+        ne = C.c_int()
+        call("sph_gnn_graph", self._h, C.byref(ne), None, None, None)
+        off = np.empty(self.n + 1, np.int32)
+        s = np.empty(ne.value, np.int32)
+        r = np.empty(ne.value, np.int32)
+        call("sph_gnn_graph", self._h, C.byref(ne), _hp(off), _hp(s), _hp(r))
+        return off, s, r
+
+    def features(self):
+        off, s, _ = self.graph()
+        xn = np.empty((self.n, W), np.float32)
+        xe = np.empty((s.size, W), np.float32)
+        call("sph_gnn_features", self._h, _hp(xn), _hp(xe))
+        return xn, xe
+
+    def forward(self, want_out=True):
+        out = np.empty((self.n, 3), np.float32) if want_out else None
+        loss = C.c_double()
+        call("sph_gnn_forward", self._h, _hp(out), C.byref(loss))
+        return out, loss.value
+
+    def backward(self) -> None:
+        call("sph_gnn_backward", self._h)
+
+    def zero_grad(self) -> None:
+        call("sph_gnn_zero_grad", self._h)
+
+    def adam(self, lr: float, clip: float = 1.0) -> None:
+        call("sph_gnn_adam", self._h, C.c_float(lr), C.c_float(clip))
+
+
+def mfma_probe(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """D = A (16x32) @ B (32x16) through one gfx950 MFMA (inputs rounded to BF16)."""
+    a = _to_bf16_bits(A)
+    b = _to_bf16_bits(B)
+    D = np.empty((16, 16), np.float32)
+    call("sph_gnn_mfma_probe", _hp(a), _hp(b), _hp(D))
+    return D
+
+
+def _to_bf16_bits(x: np.ndarray) -> np.ndarray:
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return r.astype(np.uint16)
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    return (_to_bf16_bits(x).astype(np.uint32) << 16).view(np.float32).reshape(np.shape(x))
+
+
+# ------------------------------------------------------------------- data
+@dataclass
+class Snapshot:
+    params: SphParams
+    pos: np.ndarray     # (n, 3)
+    hist: np.ndarray    # (history, n, 3), newest first
+    rho: np.ndarray
+    type: np.ndarray
+    target: np.ndarray  # (n, 3) raw: dvx, dvz, drho over the next k steps
+
+
+@dataclass
+class Dataset:
+    k: int
+    history: int
+    snaps: list = field(default_factory=list)
+
+
+def record(case: Case, nsteps: int, k: int, history: int) -> Dataset:
+    """Runs the plain solver and keeps a snapshot every k steps (with history)."""
+    ds = Dataset(k, history)
+    frames = []
+    with Simulation(case) as sim:
+        for i in range(nsteps // k + 1):
+            st = sim.state()
+            frames.append(st)
+            if i < nsteps // k:
+                sim.step(k)
+    for t in range(history - 1, len(frames) - 1):
+        cur, nxt = frames[t], frames[t + 1]
+        hist = np.stack([frames[t - j]["vel"] for j in range(history)])
+        tgt = np.stack([nxt["vel"][:, 0] - cur["vel"][:, 0], nxt["vel"][:, 2] - cur["vel"][:, 2],
+                        nxt["rho"] - cur["rho"]], 1).astype(np.float32)
+        ds.snaps.append(Snapshot(case.params, cur["pos"], hist.astype(np.float32), cur["rho"], cur["type"], tgt))
+    return ds
