@@ -207,6 +207,8 @@ int sph_sim_create_batch(const sph_params* params, int K, const int* counts, con
                          const float* vel, const float* rho, const float* prs, const int* type,
                          sph_sim** out);
 int sph_sim_num_instances(sph_sim* s, int* K);
+/* Parameters of instance k (numParticles holds the instance's particle count). */
+int sph_sim_instance_params(sph_sim* s, int k, sph_params* p);
 /* Per-instance surge front and column height (arrays of K doubles). */
 int sph_sim_probe_batch(sph_sim* s, double band, double* front, double* height);
 /* Times the neighbour kernels of the current state over the candidate launch
@@ -214,6 +216,14 @@ int sph_sim_probe_batch(sph_sim* s, double band, double* front, double* height);
    the fastest; the state is not advanced.  *best receives the choice and
    *best_us its kernel A + B time in microseconds (either may be null). */
 int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* best_us);
+/* Device-to-device state exchange in ORIGINAL particle order (device
+   pointers, reference layouts: pos/vel AoS float[3N], rho/p float[N],
+   type int[N]; any of the outputs may be null).  set_state replaces the whole
+   state (positions are widened to FP64, the sorted order restarts from the
+   original order) and keeps any captured graph valid. */
+int sph_sim_get_state_device(sph_sim* s, float* pos, float* vel, float* rho, float* p, int* type);
+int sph_sim_set_state_device(sph_sim* s, const float* pos, const float* vel, const float* rho,
+                             const float* p, const int* type);
 int sph_sim_destroy(sph_sim* s);
 int sph_sim_get_tuning(sph_sim* s, sph_tuning* t);
 int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t);
@@ -276,7 +286,29 @@ typedef struct sph_gnn_cfg {
     float dv_scale;      /* output velocity-change normalisation */
     float drho_scale;    /* output density-change normalisation */
     float rho_scale;     /* input density deviation scale ((rho/rho0 - 1) * rho_scale) */
+    float out_mean[3];   /* output = normalised prediction * scale + out_mean (dvx, dvz, drho) */
 } sph_gnn_cfg;
+
+typedef struct sph_rollout_stats {
+    int accepted;        /* intervals advanced by the surrogate */
+    int fallbacks;       /* surrogate intervals rejected and redone with k plain steps */
+    long long plain_steps; /* plain SPH steps taken (history warm-up and fallbacks) */
+    double max_dev;      /* largest density deviation the judge measured */
+    double mean_dev;     /* mean over judged intervals */
+    double seconds;      /* wall-clock of the rollout */
+} sph_rollout_stats;
+
+/* Advances the single-instance 2-D context `sim` by `intervals` intervals of
+   k plain-step lengths, each by one surrogate step judged by the plain
+   method: the SPH summation density (the kernel sum of kernel A's Wendland
+   weights) of the predicted positions must change like the predicted density
+   -- mean |d rho_sum - d rho_pred| / rho0 over fluid particles with a full
+   support <= dev_tol -- and no fluid particle may leave the tank; otherwise
+   the interval is redone with k plain steps.  The first `history` intervals
+   are plain (they fill the velocity history).  front/height (may be null)
+   receive the probes after every interval. */
+int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, float dev_tol, double band,
+                    double* front, double* height, sph_rollout_stats* stats);
 
 int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out);
 int sph_gnn_destroy(sph_gnn* g);

@@ -138,7 +138,7 @@ extern "C" {
 // target n x 3; type n.  Returns the loss (mean over fluid nodes and 3 outputs).
 double orc_gnn_loss(int L, const float* params, const int* layout5, int nlin, int n, int ne, const float* Xn,
                     const float* Xe, const int* off, const int* send, const int* recv, const float* target,
-                    const int* type, int bf16, double* G) {
+                    const int* type, int bf16, double* G, double* Yout) {
   Net net{L, params, std::vector<int>(layout5, layout5 + 5 * nlin), bf16 != 0};
   const int enc_n = 0, enc_e = 3, dec = 6 + 6 * L;
   auto pe = [](int l) { return 6 + 6 * l; };
@@ -177,6 +177,9 @@ double orc_gnn_loss(int L, const float* params, const int* layout5, int nlin, in
   }
   Vec Y;
   mlp_fwd(net, dec, V[L], n, adec, Y);
+  if (Yout)
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < 3; ++c) Yout[(size_t)i * 3 + c] = Y[(size_t)i * W + c];
   int nf = 0;
   for (int i = 0; i < n; ++i) nf += type[i] == 1;
   const double inv = nf ? 1.0 / (3.0 * nf) : 0.0;

@@ -46,7 +46,7 @@ def load() -> C.CDLL:
         "orc_eos_density64": ([sp, C.c_double], C.c_double),
         "orc_run": ([sp, I, I] + [P] * 6 + [I, C.c_double, I, P, P, P], I),
         "orc_eval": ([sp, I, I] + [P] * 15, None),
-        "orc_gnn_loss": ([I, P, P, I, I, I] + [P] * 7 + [I, P], C.c_double),
+        "orc_gnn_loss": ([I, P, P, I, I, I] + [P] * 7 + [I, P, P], C.c_double),
     }
     for name, (args, res) in protos.items():
         f = getattr(lib, name)
@@ -158,18 +158,19 @@ def run(prm, case, nsteps: int, mode: int = 1, band: float = 0.0, probe_every: i
 
 
 def gnn_loss(layers: int, params: np.ndarray, layout: np.ndarray, Xn, Xe, off, send, recv, target, typ,
-             bf16: bool = False, grad: bool = True):
-    """Loss (and FP64 gradient) of the message-passing network on a given graph."""
+             bf16: bool = False, grad: bool = True, outputs: bool = False):
+    """Loss (and FP64 gradient, and the n x 3 outputs) of the message-passing network on a given graph."""
     params = np.ascontiguousarray(params, np.float32)
     layout = np.ascontiguousarray(layout, np.int32)
     G = np.zeros(params.size, np.float64) if grad else None
     n, ne = Xn.shape[0], Xe.shape[0]
+    Y = np.zeros((n, 3), np.float64) if outputs else None
     loss = load().orc_gnn_loss(int(layers), _hp(params), _hp(layout), layout.size // 5, n, ne,
                                _hp(np.ascontiguousarray(Xn, np.float32)), _hp(np.ascontiguousarray(Xe, np.float32)),
                                _hp(np.ascontiguousarray(off, np.int32)), _hp(np.ascontiguousarray(send, np.int32)),
                                _hp(np.ascontiguousarray(recv, np.int32)), _hp(np.ascontiguousarray(target, np.float32)),
-                               _hp(np.ascontiguousarray(typ, np.int32)), 1 if bf16 else 0, _hp(G))
-    return loss, G
+                               _hp(np.ascontiguousarray(typ, np.int32)), 1 if bf16 else 0, _hp(G), _hp(Y))
+    return (loss, G, Y) if outputs else (loss, G)
 
 
 def eval_rates(prm, case, mode: int = 1) -> dict:

@@ -62,12 +62,25 @@ def test_loss_and_gradient_match_oracle(gpu, orc, layers):
     xn, xe = g.features()
     off, send, recv = g.graph()
     g.zero_grad()
-    _, loss = g.forward(want_out=False)
+    out, loss = g.forward(want_out=True)
     g.backward()
     grad = g.grads().astype(np.float64)
     params = g.params()
-    lb, gb = orc.gnn_loss(layers, params, g.layout, xn, xe, off, send, recv, target, case.type, bf16=True)
+    lb, gb, yb = orc.gnn_loss(layers, params, g.layout, xn, xe, off, send, recv, target, case.type, bf16=True,
+                              outputs=True)
     l64, g64 = orc.gnn_loss(layers, params, g.layout, xn, xe, off, send, recv, target, case.type, bf16=False)
+    # diagnostics: outputs, then the gradient error of every linear layer
+    yerr = np.linalg.norm(out - yb) / np.linalg.norm(yb)
+    print(f"layers={layers} n={case.n} edges={send.size} output rel err {yerr:.3e} loss {loss:.6f} vs {lb:.6f}")
+    lay = g.layout.reshape(-1, 5)
+    for li, (w, N, K, b, _) in enumerate(lay):
+        gw, ow = grad[w: w + N * K], gb[w: w + N * K]
+        gbb, obb = grad[b: b + N], gb[b: b + N]
+        ew = np.linalg.norm(gw - ow) / (np.linalg.norm(ow) + 1e-30)
+        eb = np.linalg.norm(gbb - obb) / (np.linalg.norm(obb) + 1e-30)
+        print(f"  lin {li:2d} K={K:3d} |gW|={np.linalg.norm(ow):.3e} errW={ew:.2e} |gb|={np.linalg.norm(obb):.3e} "
+              f"errb={eb:.2e}")
+    assert yerr < 1e-2, yerr
     # same BF16 rounding points: FP32 accumulation is the only difference
     assert loss == pytest.approx(lb, rel=2e-3)
     rel = np.linalg.norm(grad - gb) / np.linalg.norm(gb)
@@ -76,6 +89,29 @@ def test_loss_and_gradient_match_oracle(gpu, orc, layers):
     assert loss == pytest.approx(l64, rel=0.05)
     cos = float(grad @ g64 / np.linalg.norm(grad) / np.linalg.norm(g64))
     assert cos > 0.99, cos
+    g.close()
+
+
+def test_rollout_fallback_reproduces_plain_run(gpu):
+    """With a judge that rejects every surrogate step the rollout is exactly the
+    plain solver; with one that accepts everything the surrogate advances."""
+    from tmp_sph_amd import sph
+    from tmp_sph_amd.surrogate import Gnn
+    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.02))
+    k, intervals = 5, 9
+    g = Gnn(layers=2, max_nodes=4096, max_edges=120_000, train=False, seed=2)
+    with sph.Simulation(case) as a, sph.Simulation(case) as b:
+        st, f, h = g.rollout(a, k, intervals, dev_tol=-1.0)
+        assert st["accepted"] == 0 and st["fallbacks"] == intervals - 4
+        assert st["plain_steps"] == k * intervals
+        b.step(k * intervals)
+        np.testing.assert_array_equal(a.state()["pos"], b.state()["pos"])
+        fb, hb = b.probe(0.02)
+        assert f[-1] == fb and h[-1] == hb
+    with sph.Simulation(case) as c:
+        st, f, h = g.rollout(c, k, intervals, dev_tol=1e30)
+        assert st["accepted"] == intervals - 4 and st["fallbacks"] == 0
+        assert np.all(np.isfinite(f)) and np.all(np.isfinite(h))
     g.close()
 
 

@@ -150,7 +150,7 @@ _PROTOS = {
     "sph_sim_pair_stats": [P] + [C.POINTER(C.c_longlong)] * 4,
     "sph_sim_create_batch": [C.POINTER(SphParams), I, IP, P, P, P, P, P, C.POINTER(C.c_void_p)],
     "sph_sim_num_instances": [P, IP],
-    "sph_sim_probe_batch": [P, C.c_double, DP, DP],
+    "sph_sim_probe_batch": [P, C.c_double, P, P],
     "sph_sim_autotune": [P, I, C.POINTER(SphTuning), DP],
     "sph_gnn_create": [P, C.POINTER(C.c_void_p)],
     "sph_gnn_destroy": [P],
@@ -167,6 +167,10 @@ _PROTOS = {
     "sph_gnn_zero_grad": [P],
     "sph_gnn_adam": [P, C.c_float, C.c_float],
     "sph_gnn_mfma_probe": [P, P, P],
+    "sph_sim_get_state_device": [P, P, P, P, P, P],
+    "sph_sim_set_state_device": [P, P, P, P, P, P],
+    "sph_sim_instance_params": [P, I, C.POINTER(SphParams)],
+    "sph_gnn_rollout": [P, P, I, I, C.c_float, C.c_double, P, P, P],
 }
 
 EXPORTS = ["sph_last_error", *_PROTOS]

@@ -9,11 +9,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "gnn_kernels.h"
+#include "sph_device.h"
 #include "sph_kernels.h"
 
 using namespace sph;
@@ -181,7 +183,8 @@ __global__ void reverse_kernel(const int* __restrict__ off, const int* __restric
   if (e >= ne) return;
   const int p = recv[e], q = send[e];
   const int target = inv[p];
-  int lo = off[q], hi = off[q + 1];
+  const int end = off[q + 1];
+  int lo = off[q], hi = end;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     if (inv[send[mid]] < target)
@@ -189,7 +192,8 @@ __global__ void reverse_kernel(const int* __restrict__ off, const int* __restric
     else
       hi = mid;
   }
-  rev[e] = lo;
+  // the pair test is symmetric, so the reverse edge exists; never point outside
+  rev[e] = (lo < end && send[lo] == p) ? lo : e;
 }
 
 // exclusive scan of deg[0..n) into off[0..n], 1024 elements per block
@@ -688,6 +692,251 @@ extern "C" int sph_gnn_get_grads(sph_gnn* g, float* host) {
   return SPH_OK;
 }
 
+// Radius graph and input features of the state held in g->pos / vhist / rho /
+// type (device, original order): hash, radix sort, cell ranges, degrees,
+// scan, edges, reverse edges.  One host synchronisation (the edge count).
+static int build_graph_features(sph_gnn* g, const sph_params& p, int n) {
+  hipStream_t s = g->stream;
+  g->prm = p;
+  g->n = n;
+  const size_t N = (size_t)n;
+  if (p.numCells > g->cells_cap) {
+    if (g->cellStart) {
+      hipFree(g->cellStart);
+      hipFree(g->cellEnd);
+      g->allocs.erase(std::remove(g->allocs.begin(), g->allocs.end(), (void*)g->cellStart), g->allocs.end());
+      g->allocs.erase(std::remove(g->allocs.begin(), g->allocs.end(), (void*)g->cellEnd), g->allocs.end());
+    }
+    int rc = galloc(g, &g->cellStart, (size_t)p.numCells);
+    if (rc) return rc;
+    rc = galloc(g, &g->cellEnd, (size_t)p.numCells);
+    if (rc) return rc;
+    g->cells_cap = p.numCells;
+  }
+  SPH_HIP(launch_hash_aos(p, g->pos, g->keys, g->idx, n, s), "graph hash");
+  bool alt = false;
+  SPH_HIP(radix_sort_pairs(g->keys, g->idx, g->keys2, g->idx2, g->hist_sort, g->totals, n, key_bits(p.numCells), s,
+                           &alt),
+          "graph sort");
+  int* keys = alt ? g->keys2 : g->keys;
+  int* idx = alt ? g->idx2 : g->idx;
+  SPH_HIP(fill_int(g->cellStart, -1, p.numCells, s), "graph cells");
+  hipLaunchKernelGGL(cells_only_kernel, dim3(nblk(n)), dim3(256), 0, s, keys, g->cellStart, g->cellEnd, n);
+  hipLaunchKernelGGL(invert_kernel, dim3(nblk(n)), dim3(256), 0, s, idx, g->inv, n);
+  hipLaunchKernelGGL(degree_kernel, dim3(nblk(n)), dim3(256), 0, s, p, g->pos, keys, idx, g->cellStart, g->cellEnd,
+                     g->deg, n);
+  const int nb = (n + 1023) / 1024;
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(1024), 0, s, g->deg, g->off, g->scan_blocks, n);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, s, g->scan_blocks, nb, g->off + n);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(1024), 0, s, g->off, g->scan_blocks, n);
+  SPH_HIP(hipGetLastError(), "graph scan");
+  int ne = 0;
+  SPH_HIP(hipMemcpyAsync(&ne, g->off + n, sizeof(int), hipMemcpyDeviceToHost, s), "graph edges");
+  SPH_HIP(hipStreamSynchronize(s), "graph sync");
+  if (ne > g->cfg.max_edges) return set_error(SPH_E_INVAL, "surrogate graph: more edges than max_edges");
+  g->ne = ne;
+  hipLaunchKernelGGL(fill_edges_kernel, dim3(nblk(n)), dim3(256), 0, s, p, g->pos, keys, idx, g->cellStart,
+                     g->cellEnd, g->off, g->send, g->recv, n);
+  if (ne > 0)
+    hipLaunchKernelGGL(reverse_kernel, dim3(nblk(ne)), dim3(256), 0, s, g->off, g->send, g->recv, g->inv, g->rev, ne);
+  SPH_HIP(fill_f32(g->Xn, 0.0f, (long long)N * W, s), "features");
+  SPH_HIP(fill_f32(g->Xe, 0.0f, (long long)ne * W, s), "features");
+  hipLaunchKernelGGL(node_features_kernel, dim3(nblk(n)), dim3(256), 0, s, p, g->cfg, g->pos, g->vhist, g->rho,
+                     g->type, g->Xn, n);
+  if (ne > 0)
+    hipLaunchKernelGGL(edge_features_kernel, dim3(nblk(ne)), dim3(256), 0, s, p, g->pos, g->send, g->recv, g->Xe, ne);
+  SPH_HIP(hipGetLastError(), "features");
+  return SPH_OK;
+}
+
+// ---------------------------------------------------------------- rollout
+namespace {
+
+// Wendland C2 in 2-D: adh (1 - q/2)^4 (2q + 1), zero beyond q = 2
+__device__ __forceinline__ float wendland2d(const sph_params& P, float rr) {
+  if (rr > P.rr_cut) return 0.0f;
+  const float q = sqrtf(rr) / P.h;
+  const float t = 1.0f - 0.5f * q;
+  return P.adh * t * t * t * t * (2.0f * q + 1.0f);
+}
+
+// Applies the prediction (fluid only): v' = v + dv, x' = x + k dt (v + v')/2,
+// rho' = max(rho0, rho + drho), p' = EOS(rho'); walls keep their state.
+__global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* __restrict__ Y,
+                             const float* __restrict__ pos, const float* __restrict__ vel,
+                             const float* __restrict__ rho, const int* __restrict__ type, float* __restrict__ pos2,
+                             float* __restrict__ vel2, float* __restrict__ rho2, float* __restrict__ p2, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = pos[3 * i], z = pos[3 * i + 2];
+  const float vx = vel[3 * i], vz = vel[3 * i + 2];
+  float nx = x, nz = z, nvx = vx, nvz = vz, nr = rho[i];
+  if (type[i] == 1) {
+    const float* y = Y + (size_t)i * W;
+    nvx = vx + y[0] * cfg.dv_scale + cfg.out_mean[0];
+    nvz = vz + y[1] * cfg.dv_scale + cfg.out_mean[1];
+    nr = fmaxf(rho[i] + y[2] * cfg.drho_scale + cfg.out_mean[2], P.restDensity);
+    const float T = (float)k * P.timeStep;
+    nx = x + 0.5f * T * (vx + nvx);
+    nz = z + 0.5f * T * (vz + nvz);
+  }
+  pos2[3 * i] = nx;
+  pos2[3 * i + 1] = 0.0f;
+  pos2[3 * i + 2] = nz;
+  vel2[3 * i] = nvx;
+  vel2[3 * i + 1] = 0.0f;
+  vel2[3 * i + 2] = nvz;
+  rho2[i] = nr;
+  p2[i] = eos_pressure(P, nr);
+}
+
+// Judge: for fluid particles with a full support (degree >= full_deg),
+// |(rho_sum(x') - rho_sum(x)) - (rho' - rho)| / rho0, where rho_sum is the
+// kernel sum m sum_j W_ij over the graph (self included).  Also flags fluid
+// that left the tank.  out: [0] sum of deviations, [1] count, [2] max, [3] escaped
+__global__ void judge_kernel(sph_params P, const int* __restrict__ off, const int* __restrict__ send,
+                             const float* __restrict__ pos, const float* __restrict__ pos2,
+                             const float* __restrict__ rho, const float* __restrict__ rho2,
+                             const int* __restrict__ type, int full_deg, float* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float dev = 0.0f, cnt = 0.0f, esc = 0.0f;
+  if (i < n && type[i] == 1) {
+    const float x2 = pos2[3 * i], z2 = pos2[3 * i + 2];
+    if ((double)x2 < P.domain_min[0] || (double)x2 > P.domain_max[0] || (double)z2 < P.domain_min[2]) esc = 1.0f;
+    const int b = off[i], e = off[i + 1];
+    if (e - b >= full_deg) {
+      const float W0 = wendland2d(P, 0.0f);
+      float s1 = W0, s2 = W0;
+      for (int k = b; k < e; ++k) {
+        const int j = send[k];
+        const float dx = pos[3 * i] - pos[3 * j], dz = pos[3 * i + 2] - pos[3 * j + 2];
+        const float ex = x2 - pos2[3 * j], ez = z2 - pos2[3 * j + 2];
+        s1 += wendland2d(P, dx * dx + dz * dz);
+        s2 += wendland2d(P, ex * ex + ez * ez);
+      }
+      const float dsum = P.particleMass * (s2 - s1);
+      dev = fabsf(dsum - (rho2[i] - rho[i])) / P.restDensity;
+      cnt = 1.0f;
+    }
+  }
+  float mx = dev;
+  for (int o = 32; o > 0; o >>= 1) {
+    dev += __shfl_xor(dev, o);
+    cnt += __shfl_xor(cnt, o);
+    esc += __shfl_xor(esc, o);
+    mx = fmaxf(mx, __shfl_xor(mx, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&out[0], dev);
+    atomicAdd(&out[1], cnt);
+    atomicMax(reinterpret_cast<int*>(&out[2]), __float_as_int(mx));  // non-negative floats order as ints
+    atomicAdd(&out[3], esc);
+  }
+}
+
+}  // namespace
+
+extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, float dev_tol, double band,
+                               double* front, double* height, sph_rollout_stats* stats) {
+  if (!g || !sim || k <= 0 || intervals < 0) return set_error(SPH_E_INVAL, "sph_gnn_rollout: bad argument");
+  int K = 0;
+  int rc = sph_sim_num_instances(sim, &K);
+  if (rc) return rc;
+  if (K != 1) return set_error(SPH_E_INVAL, "sph_gnn_rollout: single-instance contexts only");
+  sph_params P;
+  rc = sph_sim_instance_params(sim, 0, &P);
+  if (rc) return rc;
+  if (P.dim != 2) return set_error(SPH_E_INVAL, "sph_gnn_rollout: the surrogate is 2-D");
+  const int n = P.numParticles;
+  if (n > g->cfg.max_nodes) return set_error(SPH_E_INVAL, "sph_gnn_rollout: more particles than max_nodes");
+  hipStream_t s = g->stream;
+  const size_t N = (size_t)n;
+  const int H = g->cfg.history;
+  // state buffers: current velocity / pressure and the prediction
+  float *vel = nullptr, *prs = nullptr, *pos2 = nullptr, *vel2 = nullptr, *rho2 = nullptr, *p2 = nullptr;
+  float* judge = nullptr;
+  hipError_t he = hipMalloc(&vel, 3 * N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&prs, N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&pos2, 3 * N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&vel2, 3 * N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&rho2, N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&p2, N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&judge, 4 * sizeof(float));
+  if (he != hipSuccess) {
+    hipFree(vel); hipFree(prs); hipFree(pos2); hipFree(vel2); hipFree(rho2); hipFree(p2); hipFree(judge);
+    return hip_status(he, "sph_gnn_rollout: scratch");
+  }
+  // lattice neighbour count within the support (full support = 90 % of it)
+  const double rc_lat = std::sqrt((double)P.rr_cut) / (double)P.particleR;
+  int full = 0;
+  for (int a = -4; a <= 4; ++a)
+    for (int b = -4; b <= 4; ++b)
+      if ((a || b) && a * a + b * b <= rc_lat * rc_lat) ++full;
+  const int full_deg = (int)(0.9 * full);
+  sph_rollout_stats st{};
+  double dev_sum = 0.0;
+  int judged = 0, filled = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  auto push_history = [&](const float* v) -> hipError_t {
+    for (int t = H - 1; t > 0; --t)
+      HIPCHK(hipMemcpyAsync(g->vhist + (size_t)t * 3 * N, g->vhist + (size_t)(t - 1) * 3 * N, 3 * N * sizeof(float),
+                            hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g->vhist, v, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return hipStreamSynchronize(s);  // `v` is rewritten by the context's stream next
+  };
+  rc = sph_sim_get_state_device(sim, g->pos, vel, g->rho, prs, g->type);
+  if (!rc && (he = push_history(vel)) != hipSuccess) rc = hip_status(he, "rollout history");
+  filled = 1;
+  for (int it = 0; it < intervals && !rc; ++it) {
+    bool plain = filled < H;
+    if (!plain) {
+      rc = build_graph_features(g, P, n);
+      if (rc) break;
+      if ((he = forward(g)) != hipSuccess) { rc = hip_status(he, "rollout forward"); break; }
+      hipLaunchKernelGGL(apply_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->cfg, k, g->Y, g->pos, vel, g->rho,
+                         g->type, pos2, vel2, rho2, p2, n);
+      hipMemsetAsync(judge, 0, 4 * sizeof(float), s);
+      hipLaunchKernelGGL(judge_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->off, g->send, g->pos, pos2, g->rho, rho2,
+                         g->type, full_deg, judge, n);
+      float jh[4];
+      he = hipMemcpyAsync(jh, judge, sizeof(jh), hipMemcpyDeviceToHost, s);
+      if (he == hipSuccess) he = hipStreamSynchronize(s);
+      if (he != hipSuccess) { rc = hip_status(he, "rollout judge"); break; }
+      const double dev = jh[1] > 0 ? (double)jh[0] / (double)jh[1] : 0.0;
+      dev_sum += dev;
+      ++judged;
+      st.max_dev = std::max(st.max_dev, dev);
+      if (dev <= dev_tol && jh[3] == 0.0f) {
+        rc = sph_sim_set_state_device(sim, pos2, vel2, rho2, p2, g->type);
+        if (rc) break;
+        ++st.accepted;
+      } else {
+        plain = true;
+        ++st.fallbacks;
+      }
+    }
+    if (plain) {
+      rc = sph_sim_step(sim, k);
+      if (rc) break;
+      st.plain_steps += k;
+    }
+    rc = sph_sim_get_state_device(sim, g->pos, vel, g->rho, prs, g->type);
+    if (rc) break;
+    if ((he = push_history(vel)) != hipSuccess) { rc = hip_status(he, "rollout history"); break; }
+    filled = std::min(filled + 1, H);
+    if (front && height) {
+      rc = sph_sim_probe(sim, band, &front[it], &height[it]);
+      if (rc) break;
+    }
+  }
+  hipStreamSynchronize(s);
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  st.mean_dev = judged ? dev_sum / judged : 0.0;
+  if (stats) *stats = st;
+  hipFree(vel); hipFree(prs); hipFree(pos2); hipFree(vel2); hipFree(rho2); hipFree(p2); hipFree(judge);
+  return rc;
+}
+
 extern "C" int sph_gnn_load_state(sph_gnn* g, const sph_params* p, int n, const float* pos, const float* hist,
                                   const float* rho, const int* type, const float* target) {
   if (!g || !p || n <= 0 || !pos || !hist || !rho || !type) return set_error(SPH_E_INVAL, "sph_gnn_load_state: bad argument");
@@ -704,54 +953,8 @@ extern "C" int sph_gnn_load_state(sph_gnn* g, const sph_params* p, int n, const 
   SPH_HIP(hipMemcpyAsync(g->type, type, N * sizeof(int), hipMemcpyHostToDevice, s), "upload");
   g->has_target = target != nullptr;
   if (target) SPH_HIP(hipMemcpyAsync(g->target, target, 3 * N * sizeof(float), hipMemcpyHostToDevice, s), "upload");
-  // graph: hash, sort, cells, degrees, scan, edges, reverse edges
-  if (p->numCells > g->cells_cap) {
-    if (g->cellStart) {
-      hipFree(g->cellStart);
-      hipFree(g->cellEnd);
-      g->allocs.erase(std::remove(g->allocs.begin(), g->allocs.end(), (void*)g->cellStart), g->allocs.end());
-      g->allocs.erase(std::remove(g->allocs.begin(), g->allocs.end(), (void*)g->cellEnd), g->allocs.end());
-    }
-    int rc = galloc(g, &g->cellStart, (size_t)p->numCells);
-    if (rc) return rc;
-    rc = galloc(g, &g->cellEnd, (size_t)p->numCells);
-    if (rc) return rc;
-    g->cells_cap = p->numCells;
-  }
-  SPH_HIP(launch_hash_aos(*p, g->pos, g->keys, g->idx, n, s), "graph hash");
-  bool alt = false;
-  SPH_HIP(radix_sort_pairs(g->keys, g->idx, g->keys2, g->idx2, g->hist_sort, g->totals, n, key_bits(p->numCells), s,
-                           &alt),
-          "graph sort");
-  int* keys = alt ? g->keys2 : g->keys;
-  int* idx = alt ? g->idx2 : g->idx;
-  SPH_HIP(fill_int(g->cellStart, -1, p->numCells, s), "graph cells");
-  hipLaunchKernelGGL(cells_only_kernel, dim3(nblk(n)), dim3(256), 0, s, keys, g->cellStart, g->cellEnd, n);
-  hipLaunchKernelGGL(invert_kernel, dim3(nblk(n)), dim3(256), 0, s, idx, g->inv, n);
-  hipLaunchKernelGGL(degree_kernel, dim3(nblk(n)), dim3(256), 0, s, *p, g->pos, keys, idx, g->cellStart, g->cellEnd,
-                     g->deg, n);
-  const int nb = (n + 1023) / 1024;
-  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(1024), 0, s, g->deg, g->off, g->scan_blocks, n);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, s, g->scan_blocks, nb, g->off + n);
-  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(1024), 0, s, g->off, g->scan_blocks, n);
-  SPH_HIP(hipGetLastError(), "graph scan");
-  int ne = 0;
-  SPH_HIP(hipMemcpyAsync(&ne, g->off + n, sizeof(int), hipMemcpyDeviceToHost, s), "graph edges");
-  SPH_HIP(hipStreamSynchronize(s), "graph sync");
-  if (ne > g->cfg.max_edges) return set_error(SPH_E_INVAL, "sph_gnn_load_state: more edges than max_edges");
-  g->ne = ne;
-  hipLaunchKernelGGL(fill_edges_kernel, dim3(nblk(n)), dim3(256), 0, s, *p, g->pos, keys, idx, g->cellStart,
-                     g->cellEnd, g->off, g->send, g->recv, n);
-  if (ne > 0)
-    hipLaunchKernelGGL(reverse_kernel, dim3(nblk(ne)), dim3(256), 0, s, g->off, g->send, g->recv, g->inv, g->rev, ne);
-  // features
-  SPH_HIP(fill_f32(g->Xn, 0.0f, (long long)N * W, s), "features");
-  SPH_HIP(fill_f32(g->Xe, 0.0f, (long long)ne * W, s), "features");
-  hipLaunchKernelGGL(node_features_kernel, dim3(nblk(n)), dim3(256), 0, s, *p, g->cfg, g->pos, g->vhist, g->rho,
-                     g->type, g->Xn, n);
-  if (ne > 0)
-    hipLaunchKernelGGL(edge_features_kernel, dim3(nblk(ne)), dim3(256), 0, s, *p, g->pos, g->send, g->recv, g->Xe, ne);
-  SPH_HIP(hipGetLastError(), "features");
+  int rc = build_graph_features(g, *p, n);
+  if (rc) return rc;
   std::vector<int> ty(type, type + n);
   g->nfluid = 0;
   for (int t : ty) g->nfluid += (t == 1);

@@ -476,16 +476,39 @@ __global__ void download_rates_kernel(FastSet st, FastWork w, float* __restrict_
 // bit patterns order like the values and an integer atomicMax reduces them.
 __global__ void probe_kernel(InstTable T, FastSet st, double band, unsigned long long* __restrict__ out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int ty = st.TY[i];
-  if (type_of(ty) != 1) return;
-  const int k = inst_of(ty);
-  const double x = st.X[0][i], z = st.X[2][i];
-  const double y = st.X[1] ? st.X[1][i] : 0.0;
-  const sph_params& P = T.P[k];
-  if (outside64(P, x, y, z)) return;
-  atomicMax(&out[2 * k], (unsigned long long)__double_as_longlong(x));
-  if (x <= band) atomicMax(&out[2 * k + 1], (unsigned long long)__double_as_longlong(z));
+  double front = 0.0, height = 0.0;
+  int k = 0;
+  if (i < n) {
+    const int ty = st.TY[i];
+    k = inst_of(ty);
+    if (type_of(ty) == 1) {
+      const double x = st.X[0][i], z = st.X[2][i];
+      const double y = st.X[1] ? st.X[1][i] : 0.0;
+      if (!outside64(T.P[k], x, y, z)) {
+        front = x;
+        if (x <= band) height = z;
+      }
+    }
+  }
+  // reduce over the lanes of each distinct instance in the wave (one pass
+  // when the wave lies inside one instance), then one atomic per instance
+  // every lane of the wave runs the loop (no early exit above)
+  bool todo = i < n;
+  for (unsigned long long pending = __ballot(todo); pending; pending = __ballot(todo)) {
+    const int first = __ffsll((long long)pending) - 1;
+    const int u = __shfl(k, first);
+    const bool mine = todo && k == u;
+    double f = mine ? front : 0.0, h = mine ? height : 0.0;
+    for (int off = 32; off > 0; off >>= 1) {
+      f = fmax(f, __shfl_xor(f, off));
+      h = fmax(h, __shfl_xor(h, off));
+    }
+    if ((int)__lane_id() == first) {
+      atomicMax(&out[2 * u], (unsigned long long)__double_as_longlong(f));
+      atomicMax(&out[2 * u + 1], (unsigned long long)__double_as_longlong(h));
+    }
+    if (mine) todo = false;
+  }
 }
 
 template <class K2, class K3, class... A>

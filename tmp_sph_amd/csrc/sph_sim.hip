@@ -308,6 +308,12 @@ extern "C" int sph_sim_num_instances(sph_sim* s, int* K) {
   return SPH_OK;
 }
 
+extern "C" int sph_sim_instance_params(sph_sim* s, int k, sph_params* p) {
+  if (!s || !p || k < 0 || k >= s->K) return set_error(SPH_E_INVAL, "sph_sim_instance_params: bad argument");
+  *p = s->P[k];
+  return SPH_OK;
+}
+
 extern "C" int sph_sim_get_tuning(sph_sim* s, sph_tuning* t) {
   if (!s || !t) return set_error(SPH_E_INVAL, "null argument");
   *t = s->tun;
@@ -571,6 +577,47 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   if (rc) return rc;
   if (best) *best = s->tun;
   if (best_us) *best_us = pick_us;
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_get_state_device(sph_sim* s, float* pos, float* vel, float* rho, float* p, int* type) {
+  if (!s) return set_error(SPH_E_INVAL, "null sim");
+  SPH_HIP(launch_download_fast(s->set[0], pos, vel, rho, p, type, nullptr, s->n, s->stream), "get state");
+  SPH_HIP(hipStreamSynchronize(s->stream), "get state sync");
+  return SPH_OK;
+}
+
+// instance id of every particle (concatenated order), rebuilt on the device
+__global__ void inst_of_index_kernel(const int* __restrict__ starts, int K, int* __restrict__ inst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = K - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (starts[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  inst[i] = lo;
+}
+
+extern "C" int sph_sim_set_state_device(sph_sim* s, const float* pos, const float* vel, const float* rho,
+                                        const float* p, const int* type) {
+  if (!s || !pos || !vel || !rho || !p || !type) return set_error(SPH_E_INVAL, "sph_sim_set_state_device: null array");
+  // the instance map is rebuilt into the work array idx_alt (free between steps)
+  std::vector<int> starts(s->K);
+  for (int k = 0, o = 0; k < s->K; ++k) {
+    starts[k] = o;
+    o += s->P[k].numParticles;
+  }
+  int* dstarts = s->w.keys_alt;  // K <= n ints of scratch
+  SPH_HIP(hipMemcpyAsync(dstarts, starts.data(), sizeof(int) * s->K, hipMemcpyHostToDevice, s->stream), "set state");
+  hipLaunchKernelGGL(inst_of_index_kernel, dim3((s->n + 255) / 256), dim3(256), 0, s->stream, dstarts, s->K,
+                     s->w.idx_alt, s->n);
+  SPH_HIP(hipGetLastError(), "set state");
+  SPH_HIP(launch_upload_fast(s->dim, s->set[0], pos, vel, rho, p, type, s->w.idx_alt, s->n, s->stream), "set state");
+  SPH_HIP(hipStreamSynchronize(s->stream), "set state sync");
   return SPH_OK;
 }
 

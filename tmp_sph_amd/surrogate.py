@@ -25,7 +25,15 @@ W = 128
 class GnnCfg(C.Structure):
     _fields_ = [("layers", C.c_int), ("history", C.c_int), ("max_nodes", C.c_int), ("max_edges", C.c_int),
                 ("train", C.c_int), ("seed", C.c_uint), ("v_scale", C.c_float), ("dv_scale", C.c_float),
-                ("drho_scale", C.c_float), ("rho_scale", C.c_float)]
+                ("drho_scale", C.c_float), ("rho_scale", C.c_float), ("out_mean", C.c_float * 3)]
+
+
+class RolloutStats(C.Structure):
+    _fields_ = [("accepted", C.c_int), ("fallbacks", C.c_int), ("plain_steps", C.c_longlong),
+                ("max_dev", C.c_double), ("mean_dev", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def gnn_layout(layers: int) -> np.ndarray:
@@ -47,10 +55,10 @@ class Gnn:
     """The surrogate network on the current device."""
 
     def __init__(self, layers=10, history=5, max_nodes=60_000, max_edges=1_500_000, train=True, seed=1,
-                 v_scale=1.0, dv_scale=0.1, drho_scale=1.0, rho_scale=100.0):
+                 v_scale=1.0, dv_scale=0.1, drho_scale=1.0, rho_scale=100.0, out_mean=(0.0, 0.0, 0.0)):
         load()
         self.cfg = GnnCfg(layers, history, max_nodes, max_edges, 1 if train else 0, seed, v_scale, dv_scale,
-                          drho_scale, rho_scale)
+                          drho_scale, rho_scale, (C.c_float * 3)(*out_mean))
         h = C.c_void_p()
         call("sph_gnn_create", C.byref(self.cfg), C.byref(h))
         self._h = h
@@ -127,6 +135,16 @@ class Gnn:
     def adam(self, lr: float, clip: float = 1.0) -> None:
         call("sph_gnn_adam", self._h, C.c_float(lr), C.c_float(clip))
 
+    def rollout(self, sim: Simulation, k: int, intervals: int, dev_tol: float = 0.01, band: float = 0.02):
+        """Advances `sim` by `intervals` x k steps with the surrogate, judged
+        and backed up by the plain step; returns (stats, front, height)."""
+        f = np.zeros(max(intervals, 1))
+        h = np.zeros(max(intervals, 1))
+        st = RolloutStats()
+        call("sph_gnn_rollout", self._h, sim._h, int(k), int(intervals), C.c_float(dev_tol), C.c_double(band),
+             _hp(f), _hp(h), C.byref(st))
+        return st.as_dict(), f[:intervals], h[:intervals]
+
 
 def mfma_probe(A: np.ndarray, B: np.ndarray) -> np.ndarray:
     """D = A (16x32) @ B (32x16) through one gfx950 MFMA (inputs rounded to BF16)."""
@@ -163,6 +181,84 @@ class Dataset:
     k: int
     history: int
     snaps: list = field(default_factory=list)
+
+
+def record_batch(cases: list, nsteps: int, k: int, history: int) -> Dataset:
+    """record() for several cases at once, stepped as one batched context."""
+    ds = Dataset(k, history)
+    frames = []
+    with Simulation(cases) as sim:
+        for i in range(nsteps // k + 1):
+            frames.append(sim.state())
+            if i < nsteps // k:
+                sim.step(k)
+        offs = sim.offsets
+    for c, case in enumerate(cases):
+        lo, hi = offs[c], offs[c + 1]
+        fr = [{key: v[lo:hi] for key, v in f.items()} for f in frames]
+        for t in range(history - 1, len(fr) - 1):
+            cur, nxt = fr[t], fr[t + 1]
+            hist = np.stack([fr[t - j]["vel"] for j in range(history)])
+            tgt = np.stack([nxt["vel"][:, 0] - cur["vel"][:, 0], nxt["vel"][:, 2] - cur["vel"][:, 2],
+                            nxt["rho"] - cur["rho"]], 1).astype(np.float32)
+            ds.snaps.append(Snapshot(case.params, cur["pos"], hist.astype(np.float32), cur["rho"], cur["type"], tgt))
+    return ds
+
+
+def normalisation(snaps: list) -> dict:
+    """Output scales and means over the fluid particles of the snapshots."""
+    tg = np.concatenate([s.target[s.type == 1] for s in snaps]).astype(np.float64)
+    mean = tg.mean(0)
+    std = tg.std(0) + 1e-9
+    return {"dv_scale": float(max(std[0], std[1])), "drho_scale": float(std[2]),
+            "out_mean": (float(mean[0]), float(mean[1]), float(mean[2]))}
+
+
+def normalised_target(s: Snapshot, norm: dict) -> np.ndarray:
+    m = np.asarray(norm["out_mean"], np.float64)
+    sc = np.array([norm["dv_scale"], norm["dv_scale"], norm["drho_scale"]])
+    return ((s.target - m) / sc).astype(np.float32)
+
+
+def train(g: "Gnn", snaps: list, norm: dict, iters: int, lr: float = 3e-4, lr_final: float = 3e-5,
+          clip: float = 1.0, seed: int = 0, seconds: float | None = None) -> list:
+    """Adam on single-snapshot batches; cosine learning-rate decay.  Stops
+    after `iters` steps or `seconds` of wall-clock, whichever comes first."""
+    import time
+    rng = np.random.default_rng(seed)
+    losses = []
+    t0 = time.time()
+    for it in range(iters):
+        s = snaps[rng.integers(len(snaps))]
+        g.load(s.params, s.pos, s.hist, s.rho, s.type, normalised_target(s, norm))
+        g.zero_grad()
+        _, loss = g.forward(want_out=False)
+        g.backward()
+        frac = it / max(iters - 1, 1)
+        g.adam(lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac)), clip)
+        losses.append(loss)
+        if seconds is not None and time.time() - t0 > seconds:
+            break
+    return losses
+
+
+def evaluate(g: "Gnn", snaps: list, norm: dict) -> dict:
+    """Held-out mean squared error of the normalised outputs, next to the
+    errors of the method without the network: predicting the mean change
+    (constant acceleration) and predicting no change (constant velocity)."""
+    net, mean_pred, zero_pred = [], [], []
+    m = np.asarray(norm["out_mean"], np.float64)
+    sc = np.array([norm["dv_scale"], norm["dv_scale"], norm["drho_scale"]])
+    for s in snaps:
+        t = normalised_target(s, norm)
+        g.load(s.params, s.pos, s.hist, s.rho, s.type, t)
+        _, loss = g.forward(want_out=False)
+        fl = s.type == 1
+        net.append(loss)
+        mean_pred.append(float(np.mean(t[fl].astype(np.float64) ** 2)))
+        zero_pred.append(float(np.mean(((0.0 - m) / sc - t[fl]) ** 2)))
+    return {"network": float(np.mean(net)), "constant_acceleration": float(np.mean(mean_pred)),
+            "constant_velocity": float(np.mean(zero_pred))}
 
 
 def record(case: Case, nsteps: int, k: int, history: int) -> Dataset:
