@@ -191,6 +191,10 @@ typedef struct sph_tuning {
     int nbr_unroll;      /* candidate loads issued together: 1, 2, 4 or 8 */
     int xcd_remap;       /* 1: each XCD gets a contiguous run of target tiles */
     int flags;           /* bit 0: per-kernel HIP event timing in sph_sim_step */
+    int nbr_lds;         /* 1: one-wave blocks stage the wave's candidate rows in LDS
+                            (nbr_block is then ignored) */
+    int nbr_compact;     /* 1: scan candidates into per-lane LDS lists, then run the
+                            pair body densely over the accepted ones */
 } sph_tuning;
 
 /* Creates a context on the current device from host arrays in the layout of
@@ -332,6 +336,16 @@ int sph_gnn_load_state(sph_gnn* g, const sph_params* p, int n, const float* pos,
 int sph_gnn_graph(sph_gnn* g, int* edges, int* offsets, int* senders, int* receivers);
 /* Node and edge input features (128 wide, FP32) of the loaded state. */
 int sph_gnn_features(sph_gnn* g, float* node_feat, float* edge_feat);
+/* Activations of the last forward pass (needs train = 1), for checking the
+   backward pass against a CPU restatement evaluated at the same activations.
+   kind 0/1: hidden activations h1/h2 of MLP call `index` (BF16 as uint16,
+   rows x 128; calls: 0 node encoder, 1 edge encoder, 2 + 2l edge update and
+   3 + 2l node update of block l, 2 + 2L decoder; rows = edges for the edge
+   encoder and edge updates, nodes otherwise); 2: node latents V_index
+   (n x 128 FP32, index 0..L); 3: edge latents E_index (edges x 128);
+   4: aggregated messages A_index (n x 128, index 0..L-1); 5: decoder output
+   (n x 128 FP32, columns 3.. are zero).  count must equal the element count. */
+int sph_gnn_get_tensor(sph_gnn* g, int kind, int index, void* host, long long count);
 /* Forward pass; out[n][3] (normalised outputs) may be null; *loss gets the
    mean squared error over fluid nodes when a target was loaded. */
 int sph_gnn_forward(sph_gnn* g, float* out, double* loss);

@@ -129,6 +129,86 @@ void mlp_bwd(const Net& net, int first, const MlpAct& a, int M, const Vec& dout,
   if (dx) lin_back(net, first, dz1, M, nullptr, *dx);
 }
 
+struct Acts {
+  MlpAct en, ee, dec;
+  std::vector<MlpAct> e, v;  // per processor block
+};
+
+// Backward pass from dY (n x 128) given every MLP's recorded activations;
+// accumulates parameter gradients into G.
+void backward(const Net& net, int L, int n, int ne, const int* send, const int* recv, const Acts& a,
+              const Vec& dY, double* G) {
+  const int enc_n = 0, enc_e = 3, dec = 6 + 6 * L;
+  auto pe = [](int l) { return 6 + 6 * l; };
+  auto pn = [](int l) { return 9 + 6 * l; };
+  Vec dV, dE((size_t)ne * W, 0.0), dx;
+  mlp_bwd(net, dec, a.dec, n, dY, G, &dV);
+  for (int l = L - 1; l >= 0; --l) {
+    // node update V_{l+1} = V_l + MLP([V_l, A_l]); A_l = sum of E_{l+1} over received edges
+    mlp_bwd(net, pn(l), a.v[l], n, dV, G, &dx);
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < W; ++c) dV[(size_t)i * W + c] += dx[(size_t)i * 2 * W + c];
+    for (int e = 0; e < ne; ++e)
+      for (int c = 0; c < W; ++c) dE[(size_t)e * W + c] += dx[(size_t)recv[e] * 2 * W + W + c];
+    // edge update E_{l+1} = E_l + MLP([E_l, V_l[recv], V_l[send]])
+    mlp_bwd(net, pe(l), a.e[l], ne, dE, G, &dx);
+    for (int e = 0; e < ne; ++e)
+      for (int c = 0; c < W; ++c) {
+        dE[(size_t)e * W + c] += dx[(size_t)e * 3 * W + c];
+        dV[(size_t)recv[e] * W + c] += dx[(size_t)e * 3 * W + W + c];
+        dV[(size_t)send[e] * W + c] += dx[(size_t)e * 3 * W + 2 * W + c];
+      }
+  }
+  mlp_bwd(net, enc_e, a.ee, ne, dE, G, nullptr);
+  mlp_bwd(net, enc_n, a.en, n, dV, G, nullptr);
+}
+
+Vec edge_input(const Vec& E, const Vec& V, int ne, const int* send, const int* recv) {
+  Vec in((size_t)ne * 3 * W);
+  for (int e = 0; e < ne; ++e)
+    for (int c = 0; c < W; ++c) {
+      in[(size_t)e * 3 * W + c] = E[(size_t)e * W + c];
+      in[(size_t)e * 3 * W + W + c] = V[(size_t)recv[e] * W + c];
+      in[(size_t)e * 3 * W + 2 * W + c] = V[(size_t)send[e] * W + c];
+    }
+  return in;
+}
+
+Vec node_input(const Vec& V, const Vec& A, int n) {
+  Vec in((size_t)n * 2 * W);
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < W; ++c) {
+      in[(size_t)i * 2 * W + c] = V[(size_t)i * W + c];
+      in[(size_t)i * 2 * W + W + c] = A[(size_t)i * W + c];
+    }
+  return in;
+}
+
+// mean squared error over fluid nodes and 3 outputs, and its gradient
+double loss_grad(int n, const Vec& Y, const float* target, const int* type, Vec& dY) {
+  int nf = 0;
+  for (int i = 0; i < n; ++i) nf += type[i] == 1;
+  const double inv = nf ? 1.0 / (3.0 * nf) : 0.0;
+  double loss = 0.0;
+  dY.assign((size_t)n * W, 0.0);
+  for (int i = 0; i < n; ++i) {
+    if (type[i] != 1) continue;
+    for (int c = 0; c < 3; ++c) {
+      const double r = Y[(size_t)i * W + c] - target[(size_t)i * 3 + c];
+      loss += r * r * inv;
+      dY[(size_t)i * W + c] = 2.0 * r * inv;
+    }
+  }
+  return loss;
+}
+
+double bf16_to_double(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return (double)f;
+}
+
 }  // namespace
 
 extern "C" {
@@ -145,73 +225,76 @@ double orc_gnn_loss(int L, const float* params, const int* layout5, int nlin, in
   auto pn = [](int l) { return 9 + 6 * l; };
   Vec xn(Xn, Xn + (size_t)n * W), xe(Xe, Xe + (size_t)ne * W);
   std::vector<Vec> V(L + 1), E(L + 1), A(L);
-  std::vector<MlpAct> ae(L), an(L);
-  MlpAct aen, aee, adec;
-  mlp_fwd(net, enc_n, xn, n, aen, V[0]);
-  mlp_fwd(net, enc_e, xe, ne, aee, E[0]);
+  Acts act;
+  act.e.resize(L);
+  act.v.resize(L);
+  mlp_fwd(net, enc_n, xn, n, act.en, V[0]);
+  mlp_fwd(net, enc_e, xe, ne, act.ee, E[0]);
   for (int l = 0; l < L; ++l) {
-    Vec in((size_t)ne * 3 * W);
-    for (int e = 0; e < ne; ++e)
-      for (int c = 0; c < W; ++c) {
-        in[(size_t)e * 3 * W + c] = E[l][(size_t)e * W + c];
-        in[(size_t)e * 3 * W + W + c] = V[l][(size_t)recv[e] * W + c];
-        in[(size_t)e * 3 * W + 2 * W + c] = V[l][(size_t)send[e] * W + c];
-      }
     Vec d;
-    mlp_fwd(net, pe(l), in, ne, ae[l], d);
+    mlp_fwd(net, pe(l), edge_input(E[l], V[l], ne, send, recv), ne, act.e[l], d);
     E[l + 1] = E[l];
     for (size_t i = 0; i < d.size(); ++i) E[l + 1][i] += d[i];
     A[l].assign((size_t)n * W, 0.0);
     for (int i = 0; i < n; ++i)
       for (int e = off[i]; e < off[i + 1]; ++e)
         for (int c = 0; c < W; ++c) A[l][(size_t)i * W + c] += E[l + 1][(size_t)e * W + c];
-    Vec nin((size_t)n * 2 * W);
-    for (int i = 0; i < n; ++i)
-      for (int c = 0; c < W; ++c) {
-        nin[(size_t)i * 2 * W + c] = V[l][(size_t)i * W + c];
-        nin[(size_t)i * 2 * W + W + c] = A[l][(size_t)i * W + c];
-      }
-    mlp_fwd(net, pn(l), nin, n, an[l], d);
+    mlp_fwd(net, pn(l), node_input(V[l], A[l], n), n, act.v[l], d);
     V[l + 1] = V[l];
     for (size_t i = 0; i < d.size(); ++i) V[l + 1][i] += d[i];
   }
   Vec Y;
-  mlp_fwd(net, dec, V[L], n, adec, Y);
+  mlp_fwd(net, dec, V[L], n, act.dec, Y);
   if (Yout)
     for (int i = 0; i < n; ++i)
       for (int c = 0; c < 3; ++c) Yout[(size_t)i * 3 + c] = Y[(size_t)i * W + c];
-  int nf = 0;
-  for (int i = 0; i < n; ++i) nf += type[i] == 1;
-  const double inv = nf ? 1.0 / (3.0 * nf) : 0.0;
-  double loss = 0.0;
-  Vec dY((size_t)n * W, 0.0);
-  for (int i = 0; i < n; ++i) {
-    if (type[i] != 1) continue;
-    for (int c = 0; c < 3; ++c) {
-      const double r = Y[(size_t)i * W + c] - target[(size_t)i * 3 + c];
-      loss += r * r * inv;
-      dY[(size_t)i * W + c] = 2.0 * r * inv;
-    }
+  Vec dY;
+  const double loss = loss_grad(n, Y, target, type, dY);
+  if (G) backward(net, L, n, ne, send, recv, act, dY, G);
+  return loss;
+}
+
+// Gradient of the BF16 network evaluated at activations recorded by another
+// implementation: Vs (L+1) x n x 128 and Es (L+1) x ne x 128 node and edge
+// latents, As L x n x 128 aggregated messages, H the hidden activations h1, h2
+// (BF16 bit patterns) of MLP calls 0 .. 2+2L in order (rows = ne for calls 1
+// and 2+2l, n otherwise), Y n x 128 decoder output.  The backward pass is the
+// one orc_gnn_loss runs (operands rounded to BF16 where the device rounds
+// them), but its ReLU masks and layer inputs come from the recorded forward,
+// so it does not inherit the forward's sensitivity to rounding-boundary
+// crossings.  Returns the loss of Y.
+double orc_gnn_backward_at(int L, const float* params, const int* layout5, int nlin, int n, int ne,
+                           const float* Xn, const float* Xe, const int* send, const int* recv, const float* target,
+                           const int* type, const float* Vs, const float* Es, const float* As, const uint16_t* H,
+                           const float* Yin, double* G) {
+  Net net{L, params, std::vector<int>(layout5, layout5 + 5 * nlin), true};
+  auto vec = [](const float* p, size_t count) { return Vec(p, p + count); };
+  const size_t nW = (size_t)n * W, eW = (size_t)ne * W;
+  const uint16_t* h = H;
+  auto take = [&](MlpAct& a, Vec x, int rows) {
+    a.x = std::move(x);
+    a.h1.resize((size_t)rows * W);
+    a.h2.resize((size_t)rows * W);
+    for (size_t i = 0; i < a.h1.size(); ++i) a.h1[i] = bf16_to_double(h[i]);
+    h += (size_t)rows * W;
+    for (size_t i = 0; i < a.h2.size(); ++i) a.h2[i] = bf16_to_double(h[i]);
+    h += (size_t)rows * W;
+  };
+  Acts act;
+  act.e.resize(L);
+  act.v.resize(L);
+  take(act.en, vec(Xn, nW), n);
+  take(act.ee, vec(Xe, eW), ne);
+  for (int l = 0; l < L; ++l) {
+    const Vec Vl = vec(Vs + l * nW, nW), El = vec(Es + l * eW, eW), Al = vec(As + l * nW, nW);
+    take(act.e[l], edge_input(El, Vl, ne, send, recv), ne);
+    take(act.v[l], node_input(Vl, Al, n), n);
   }
-  if (!G) return loss;
-  Vec dV, dE((size_t)ne * W, 0.0), dx;
-  mlp_bwd(net, dec, adec, n, dY, G, &dV);
-  for (int l = L - 1; l >= 0; --l) {
-    mlp_bwd(net, pn(l), an[l], n, dV, G, &dx);
-    for (int i = 0; i < n; ++i)
-      for (int c = 0; c < W; ++c) dV[(size_t)i * W + c] += dx[(size_t)i * 2 * W + c];
-    for (int e = 0; e < ne; ++e)
-      for (int c = 0; c < W; ++c) dE[(size_t)e * W + c] += dx[(size_t)recv[e] * 2 * W + W + c];
-    mlp_bwd(net, pe(l), ae[l], ne, dE, G, &dx);
-    for (int e = 0; e < ne; ++e)
-      for (int c = 0; c < W; ++c) {
-        dE[(size_t)e * W + c] += dx[(size_t)e * 3 * W + c];
-        dV[(size_t)recv[e] * W + c] += dx[(size_t)e * 3 * W + W + c];
-        dV[(size_t)send[e] * W + c] += dx[(size_t)e * 3 * W + 2 * W + c];
-      }
-  }
-  mlp_bwd(net, enc_e, aee, ne, dE, G, nullptr);
-  mlp_bwd(net, enc_n, aen, n, dV, G, nullptr);
+  take(act.dec, vec(Vs + L * nW, nW), n);
+  const Vec Y = vec(Yin, nW);
+  Vec dY;
+  const double loss = loss_grad(n, Y, target, type, dY);
+  backward(net, L, n, ne, send, recv, act, dY, G);
   return loss;
 }
 

@@ -38,15 +38,16 @@ def load() -> C.CDLL:
         "orc_calcHash": ([sp, P, P, P, I], None),
         "orc_sortParticles": ([sp, P, P, I], None),
         "orc_reorder": ([sp] + [P] * 14 + [I, P, P], None),
-        "orc_SPH_NS": ([sp] + [P] * 11 + [I, P, P, P], None),
+        "orc_SPH_NS": ([sp] + [P] * 11 + [I, P, P, P, P], None),
         "orc_PC": ([sp, I] + [P] * 10 + [I, P], None),
         "orc_eos_pressure": ([sp, C.c_float], C.c_float),
         "orc_eos_density": ([sp, C.c_float], C.c_float),
         "orc_eos_pressure64": ([sp, C.c_double], C.c_double),
         "orc_eos_density64": ([sp, C.c_double], C.c_double),
         "orc_run": ([sp, I, I] + [P] * 6 + [I, C.c_double, I, P, P, P], I),
-        "orc_eval": ([sp, I, I] + [P] * 15, None),
+        "orc_eval": ([sp, I, I] + [P] * 16, None),
         "orc_gnn_loss": ([I, P, P, I, I, I] + [P] * 7 + [I, P, P], C.c_double),
+        "orc_gnn_backward_at": ([I, P, P, I, I, I] + [P] * 11, C.c_double),
     }
     for name, (args, res) in protos.items():
         f = getattr(lib, name)
@@ -97,18 +98,21 @@ def reorder(prm, pos, vel, rho, p, typ, h, idx, table_r=None):
 
 
 def SPH_NS(prm, sPos, sVel, sRho, sP, sType, cellStart, cellEnd, h):
-    """Kernels A and B on sorted reference-layout arrays; returns outputs and magnitude sums."""
+    """Kernels A and B on sorted reference-layout arrays; returns outputs, magnitude
+    sums (mag_*) and sens_drho, the absolute change of drho that 2-ulp changes of
+    the input densities can cause (an additive error budget)."""
     n = sPos.shape[0]
     rho = np.array(sRho, np.float32)
     p = np.array(sP, np.float32)
     out = {"drho": np.empty(n, np.float32), "acc": np.empty((n, 3), np.float32), "dofv": np.empty(n, np.float32),
-           "mag_drho": np.empty(n), "mag_acc": np.empty((n, 3)), "mag_dofv": np.empty(n)}
+           "mag_drho": np.empty(n), "mag_acc": np.empty((n, 3)), "mag_dofv": np.empty(n),
+           "sens_drho": np.empty(n)}
     load().orc_SPH_NS(C.byref(prm), _hp(np.ascontiguousarray(sPos, np.float32)),
                       _hp(np.ascontiguousarray(sVel, np.float32)), _hp(rho), _hp(p),
                       _hp(np.ascontiguousarray(sType, np.int32)), _hp(out["drho"]), _hp(out["acc"]),
                       _hp(np.ascontiguousarray(cellStart, np.int32)), _hp(np.ascontiguousarray(cellEnd, np.int32)),
                       _hp(np.ascontiguousarray(h, np.int32)), _hp(out["dofv"]), n, _hp(out["mag_drho"]),
-                      _hp(out["mag_acc"]), _hp(out["mag_dofv"]))
+                      _hp(out["mag_acc"]), _hp(out["mag_dofv"]), _hp(out["sens_drho"]))
     out["rho"] = rho
     out["p"] = p
     return out
@@ -173,17 +177,35 @@ def gnn_loss(layers: int, params: np.ndarray, layout: np.ndarray, Xn, Xe, off, s
     return (loss, G, Y) if outputs else (loss, G)
 
 
+def gnn_backward_at(layers: int, params: np.ndarray, layout: np.ndarray, Xn, Xe, send, recv, target, typ,
+                    V, E, A, H, Y):
+    """Loss and FP64 gradient of the BF16 network at recorded activations:
+    V (L+1, n, 128), E (L+1, ne, 128), A (L, n, 128), H the h1/h2 BF16 bit
+    patterns of every MLP call in order (flattened), Y (n, 128)."""
+    params = np.ascontiguousarray(params, np.float32)
+    layout = np.ascontiguousarray(layout, np.int32)
+    G = np.zeros(params.size, np.float64)
+    n, ne = Xn.shape[0], Xe.shape[0]
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    i32 = lambda a: np.ascontiguousarray(a, np.int32)
+    loss = load().orc_gnn_backward_at(int(layers), _hp(params), _hp(layout), layout.size // 5, n, ne, _hp(f32(Xn)),
+                                      _hp(f32(Xe)), _hp(i32(send)), _hp(i32(recv)), _hp(f32(target)), _hp(i32(typ)),
+                                      _hp(f32(V)), _hp(f32(E)), _hp(f32(A)),
+                                      _hp(np.ascontiguousarray(H, np.uint16)), _hp(f32(Y)), _hp(G))
+    return loss, G
+
+
 def eval_rates(prm, case, mode: int = 1) -> dict:
     """Hash, sort, gather and kernels A and B of the state (original order in/out)."""
     n = case.n
     out = {"drho": np.empty(n, np.float32), "acc": np.empty((n, 3), np.float32), "dofv": np.empty(n, np.float32),
            "p": np.empty(n, np.float32), "rho": np.empty(n, np.float32), "keys": np.empty(n, np.int32),
            "perm": np.empty(n, np.int32), "mag_drho": np.empty(n), "mag_acc": np.empty((n, 3)),
-           "mag_dofv": np.empty(n)}
+           "mag_dofv": np.empty(n), "sens_drho": np.empty(n)}
     load().orc_eval(C.byref(prm), int(mode), n, _hp(np.ascontiguousarray(case.pos, np.float32)),
                     _hp(np.ascontiguousarray(case.vel, np.float32)), _hp(np.ascontiguousarray(case.rho, np.float32)),
                     _hp(np.ascontiguousarray(case.p, np.float32)), _hp(np.ascontiguousarray(case.type, np.int32)),
                     _hp(out["drho"]), _hp(out["acc"]), _hp(out["dofv"]), _hp(out["p"]), _hp(out["rho"]),
                     _hp(out["keys"]), _hp(out["perm"]), _hp(out["mag_drho"]), _hp(out["mag_acc"]),
-                    _hp(out["mag_dofv"]))
+                    _hp(out["mag_dofv"]), _hp(out["sens_drho"]))
     return out

@@ -107,9 +107,10 @@ inline void for_candidates(const sph_params& P, int hash, const int* cellStart,
 
 // Magnitude sums used as the tolerance scale of near-cancelling results.
 struct Mags {
-  double* drho;  // sum |continuity terms|
-  double* acc;   // per component sum |vtmp * dx_k|  (3N)
-  double* dofv;  // sum |divergence terms| (filled by kernel A, read by kernel B)
+  double* drho;       // sum |continuity terms|
+  double* acc;        // per component sum |vtmp * dx_k|  (3N)
+  double* dofv;       // sum |divergence terms| (filled by kernel A, read by kernel B)
+  double* sens_drho;  // absolute change of drho that 2-ulp input densities can cause
 };
 
 // Kernel A (opt_src_v1/equation/ns.cu:30-134).
@@ -181,7 +182,7 @@ void kernelB(const sph_params& P, const R* sPos, const R* sVel, const R* sRho, c
     const R pres0 = sP[i], dens0 = sRho[i], dofv0 = dofv[i];
     const int ti = sType[i];
     R dd = 0, ax = 0, ay = 0, az = 0;
-    double md = 0, mx = 0, my = 0, mz = 0;
+    double md = 0, mx = 0, my = 0, mz = 0, sd = 0;
     for_candidates<DIM>(P, hash[i], cellStart, cellEnd, [&](int j) {
       if (j == i) return;
       const R dx = px - sPos[3 * j], dy = py - sPos[3 * j + 1], dz = pz - sPos[3 * j + 2];
@@ -197,9 +198,14 @@ void kernelB(const sph_params& P, const R* sPos, const R* sVel, const R* sRho, c
         // the diffusion term takes the difference of two FP32 densities near
         // rho0, so a one-ulp change of either input (e.g. a wall density from
         // kernel A) moves it by |d t2 / d rho| * ulp: part of the scale
-        const double ulp_rho = 2.0 * std::ldexp(std::max(std::fabs((double)dens0), std::fabs((double)dens1)), -23);
-        const double t2_sens = std::fabs((double)rr * fr * delta * h * cs / (rr + eta * eta)) * ulp_rho;
-        md += (std::fabs((double)t1) + std::fabs((double)t2) + t2_sens) * (double)m / (double)dens1;
+        // (an absolute error budget, not a magnitude: near rho0 one ulp of a
+        // wall density moves drho by far more than 1e-5 of the term sums).
+        // Only wall densities are computed here; fluid densities are inputs.
+        if (ti != 1 || sType[j] != 1) {
+          const double ulp_rho = 2.0 * std::ldexp(std::max(std::fabs((double)dens0), std::fabs((double)dens1)), -23);
+          sd += std::fabs((double)rr * fr * delta * h * cs / (rr + eta * eta)) * ulp_rho * (double)m / (double)dens1;
+        }
+        md += (std::fabs((double)t1) + std::fabs((double)t2)) * (double)m / (double)dens1;
         R vtmp = -(pres0 + pres1) * m * fr / (dens0 * dens1);
         double vmag = std::fabs((double)vtmp);
         if (ti == 1 && sType[j] == 1) {
@@ -224,6 +230,7 @@ void kernelB(const sph_params& P, const R* sPos, const R* sVel, const R* sRho, c
     acc[3 * i + 1] = ay;
     acc[3 * i + 2] = az + (R)P.gravity;
     if (mags.drho) mags.drho[i] = md;
+    if (mags.sens_drho) mags.sens_drho[i] = sd;
     if (mags.acc) {
       mags.acc[3 * i] = mx;
       mags.acc[3 * i + 1] = my;
@@ -441,7 +448,7 @@ struct Sim {
   }
 
   void step() {
-    Mags none{nullptr, nullptr, nullptr};
+    Mags none{nullptr, nullptr, nullptr, nullptr};
     sort_into(0, 1);
     rates(1, none);
     integrate(1, true);
@@ -504,12 +511,13 @@ int run_mode(const sph_params* P, int n, float* pos, double* pos64, float* vel, 
 template <class R, class X>
 void eval_mode(const sph_params* P, int n, const float* pos, const float* vel, const float* rho,
                const float* p, const int* type, float* drho, float* acc, float* dofv, float* p_out,
-               float* rho_out, int* keys, int* perm, double* mag_drho, double* mag_acc, double* mag_dofv) {
+               float* rho_out, int* keys, int* perm, double* mag_drho, double* mag_acc, double* mag_dofv,
+               double* sens_drho) {
   Sim<R, X> sim;
   sim.init(*P, n, pos, vel, rho, p, type);
   sim.sort_into(0, 1);
-  std::vector<double> md(n), ma(3 * (size_t)n), mdv(n);
-  sim.rates(1, Mags{md.data(), ma.data(), mdv.data()});
+  std::vector<double> md(n), ma(3 * (size_t)n), mdv(n), sd(n);
+  sim.rates(1, Mags{md.data(), ma.data(), mdv.data(), sd.data()});
   const State<R, X>& st = sim.s[1];
   for (int i = 0; i < n; ++i) {
     const int o = st.id[i];
@@ -523,6 +531,7 @@ void eval_mode(const sph_params* P, int n, const float* pos, const float* vel, c
     }
     if (mag_drho) mag_drho[o] = md[i];
     if (mag_dofv) mag_dofv[o] = mdv[i];
+    if (sens_drho) sens_drho[o] = sd[i];
     if (keys) keys[i] = sim.key[i];
     if (perm) perm[i] = sim.idx[i];
   }
@@ -570,14 +579,14 @@ void orc_reorder(const sph_params* P, const float* oPos, const float* oVel, cons
 void orc_SPH_NS(const sph_params* P, const float* sPos, const float* sVel, float* sRho, float* sP,
                 const int* sType, float* drho, float* acc, const int* cellStart, const int* cellEnd,
                 const int* hash, float* dofv, int n, double* mag_drho, double* mag_acc,
-                double* mag_dofv) {
+                double* mag_dofv, double* sens_drho) {
   std::vector<double> tmp;
   if (!mag_dofv && (mag_drho || mag_acc)) {
     tmp.resize(n);
     mag_dofv = tmp.data();
   }
   run_ns<float>(*P, sPos, sVel, sRho, sP, sType, drho, acc, cellStart, cellEnd, hash, dofv, n,
-                Mags{mag_drho, mag_acc, mag_dofv});
+                Mags{mag_drho, mag_acc, mag_dofv, sens_drho});
 }
 
 // PC_prediction (pred=1) / PC_correction (pred=0) on the reference layout.
@@ -657,13 +666,13 @@ int orc_run(const sph_params* P, int mode, int n, float* pos, double* pos64, flo
 void orc_eval(const sph_params* P, int mode, int n, const float* pos, const float* vel, const float* rho,
               const float* p, const int* type, float* drho, float* acc, float* dofv, float* p_out,
               float* rho_out, int* keys, int* perm, double* mag_drho, double* mag_acc,
-              double* mag_dofv) {
+              double* mag_dofv, double* sens_drho) {
   if (mode == 2)
     eval_mode<double, double>(P, n, pos, vel, rho, p, type, drho, acc, dofv, p_out, rho_out, keys, perm,
-                              mag_drho, mag_acc, mag_dofv);
+                              mag_drho, mag_acc, mag_dofv, sens_drho);
   else
     eval_mode<float, double>(P, n, pos, vel, rho, p, type, drho, acc, dofv, p_out, rho_out, keys, perm,
-                             mag_drho, mag_acc, mag_dofv);
+                             mag_drho, mag_acc, mag_dofv, sens_drho);
 }
 
 }  // extern "C"

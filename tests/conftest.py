@@ -57,12 +57,14 @@ def advanced(orc, case, steps, mode=1):
     return out
 
 
-def rel_err(got, want, scale):
-    """max |got - want| / max(|want|, scale), elementwise scale allowed."""
+def rel_err(got, want, scale, slack=0.0):
+    """max (|got - want| - slack)+ / max(|want|, scale), elementwise scale and
+    slack allowed; slack is an absolute error budget (the oracle's sens_drho:
+    what ulp-level input differences alone can cause)."""
     got = np.asarray(got, np.float64)
     want = np.asarray(want, np.float64)
     if not got.size:
         return 0.0
-    diff = np.abs(got - want)
+    diff = np.maximum(np.abs(got - want) - np.asarray(slack, np.float64), 0.0)
     den = np.maximum(np.maximum(np.abs(want), scale), 1e-300)
     return float(np.max(np.where(diff == 0, 0.0, diff / den)))

@@ -81,3 +81,70 @@ def test_bf16_emulation_is_close_to_fp64(orc):
     assert lb == pytest.approx(l64, rel=0.05)
     cos = float(g64 @ gb / np.linalg.norm(g64) / np.linalg.norm(gb))
     assert cos > 0.99
+
+
+def _bf16_bits(x):
+    """Round to nearest even BF16 (through FP32); returns the 16-bit patterns."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def _from_bits(b):
+    return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _q(x):
+    return _from_bits(_bf16_bits(x))
+
+
+def _forward_recording(layers, params, lay, Xn, Xe, off, send, recv):
+    """An independent NumPy statement of the BF16 forward pass that records
+    what the device keeps: h1/h2 of every MLP call, latents and messages."""
+    L = lay.reshape(-1, 5)
+    H = []
+
+    def lin(li, x):
+        w, N, K, b, _ = L[li]
+        Wm = params[w: w + N * K].reshape(N, K)
+        return _q(x) @ _q(Wm).T + params[b: b + N].astype(np.float64)
+
+    def mlp(first, x):
+        b1 = _bf16_bits(np.maximum(lin(first, x), 0.0))
+        b2 = _bf16_bits(np.maximum(lin(first + 1, _from_bits(b1)), 0.0))
+        H.extend([b1.ravel(), b2.ravel()])
+        return lin(first + 2, _from_bits(b2))
+
+    n = Xn.shape[0]
+    V = [mlp(0, Xn.astype(np.float64))]
+    E = [mlp(3, Xe.astype(np.float64))]
+    A = []
+    for l in range(layers):
+        E.append(E[l] + mlp(6 + 6 * l, np.concatenate([E[l], V[l][recv], V[l][send]], axis=1)))
+        agg = np.zeros((n, 128))
+        np.add.at(agg, recv, E[l + 1])
+        A.append(agg)
+        V.append(V[l] + mlp(9 + 6 * l, np.concatenate([V[l], agg], axis=1)))
+    Y = mlp(6 + 6 * layers, V[layers])
+    return np.stack(V), np.stack(E), np.stack(A), np.concatenate(H), Y
+
+
+@pytest.mark.parametrize("layers", [1, 3])
+def test_backward_at_recorded_activations_matches_oracle(orc, layers):
+    """The gradient the oracle computes at recorded activations (the check the
+    device backward pass is held to) equals its own end-to-end BF16 gradient
+    when the activations come from an independent NumPy forward pass; the two
+    forward statements agree on the outputs."""
+    lay, params, off, send, recv, Xn, Xe, target, typ = _problem(layers, 60, seed=4)
+    loss, G, Yo = orc.gnn_loss(layers, params, lay, Xn, Xe, off, send, recv, target, typ, bf16=True,
+                               outputs=True)
+    V, E, A, H, Y = _forward_recording(layers, params, lay, Xn, Xe, off, send, recv)
+    np.testing.assert_allclose(Y[:, :3], Yo, rtol=1e-6, atol=1e-7)
+    l2, G2 = orc.gnn_backward_at(layers, params, lay, Xn, Xe, send, recv, target, typ, V, E, A, H, Y)
+    assert l2 == pytest.approx(loss, rel=1e-6)
+    assert np.linalg.norm(G2 - G) <= 1e-6 * np.linalg.norm(G)
+    # a mask taken from the wrong activation is caught: zero one hidden layer
+    H_bad = H.copy()
+    H_bad[: 60 * 128] = 0
+    _, G3 = orc.gnn_backward_at(layers, params, lay, Xn, Xe, send, recv, target, typ, V, E, A, H_bad, Y)
+    assert np.linalg.norm(G3 - G) > 1e-3 * np.linalg.norm(G)

@@ -66,28 +66,43 @@ def test_loss_and_gradient_match_oracle(gpu, orc, layers):
     g.backward()
     grad = g.grads().astype(np.float64)
     params = g.params()
+    # the device's own forward activations
+    V = np.stack([g.tensor(2, l) for l in range(layers + 1)])
+    E = np.stack([g.tensor(3, l) for l in range(layers + 1)])
+    A = np.stack([g.tensor(4, l) for l in range(layers)])
+    H = np.concatenate([g.tensor(k, c).ravel() for c in range(3 + 2 * layers) for k in (0, 1)])
+    Y = g.tensor(5)
+    np.testing.assert_array_equal(Y[:, :3], out)
+    # 1. backward pass: the oracle's BF16 backward evaluated at those activations
+    la, ga = orc.gnn_backward_at(layers, params, g.layout, xn, xe, send, recv, target, case.type, V, E, A, H, Y)
+    rel = np.linalg.norm(grad - ga) / np.linalg.norm(ga)
+    print(f"layers={layers} n={case.n} edges={send.size} backward rel err {rel:.3e} loss {loss:.7f} vs {la:.7f}")
+    lay = g.layout.reshape(-1, 5)
+    worst = 0.0
+    for li, (w, N, K, b, _) in enumerate(lay):
+        sl = np.r_[w: w + N * K, b: b + N]
+        e = np.linalg.norm(grad[sl] - ga[sl]) / (np.linalg.norm(ga[sl]) + 1e-30)
+        worst = max(worst, e)
+        print(f"  lin {li:2d} K={K:3d} |g|={np.linalg.norm(ga[sl]):.3e} err={e:.2e}")
+    assert loss == pytest.approx(la, rel=1e-5)
+    assert rel < 1e-3, rel
+    assert worst < 1e-2, worst
+    # 2. forward pass against the oracle's own BF16 forward.  The network is
+    # discontinuous at BF16 rounding boundaries of its hidden activations:
+    # FP32 instead of FP64 accumulation alone moves the outputs by ~2e-3 and
+    # per-layer gradients by a few percent, so the end-to-end comparison is
+    # loose and the backward pass is held to check 1.
     lb, gb, yb = orc.gnn_loss(layers, params, g.layout, xn, xe, off, send, recv, target, case.type, bf16=True,
                               outputs=True)
-    l64, g64 = orc.gnn_loss(layers, params, g.layout, xn, xe, off, send, recv, target, case.type, bf16=False)
-    # diagnostics: outputs, then the gradient error of every linear layer
     yerr = np.linalg.norm(out - yb) / np.linalg.norm(yb)
-    print(f"layers={layers} n={case.n} edges={send.size} output rel err {yerr:.3e} loss {loss:.6f} vs {lb:.6f}")
-    lay = g.layout.reshape(-1, 5)
-    for li, (w, N, K, b, _) in enumerate(lay):
-        gw, ow = grad[w: w + N * K], gb[w: w + N * K]
-        gbb, obb = grad[b: b + N], gb[b: b + N]
-        ew = np.linalg.norm(gw - ow) / (np.linalg.norm(ow) + 1e-30)
-        eb = np.linalg.norm(gbb - obb) / (np.linalg.norm(obb) + 1e-30)
-        print(f"  lin {li:2d} K={K:3d} |gW|={np.linalg.norm(ow):.3e} errW={ew:.2e} |gb|={np.linalg.norm(obb):.3e} "
-              f"errb={eb:.2e}")
-    assert yerr < 1e-2, yerr
-    # same BF16 rounding points: FP32 accumulation is the only difference
+    print(f"  forward vs oracle: output rel err {yerr:.3e}, loss {loss:.6f} vs {lb:.6f}")
+    assert yerr < 2e-2, yerr
     assert loss == pytest.approx(lb, rel=2e-3)
-    rel = np.linalg.norm(grad - gb) / np.linalg.norm(gb)
-    assert rel < 2e-2, rel
-    # against the pure FP64 network: the BF16 operands cost a few percent
+    # 3. against the pure FP64 network: the BF16 operands cost a few percent
+    l64, g64 = orc.gnn_loss(layers, params, g.layout, xn, xe, off, send, recv, target, case.type, bf16=False)
     assert loss == pytest.approx(l64, rel=0.05)
     cos = float(grad @ g64 / np.linalg.norm(grad) / np.linalg.norm(g64))
+    print(f"  cosine to the FP64 gradient {cos:.5f}")
     assert cos > 0.99, cos
     g.close()
 

@@ -38,14 +38,14 @@ def _gpu_ns(sph, prm, st):
 
 
 def _check_ns(prm, got, want, fluid, wall):
-    assert rel_err(got["drho"][fluid], want["drho"][fluid], want["mag_drho"][fluid]) < TOL
+    assert rel_err(got["drho"][fluid], want["drho"][fluid], want["mag_drho"][fluid], want["sens_drho"][fluid]) < TOL
     assert rel_err(got["acc"][fluid], want["acc"][fluid], want["mag_acc"][fluid]) < TOL
     assert rel_err(got["dofv"], want["dofv"], want["mag_dofv"]) < TOL
     pscale = prm.restDensity * abs(prm.gravity) * prm.h
     assert rel_err(got["p"][wall], want["p"][wall], pscale) < TOL
     assert rel_err(got["rho"][wall], want["rho"][wall], prm.restDensity) < TOL
     # walls: the reference computes kernel B for them too
-    assert rel_err(got["drho"][wall], want["drho"][wall], want["mag_drho"][wall]) < TOL
+    assert rel_err(got["drho"][wall], want["drho"][wall], want["mag_drho"][wall], want["sens_drho"][wall]) < TOL
     assert rel_err(got["acc"][wall], want["acc"][wall], want["mag_acc"][wall]) < TOL
 
 

@@ -12,12 +12,13 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def _worst(case, name, got, want, mag):
+def _worst(case, name, got, want, mag, slack=0.0):
     g = np.asarray(got, np.float64).reshape(len(case.type), -1)
     w = np.asarray(want, np.float64).reshape(len(case.type), -1)
     m = np.asarray(mag, np.float64).reshape(len(case.type), -1)
+    sl = np.broadcast_to(np.asarray(slack, np.float64).reshape(-1, 1) if np.ndim(slack) else slack, g.shape)
     fl = case.type == 1
-    r = np.where(fl[:, None], np.abs(g - w) / np.maximum(np.maximum(np.abs(w), m), 1e-300), 0.0)
+    r = np.where(fl[:, None], np.maximum(np.abs(g - w) - sl, 0.0) / np.maximum(np.maximum(np.abs(w), m), 1e-300), 0.0)
     i, c = np.unravel_index(np.argmax(r), r.shape)
     print(f"worst {name}: particle {i} comp {c} pos {case.pos[i]} got {g[i, c]:.9g} want {w[i, c]:.9g} "
           f"mag {m[i, c]:.6g} rel {r[i, c]:.3e}")
@@ -29,9 +30,9 @@ def _check_rates(case, got, want):
     np.testing.assert_array_equal(got["perm"], want["perm"])
     fl = case.type == 1
     wl = ~fl
-    _worst(case, "drho", got["drho"], want["drho"], want["mag_drho"])
+    _worst(case, "drho", got["drho"], want["drho"], want["mag_drho"], want["sens_drho"])
     _worst(case, "acc", got["acc"], want["acc"], want["mag_acc"])
-    assert rel_err(got["drho"][fl], want["drho"][fl], want["mag_drho"][fl]) < TOL
+    assert rel_err(got["drho"][fl], want["drho"][fl], want["mag_drho"][fl], want["sens_drho"][fl]) < TOL
     assert rel_err(got["acc"][fl], want["acc"][fl], want["mag_acc"][fl]) < TOL
     assert rel_err(got["dofv"], want["dofv"], want["mag_dofv"]) < TOL
     pscale = prm.restDensity * abs(prm.gravity) * prm.h

@@ -96,7 +96,14 @@ class DamBreakCfg(C.Structure):
 
 
 class SphTuning(C.Structure):
-    _fields_ = [("nbr_block", C.c_int), ("nbr_unroll", C.c_int), ("xcd_remap", C.c_int), ("flags", C.c_int)]
+    _fields_ = [
+        ("nbr_block", C.c_int),
+        ("nbr_unroll", C.c_int),
+        ("xcd_remap", C.c_int),
+        ("flags", C.c_int),
+        ("nbr_lds", C.c_int),
+        ("nbr_compact", C.c_int),
+    ]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -162,6 +169,7 @@ _PROTOS = {
     "sph_gnn_load_state": [P, C.POINTER(SphParams), I, P, P, P, P, P],
     "sph_gnn_graph": [P, IP, P, P, P],
     "sph_gnn_features": [P, P, P],
+    "sph_gnn_get_tensor": [P, I, I, P, C.c_longlong],
     "sph_gnn_forward": [P, P, DP],
     "sph_gnn_backward": [P],
     "sph_gnn_zero_grad": [P],

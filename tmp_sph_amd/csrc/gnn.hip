@@ -978,6 +978,53 @@ extern "C" int sph_gnn_features(sph_gnn* g, float* node_feat, float* edge_feat) 
   return SPH_OK;
 }
 
+extern "C" int sph_gnn_get_tensor(sph_gnn* g, int kind, int index, void* host, long long count) {
+  if (!g || !host) return set_error(SPH_E_INVAL, "null argument");
+  if (!g->cfg.train) return set_error(SPH_E_STATE, "sph_gnn_get_tensor: activations are kept only with train=1");
+  if (g->n <= 0) return set_error(SPH_E_STATE, "sph_gnn_get_tensor: no state loaded");
+  const int L = g->L, ncalls = 3 + 2 * L;
+  const void* src = nullptr;
+  long long want = 0;
+  size_t esz = sizeof(float);
+  switch (kind) {
+    case 0:
+    case 1: {
+      if (index < 0 || index >= ncalls) return set_error(SPH_E_INVAL, "sph_gnn_get_tensor: bad MLP call index");
+      // edge-set calls: the edge encoder (1) and every edge update (2 + 2l)
+      const bool edges = index == 1 || (index >= 2 && index < 2 + 2 * L && index % 2 == 0);
+      want = (long long)(edges ? g->ne : g->n) * W;
+      src = kind == 0 ? h1_of(g, index) : h2_of(g, index);
+      esz = sizeof(bf16);
+      break;
+    }
+    case 2:
+      if (index < 0 || index > L) return set_error(SPH_E_INVAL, "sph_gnn_get_tensor: bad layer");
+      src = V_of(g, index);
+      want = (long long)g->n * W;
+      break;
+    case 3:
+      if (index < 0 || index > L) return set_error(SPH_E_INVAL, "sph_gnn_get_tensor: bad layer");
+      src = E_of(g, index);
+      want = (long long)g->ne * W;
+      break;
+    case 4:
+      if (index < 0 || index >= L) return set_error(SPH_E_INVAL, "sph_gnn_get_tensor: bad layer");
+      src = A_of(g, index);
+      want = (long long)g->n * W;
+      break;
+    case 5:
+      src = g->Y;
+      want = (long long)g->n * W;
+      break;
+    default:
+      return set_error(SPH_E_INVAL, "sph_gnn_get_tensor: unknown kind");
+  }
+  if (count != want) return set_error(SPH_E_INVAL, "sph_gnn_get_tensor: count does not match the tensor");
+  SPH_HIP(hipStreamSynchronize(g->stream), "get tensor sync");
+  SPH_HIP(hipMemcpy(host, src, (size_t)want * esz, hipMemcpyDeviceToHost), "get tensor");
+  return SPH_OK;
+}
+
 extern "C" int sph_gnn_forward(sph_gnn* g, float* out, double* loss) {
   if (!g || g->n <= 0) return set_error(SPH_E_STATE, "sph_gnn_forward: no state loaded");
   hipStream_t s = g->stream;

@@ -22,47 +22,16 @@
 //   nsA  opt_src_v1/equation/ns.cu:30-134   wall p extrapolation, dofv
 //   nsB  opt_src_v1/equation/ns.cu:137-230  drho/dt, dv/dt
 //   pc   opt_src_v0/sph/sph_run.cu:388-484  predictor / corrector + EOS
-#include "sph_device.h"
-#include "sph_kernels.h"
+#include "sph_fast_common.h"
 
 namespace sph {
+
+using namespace fast;
 
 namespace {
 
 inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-
-__device__ __forceinline__ int inst_of(int ty) { return ty >> 8; }
-__device__ __forceinline__ int type_of(int ty) { return ty & 0xff; }
-
-// Calls body(P, cell_offset) once for this lane with its instance's
-// parameters.  Each pass serves the lanes that share the first active lane's
-// instance, so P is wave-uniform inside body.
-template <class F>
-__device__ __forceinline__ void with_instance(const InstTable& T, int inst, F&& body) {
-  for (;;) {
-    const int u = __builtin_amdgcn_readfirstlane(inst);
-    if (inst == u) {
-      body(T.P[u], T.cell_off[u]);
-      break;
-    }
-  }
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8), each with its own L2.  Neighbouring target tiles share most of
-// their candidates, so the block id is remapped to give every XCD one
-// contiguous run of tiles instead of every eighth tile.
-__device__ __forceinline__ int tile_id(bool remap) {
-  const int b = blockIdx.x;
-  if (!remap) return b;
-  const int nb = gridDim.x;
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, k = b >> 3;
-  return x * q + min(x, r) + k;
-}
 
 __global__ void hash_fast_kernel(InstTable T, const float4* __restrict__ PR, const int* __restrict__ TY,
                                  int* __restrict__ keys, int* __restrict__ idx, int n) {
@@ -102,204 +71,7 @@ __global__ void cells_gather_kernel(FastSet in, FastSet out, const int* __restri
   out.TY[i] = in.TY[src];
 }
 
-struct RowRange {
-  int s, e;
-};
 
-// Sorted index ranges of the stencil rows of the cell with global key `key`,
-// in the reference order (z outer, y inner; the x-adjacent cells of a row are
-// contiguous in sorted order, so each row is one range).
-template <int DIM>
-__device__ __forceinline__ void row_ranges(const sph_params& P, int coff, int key,
-                                           const int* __restrict__ cellStart,
-                                           const int* __restrict__ cellEnd,
-                                           RowRange (&rows)[DIM == 3 ? 9 : 3]) {
-  const int X = P.gridxdim, Y = P.gridydim, Z = P.gridzdim;
-  const int XY = X * Y;
-  const int hash = key - coff;
-  const int cz = hash / XY;
-  const int rem = hash - cz * XY;
-  const int cy = rem / X;
-  const int cx = rem - cy * X;
-  const int x0 = cx > 0 ? cx - 1 : 0;
-  const int x1 = cx < X - 1 ? cx + 1 : X - 1;
-  int nr = 0;
-#pragma unroll
-  for (int dz = -1; dz <= 1; ++dz) {
-#pragma unroll
-    for (int dy = (DIM == 3 ? -1 : 0); dy <= (DIM == 3 ? 1 : 0); ++dy) {
-      const int z = cz + dz, y = cy + dy;
-      int s = 0, e = 0;
-      if (z >= 0 && z < Z && y >= 0 && y < Y) {
-        const int rowBase = coff + z * XY + y * X;
-        s = -1;
-        for (int c = rowBase + x0; c <= rowBase + x1; ++c) {
-          const int st = cellStart[c];
-          if (st != -1) {
-            if (s < 0) s = st;
-            e = cellEnd[c];
-          }
-        }
-        if (s < 0) s = e = 0;
-      }
-      rows[nr].s = s;
-      rows[nr].e = e;
-      ++nr;
-    }
-  }
-}
-
-// Kernel A: wall pressure by Shepard extrapolation with the hydrostatic
-// correction, and the velocity divergence of fluid particles.
-template <int DIM, int UNROLL>
-__global__ __launch_bounds__(256) void nsA_fast_kernel(InstTable T, float4* PR, float4* VP,
-                                                       const int* __restrict__ TY, float* __restrict__ DV,
-                                                       const int* __restrict__ keys,
-                                                       const int* __restrict__ cellStart,
-                                                       const int* __restrict__ cellEnd, int n, int remap) {
-  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 pi = PR[i];
-  const float4 vi = VP[i];
-  const bool fi = pi.w > 0.0f;
-  const int key = keys[i];
-  with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
-    RowRange rows[DIM == 3 ? 9 : 3];
-    row_ranges<DIM>(P, coff, key, cellStart, cellEnd, rows);
-    const float inv_h = 1.0f / P.h;
-    const float cfr = -5.0f * P.adh / (P.h * P.h);
-    const float mg = 0.0f - P.gravity;
-    const float m = P.particleMass;
-    const float cut = P.rr_cut;
-    float sp = 0.0f, sw = 0.0f, dv = 0.0f;
-#pragma unroll 1
-    for (int r = 0; r < (DIM == 3 ? 9 : 3); ++r) {
-      const int s = rows[r].s, e = rows[r].e;
-      for (int j0 = s; j0 < e; j0 += UNROLL) {
-        float4 pjs[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) pjs[u] = PR[min(j0 + u, e - 1)];  // loads issued together
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-          const int j = j0 + u;
-          const float4 pj = pjs[u];
-          const float dx = pi.x - pj.x;
-          const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
-          const float dz = pi.z - pj.z;
-          const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
-          if (j < e && rr <= cut && j != i && pj.w > 0.0f) {
-            const float q = fast_sqrt(rr) * inv_h;
-            const float t = 1.0f - 0.5f * q;
-            const float t3 = t * t * t;
-            const float4 vj = VP[j];
-            if (!fi) {
-              const float w = t3 * t * ((2.0f * q + 1.0f) * P.adh);
-              sp += (vj.w - pj.w * (mg * dz)) * w;
-              sw += w;
-            } else {
-              const float dvdx = DIM == 3 ? (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz
-                                          : (vi.x - vj.x) * dx + (vi.z - vj.z) * dz;
-              dv -= cfr * t3 * dvdx * m * fast_rcp(pj.w);
-            }
-          }
-        }
-      }
-    }
-    DV[i] = dv;
-    if (!fi) {
-      float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
-      if (pw < 0.0f) pw = 0.0f;
-      reinterpret_cast<float*>(VP)[4 * i + 3] = pw;
-      reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
-    }
-  });
-}
-
-// Kernel B: continuity with delta-SPH diffusion and momentum with pressure,
-// artificial viscosity, acoustic damper and gravity.  Non-fluid targets are
-// skipped: the integrator never reads their rates.
-template <int DIM, int UNROLL>
-__global__ __launch_bounds__(256) void nsB_fast_kernel(InstTable T, const float4* __restrict__ PR,
-                                                       const float4* __restrict__ VP,
-                                                       const int* __restrict__ TY,
-                                                       const float* __restrict__ DV,
-                                                       const int* __restrict__ keys,
-                                                       const int* __restrict__ cellStart,
-                                                       const int* __restrict__ cellEnd,
-                                                       float4* __restrict__ RATE, int n, int remap) {
-  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 pi = PR[i];
-  if (!(pi.w > 0.0f)) {
-    RATE[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return;
-  }
-  const float4 vi = VP[i];
-  const float rho_i = pi.w;
-  const float dvi = DV[i];
-  const int key = keys[i];
-  with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
-    RowRange rows[DIM == 3 ? 9 : 3];
-    row_ranges<DIM>(P, coff, key, cellStart, cellEnd, rows);
-    const float inv_h = 1.0f / P.h;
-    const float cfr = -5.0f * P.adh / (P.h * P.h);
-    const float m = P.particleMass;
-    const float dhc = P.delta * P.h * P.cs;
-    const float eta2 = P.eta * P.eta;
-    const float cut = P.rr_cut;
-    const float damper = P.damper;
-    const float afa = P.afa;
-    const float dofv_i = damper * dvi;
-    const float inv_rho_i = 1.0f / rho_i;
-    const float avc = P.h * P.cs * P.restDensity * m * inv_rho_i;
-    const float pm_i = m * inv_rho_i;
-    float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
-#pragma unroll 1
-    for (int r = 0; r < (DIM == 3 ? 9 : 3); ++r) {
-      const int s = rows[r].s, e = rows[r].e;
-      for (int j0 = s; j0 < e; j0 += UNROLL) {
-        float4 pjs[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) pjs[u] = PR[min(j0 + u, e - 1)];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-          const int j = j0 + u;
-          const float4 pj = pjs[u];
-          const float dx = pi.x - pj.x;
-          const float dy = DIM == 3 ? pi.y - pj.y : 0.0f;
-          const float dz = pi.z - pj.z;
-          const float rr = DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
-          if (j < e && rr <= cut && j != i) {
-            const float4 vj = VP[j];
-            const bool fj = pj.w > 0.0f;
-            const float rho_j = fabsf(pj.w);
-            const float q = fast_sqrt(rr) * inv_h;
-            const float t = 1.0f - 0.5f * q;
-            const float fr = cfr * (t * t * t);
-            const float inv_rho_j = fast_rcp(rho_j);
-            const float inv_re = fast_rcp(rr + eta2);
-            const float dvdx = DIM == 3 ? (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz
-                                        : (vi.x - vj.x) * dx + (vi.z - vj.z) * dz;
-            const float frj = fr * inv_rho_j;
-            drho += frj * m * (rho_i * dvdx + (rho_i - rho_j) * rr * dhc * inv_re);
-            float vt = -(vi.w + vj.w) * pm_i;
-            if (fj) vt += avc * (dofv_i + damper * DV[j] + afa * dvdx * inv_re);
-            vt *= frj;
-            ax += vt * dx;
-            if (DIM == 3) ay += vt * dy;
-            az += vt * dz;
-          }
-        }
-      }
-    }
-    RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
-  });
-}
-
-__device__ __forceinline__ bool outside64(const sph_params& P, double x, double y, double z) {
-  return x > P.domain_max[0] || x < P.domain_min[0] || y > P.domain_max[1] ||
-         y < P.domain_min[1] || z > P.domain_max[2] || z < P.domain_min[2];
-}
 
 // Predictor (PRED) / corrector of the reference PC scheme with FP64
 // positions.  The predictor stores the step-start state at its own sorted
@@ -534,48 +306,6 @@ hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const
   if (e != hipSuccess || n <= 0) return e;
   return launch_dim(T.dim, cells_gather_kernel<2>, cells_gather_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), s,
                     in, out, keys, idx, cellStart, cellEnd, n);
-}
-
-template <int U>
-static hipError_t nsA_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
-                        hipStream_t s) {
-  return launch_dim(T.dim, nsA_fast_kernel<2, U>, nsA_fast_kernel<3, U>, dim3(blocks_for(n, block)), dim3(block),
-                    s, T, st.PR, st.VP, (const int*)st.TY, w.DV, (const int*)w.keys, (const int*)w.cellStart,
-                    (const int*)w.cellEnd, n, remap);
-}
-
-template <int U>
-static hipError_t nsB_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
-                        hipStream_t s) {
-  return launch_dim(T.dim, nsB_fast_kernel<2, U>, nsB_fast_kernel<3, U>, dim3(blocks_for(n, block)), dim3(block),
-                    s, T, (const float4*)st.PR, (const float4*)st.VP, (const int*)st.TY, (const float*)w.DV,
-                    (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap);
-}
-
-static int norm_block(int b) { return (b >= 64 && b <= 256 && b % 64 == 0) ? b : 256; }
-
-hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
-                           hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0;
-  switch (tu.nbr_unroll) {
-    case 1: return nsA_u<1>(T, st, w, n, b, rm, s);
-    case 2: return nsA_u<2>(T, st, w, n, b, rm, s);
-    case 8: return nsA_u<8>(T, st, w, n, b, rm, s);
-    default: return nsA_u<4>(T, st, w, n, b, rm, s);
-  }
-}
-
-hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
-                           hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0;
-  switch (tu.nbr_unroll) {
-    case 1: return nsB_u<1>(T, st, w, n, b, rm, s);
-    case 2: return nsB_u<2>(T, st, w, n, b, rm, s);
-    case 8: return nsB_u<8>(T, st, w, n, b, rm, s);
-    default: return nsB_u<4>(T, st, w, n, b, rm, s);
-  }
 }
 
 hipError_t launch_pc_fast(const InstTable& T, bool predictor, const FastSet& st, const FastWork& w,

@@ -1,0 +1,500 @@
+// Neighbour-loop kernels of the fast path: kernel A (wall pressure
+// extrapolation, velocity divergence; opt_src_v1/equation/ns.cu:30-134) and
+// kernel B (continuity with delta-SPH diffusion, momentum;
+// opt_src_v1/equation/ns.cu:137-230).
+//
+// One thread per target in sorted order; a target visits its stencil rows
+// (z outer, y inner; the x-adjacent cells of a row are one contiguous sorted
+// range) in the reference's order and its candidates in ascending index.
+// Launch variants, chosen by the autotuner (every variant visits the same
+// pairs in the same order, so all give bit-identical results):
+//   staged   one wave per block: when the wave's 64 targets lie in one cell
+//            row of one instance, the union of their stencil rows is copied
+//            once into LDS and candidates are read from there; other waves
+//            read HBM/L2 ("global").
+//   compact  only ~15% of the candidates of a 3-D cell stencil are within the
+//            cutoff, and a wave runs the pair body whenever any lane accepts,
+//            so the plain loop pays the full pair cost on almost every
+//            candidate.  The compact loop scans candidates cheaply, appends
+//            each lane's accepted ones to its own LDS list and, whenever a
+//            list is nearly full (and at the end), runs the pair body densely
+//            over all lists.
+#include "sph_fast_common.h"
+
+namespace sph {
+
+using namespace fast;
+
+namespace {
+
+inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+constexpr int STAGE_CAP = 1024;   // float4 entries per staged wave (16 KB of LDS)
+constexpr int COMPACT_CAP = 16;   // accepted candidates per lane per dense phase
+
+template <int DIM>
+constexpr int nrows() { return DIM == 3 ? 9 : 3; }
+
+// ------------------------------------------------------------ row providers
+// rows(r) returns the sorted range of stencil row r.
+template <int NR>
+struct RegRows {  // in registers, selected without dynamic indexing
+  RowRange v[NR];
+  __device__ __forceinline__ RowRange operator()(int r) const { return pick_row(v, r); }
+};
+struct LdsRows {  // in the thread's own LDS slots
+  const RowRange* slot;
+  int stride;
+  __device__ __forceinline__ RowRange operator()(int r) const { return slot[r * stride]; }
+};
+
+// ------------------------------------------------------- candidate sources
+// src.row(r) is called at the start of row r; src.at(j) returns PR[j].
+struct GlobalSrc {
+  const float4* PR;
+  __device__ __forceinline__ void row(int) {}
+  __device__ __forceinline__ float4 at(int j) const { return PR[j]; }
+};
+
+// Candidates of row r live at tile[j + shift[r]]; the shifts are wave-uniform.
+template <int NR>
+struct StagedSrc {
+  const float4* tile;
+  int shift[NR];
+  int cur;
+  __device__ __forceinline__ void row(int r) {
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) v |= shift[k] & -(int)(r == k);
+    cur = v;
+  }
+  __device__ __forceinline__ float4 at(int j) const { return tile[j + cur]; }
+};
+
+// ------------------------------------------------------------ visit loops
+// visit(accept, pair): accept(j, pj) is the candidate test, pair(j, pj) the
+// body for an accepted candidate; both see candidates in ascending order.
+template <int NR, int U, class Rows, class Src>
+struct PlainVisit {
+  Rows rows;
+  Src src;
+  template <class Accept, class Pair>
+  __device__ __forceinline__ void operator()(Accept&& accept, Pair&& pair) {
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) {
+      const RowRange g = rows(r);
+      src.row(r);
+      for (int j0 = g.s; j0 < g.e; j0 += U) {
+        float4 pjs[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) pjs[u] = src.at(min(j0 + u, g.e - 1));  // issued together
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (j0 + u < g.e && accept(j0 + u, pjs[u])) pair(j0 + u, pjs[u]);
+      }
+    }
+  }
+};
+
+template <int NR, int U, class Rows, class Src>
+struct CompactVisit {
+  Rows rows;
+  Src src;
+  int* list;  // this wave's COMPACT_CAP x 64 slots, [slot][lane]
+  const float4* PR;
+  template <class Accept, class Pair>
+  __device__ __forceinline__ void operator()(Accept&& accept, Pair&& pair) {
+    const int lane = threadIdx.x & 63;
+    int cnt = 0;
+    // Dense phase over every lane's list, in order.  Called by the whole
+    // wave (the conditions are wave-uniform); lanes keep their own order, so
+    // flushing early changes nothing in the sums.
+    auto flush = [&]() {
+#pragma unroll 1
+      for (int k = 0; k < cnt; ++k) {
+        const int jj = list[k * 64 + lane];  // [slot][lane]: conflict-free for any per-lane count
+        pair(jj, PR[jj]);
+      }
+      cnt = 0;
+    };
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) {  // same row for all lanes
+      const RowRange g = rows(r);
+      src.row(r);
+      int j = g.s;
+#pragma unroll 1
+      while (__ballot(j < g.e) != 0ull) {
+        if (__ballot(cnt > COMPACT_CAP - U) != 0ull) flush();
+        if (j < g.e) {
+          float4 pjs[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) pjs[u] = src.at(min(j + u, g.e - 1));
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (j + u < g.e && accept(j + u, pjs[u])) {
+              list[cnt * 64 + lane] = j + u;
+              ++cnt;
+            }
+        }
+        j += U;
+      }
+    }
+    flush();
+  }
+};
+
+// ------------------------------------------------------------ per target
+// The pair arithmetic below is written with explicit fmaf and compiled with
+// contraction off, so that every loop variant (whose code the compiler
+// schedules differently) rounds identically.  The squared distance is the
+// reference's unfused sum, so the pair set is exactly the reference's.
+template <int DIM>
+__device__ __forceinline__ float geom(const float4& pi, const float4& pj, float& dx, float& dy, float& dz) {
+#pragma clang fp contract(off)
+  dx = pi.x - pj.x;
+  dy = DIM == 3 ? pi.y - pj.y : 0.0f;
+  dz = pi.z - pj.z;
+  return DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
+}
+
+// Kernel A for target i (fluid neighbours only).
+template <int DIM, class Visit>
+__device__ __forceinline__ void nsA_target(const sph_params& P, int i, const float4& pi, const float4& vi,
+                                           Visit visit, float4* PR, float4* VP, float* __restrict__ DV) {
+#pragma clang fp contract(off)
+  const bool fi = pi.w > 0.0f;
+  const float inv_h = 1.0f / P.h;
+  const float cfr = -5.0f * P.adh / (P.h * P.h);
+  const float mg = 0.0f - P.gravity;
+  const float m = P.particleMass;
+  const float cut = P.rr_cut;
+  float sp = 0.0f, sw = 0.0f, dv = 0.0f;
+  visit(
+      [&](int j, const float4& pj) {
+        float dx, dy, dz;
+        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
+        return rr <= cut && j != i && pj.w > 0.0f;
+      },
+      [&](int j, const float4& pj) {
+        float dx, dy, dz;
+        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
+        const float q = fast_sqrt(rr) * inv_h;
+        const float t = fmaf(-0.5f, q, 1.0f);
+        const float t3 = t * t * t;
+        const float4 vj = VP[j];
+        if (!fi) {
+          const float w = (t3 * t) * (fmaf(2.0f, q, 1.0f) * P.adh);
+          sp = fmaf(fmaf(-pj.w, mg * dz, vj.w), w, sp);
+          sw += w;
+        } else {
+          const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
+                                      : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
+          dv = fmaf(-(cfr * t3 * m), dvdx * fast_rcp(pj.w), dv);
+        }
+      });
+  DV[i] = dv;
+  if (!fi) {
+    float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
+    if (pw < 0.0f) pw = 0.0f;
+    reinterpret_cast<float*>(VP)[4 * i + 3] = pw;
+    reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
+  }
+}
+
+// Kernel B for fluid target i (all neighbours).
+template <int DIM, class Visit>
+__device__ __forceinline__ void nsB_target(const sph_params& P, int i, const float4& pi, const float4& vi, float dvi,
+                                           Visit visit, const float4* __restrict__ VP,
+                                           const float* __restrict__ DV, float4* __restrict__ RATE) {
+#pragma clang fp contract(off)
+  const float rho_i = pi.w;
+  const float inv_h = 1.0f / P.h;
+  const float cfr = -5.0f * P.adh / (P.h * P.h);
+  const float m = P.particleMass;
+  const float dhc = P.delta * P.h * P.cs;
+  const float eta2 = P.eta * P.eta;
+  const float cut = P.rr_cut;
+  const float damper = P.damper;
+  const float afa = P.afa;
+  const float dofv_i = damper * dvi;
+  const float inv_rho_i = 1.0f / rho_i;
+  const float avc = P.h * P.cs * P.restDensity * m * inv_rho_i;
+  const float pm_i = m * inv_rho_i;
+  float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
+  visit(
+      [&](int j, const float4& pj) {
+        float dx, dy, dz;
+        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
+        return rr <= cut && j != i;
+      },
+      [&](int j, const float4& pj) {
+        float dx, dy, dz;
+        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
+        const float4 vj = VP[j];
+        const bool fj = pj.w > 0.0f;
+        const float rho_j = fabsf(pj.w);
+        const float q = fast_sqrt(rr) * inv_h;
+        const float t = fmaf(-0.5f, q, 1.0f);
+        const float fr = cfr * (t * t * t);
+        const float inv_rho_j = fast_rcp(rho_j);
+        const float inv_re = fast_rcp(rr + eta2);
+        const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
+                                    : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
+        const float frj = fr * inv_rho_j;
+        drho = fmaf(frj * m, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
+        float vt = -(vi.w + vj.w) * pm_i;
+        if (fj) vt = fmaf(avc, fmaf(damper, DV[j], dofv_i) + afa * dvdx * inv_re, vt);
+        vt *= frj;
+        ax = fmaf(vt, dx, ax);
+        if (DIM == 3) ay = fmaf(vt, dy, ay);
+        az = fmaf(vt, dz, az);
+      });
+  RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
+}
+
+// ------------------------------------------------------------ staging
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Decides whether the wave can stage its candidates (all 64 targets in one
+// cell row of one instance, union of the row ranges within STAGE_CAP) and, if
+// so, copies the union into `tile` and fills src.  Every lane of the
+// (single-wave) block must call it.
+template <int DIM>
+__device__ __forceinline__ bool stage_rows(const sph_params& P, int coff, int key, bool same_inst,
+                                           const RowRange (&rows)[DIM == 3 ? 9 : 3], const float4* PR,
+                                           float4* tile, StagedSrc<nrows<DIM>()>& src) {
+  const int row_id = (key - coff) / P.gridxdim;  // z * Y + y of the target's cell
+  const int u_row = __builtin_amdgcn_readfirstlane(row_id);
+  if (!same_inst || __ballot(row_id != u_row) != 0ull) return false;
+  int lo[nrows<DIM>()], len[nrows<DIM>()];
+  int total = 0;
+#pragma unroll
+  for (int r = 0; r < nrows<DIM>(); ++r) {
+    const bool ne = rows[r].e > rows[r].s;
+    const int l = wave_min(ne ? rows[r].s : 0x7fffffff);
+    const int h = wave_max(ne ? rows[r].e : 0);
+    lo[r] = l;
+    len[r] = l == 0x7fffffff ? 0 : h - l;
+    src.shift[r] = total - l;
+    total += len[r];
+  }
+  if (total > STAGE_CAP) return false;
+  const int lane = threadIdx.x;
+  int base = 0;
+#pragma unroll
+  for (int r = 0; r < nrows<DIM>(); ++r) {
+    for (int t = lane; t < len[r]; t += 64) tile[base + t] = PR[lo[r] + t];
+    base += len[r];
+  }
+  src.tile = tile;
+  src.cur = 0;
+  __syncthreads();
+  return true;
+}
+
+// ------------------------------------------------------------ kernels
+// STAGED: 64-thread blocks with LDS staging; else up to 256 threads.
+// COMPACT: scan/dense loop; else the plain loop.
+template <int DIM, int U, bool STAGED, bool COMPACT>
+struct Cfg {
+  static constexpr int NR = nrows<DIM>();
+  static constexpr int LIST = COMPACT ? COMPACT_CAP * (STAGED ? 64 : 256) : 1;
+  static constexpr int TILE = STAGED ? STAGE_CAP : 1;
+  static constexpr int SLOTS = (!STAGED && !COMPACT) ? NR * 256 : 1;
+};
+
+// Runs body(P, visit) for this lane with a visit loop over its stencil.  The
+// global path serves lanes instance by instance (wave-uniform parameters).
+template <int DIM, int U, bool STAGED, bool COMPACT, class Body>
+__device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key, bool active, const float4* PR,
+                                           const int* __restrict__ cs, const int* __restrict__ ce, float4* tile,
+                                           int* lists, RowRange* slots, Body&& body) {
+  constexpr int NR = nrows<DIM>();
+  int* list = lists + (threadIdx.x >> 6) * COMPACT_CAP * 64;
+  auto global_path = [&](const sph_params& P, int coff) {
+    if constexpr (!STAGED && !COMPACT) {
+      // row ranges parked in the thread's own LDS slots (no barrier needed):
+      // keeps 18 VGPRs free in 3-D, which buys occupancy for the plain loop
+      RowRange rows[NR];
+      row_ranges<DIM>(P, coff, key, cs, ce, rows);
+      const int t = threadIdx.x;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) slots[r * 256 + t] = rows[r];
+      body(P, PlainVisit<NR, U, LdsRows, GlobalSrc>{LdsRows{slots + t, 256}, GlobalSrc{PR}});
+    } else {
+      RegRows<NR> rows;
+      row_ranges<DIM>(P, coff, key, cs, ce, rows.v);
+      if constexpr (COMPACT)
+        body(P, CompactVisit<NR, U, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}, list, PR});
+      else
+        body(P, PlainVisit<NR, U, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}});
+    }
+  };
+  if constexpr (STAGED) {
+    // the whole wave takes part in the staging decision and copy
+    const int u = __builtin_amdgcn_readfirstlane(inst);
+    const bool same_inst = __ballot(inst != u) == 0ull;
+    const sph_params& P = T.P[u];
+    const int coff = T.cell_off[u];
+    RegRows<NR> rows;
+    row_ranges<DIM>(P, coff, key, cs, ce, rows.v);
+    StagedSrc<NR> src;
+    const bool staged = stage_rows<DIM>(P, coff, key, same_inst, rows.v, PR, tile, src);
+    if (!active) return;
+    if (staged) {
+      if constexpr (COMPACT)
+        body(P, CompactVisit<NR, U, RegRows<NR>, StagedSrc<NR>>{rows, src, list, PR});
+      else
+        body(P, PlainVisit<NR, U, RegRows<NR>, StagedSrc<NR>>{rows, src});
+    } else {
+      with_instance(T, inst, global_path);
+    }
+  } else {
+    if (!active) return;
+    with_instance(T, inst, global_path);
+  }
+}
+
+template <int DIM, int U, bool STAGED, bool COMPACT>
+__global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, float4* PR, float4* VP,
+                                                                const int* __restrict__ TY, float* __restrict__ DV,
+                                                                const int* __restrict__ keys,
+                                                                const int* __restrict__ cs,
+                                                                const int* __restrict__ ce, int n, int remap) {
+  using C = Cfg<DIM, U, STAGED, COMPACT>;
+  __shared__ float4 tile[C::TILE];
+  __shared__ int lists[C::LIST];
+  __shared__ RowRange slots[C::SLOTS];
+  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  if (!STAGED && !valid) return;
+  const int ic = valid ? i : n - 1;  // staged: lane 0 is always valid (the grid covers n)
+  const float4 pi = PR[ic];
+  const float4 vi = VP[ic];
+  const int key = keys[ic];
+  with_visit<DIM, U, STAGED, COMPACT>(T, inst_of(TY[ic]), key, valid, PR, cs, ce, tile, lists, slots,
+                                      [&](const sph_params& P, auto visit) {
+                                        nsA_target<DIM>(P, i, pi, vi, visit, PR, VP, DV);
+                                      });
+}
+
+template <int DIM, int U, bool STAGED, bool COMPACT>
+__global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, const float4* __restrict__ PR,
+                                                                const float4* __restrict__ VP,
+                                                                const int* __restrict__ TY,
+                                                                const float* __restrict__ DV,
+                                                                const int* __restrict__ keys,
+                                                                const int* __restrict__ cs,
+                                                                const int* __restrict__ ce,
+                                                                float4* __restrict__ RATE, int n, int remap) {
+  using C = Cfg<DIM, U, STAGED, COMPACT>;
+  __shared__ float4 tile[C::TILE];
+  __shared__ int lists[C::LIST];
+  __shared__ RowRange slots[C::SLOTS];
+  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  if (!STAGED && !valid) return;
+  const int ic = valid ? i : n - 1;
+  const float4 pi = PR[ic];
+  const bool fluid = pi.w > 0.0f;
+  if (valid && !fluid) RATE[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (!STAGED && !fluid) return;
+  const float4 vi = VP[ic];
+  const float dvi = DV[ic];
+  const int key = keys[ic];
+  with_visit<DIM, U, STAGED, COMPACT>(T, inst_of(TY[ic]), key, valid && fluid, PR, cs, ce, tile, lists, slots,
+                                      [&](const sph_params& P, auto visit) {
+                                        nsB_target<DIM>(P, i, pi, vi, dvi, visit, VP, DV, RATE);
+                                      });
+}
+
+// ------------------------------------------------------------ launchers
+template <int U, bool STAGED, bool COMPACT>
+hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
+                 hipStream_t s) {
+  const int b = STAGED ? 64 : block;
+  const dim3 g(blocks_for(n, b)), bl(b);
+  if (T.dim == 3)
+    hipLaunchKernelGGL((nsA_kernel<3, U, STAGED, COMPACT>), g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
+                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, n, remap);
+  else
+    hipLaunchKernelGGL((nsA_kernel<2, U, STAGED, COMPACT>), g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
+                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, n, remap);
+  return hipGetLastError();
+}
+
+template <int U, bool STAGED, bool COMPACT>
+hipError_t nsB_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
+                 hipStream_t s) {
+  const int b = STAGED ? 64 : block;
+  const dim3 g(blocks_for(n, b)), bl(b);
+  if (T.dim == 3)
+    hipLaunchKernelGGL((nsB_kernel<3, U, STAGED, COMPACT>), g, bl, 0, s, T, (const float4*)st.PR,
+                       (const float4*)st.VP, (const int*)st.TY, (const float*)w.DV, (const int*)w.keys,
+                       (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap);
+  else
+    hipLaunchKernelGGL((nsB_kernel<2, U, STAGED, COMPACT>), g, bl, 0, s, T, (const float4*)st.PR,
+                       (const float4*)st.VP, (const int*)st.TY, (const float*)w.DV, (const int*)w.keys,
+                       (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap);
+  return hipGetLastError();
+}
+
+template <int U>
+hipError_t nsA_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap, int mode,
+                 hipStream_t s) {
+  switch (mode) {
+    case 1: return nsA_v<U, true, false>(T, st, w, n, block, remap, s);
+    case 2: return nsA_v<U, false, true>(T, st, w, n, block, remap, s);
+    case 3: return nsA_v<U, true, true>(T, st, w, n, block, remap, s);
+    default: return nsA_v<U, false, false>(T, st, w, n, block, remap, s);
+  }
+}
+
+template <int U>
+hipError_t nsB_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap, int mode,
+                 hipStream_t s) {
+  switch (mode) {
+    case 1: return nsB_v<U, true, false>(T, st, w, n, block, remap, s);
+    case 2: return nsB_v<U, false, true>(T, st, w, n, block, remap, s);
+    case 3: return nsB_v<U, true, true>(T, st, w, n, block, remap, s);
+    default: return nsB_v<U, false, false>(T, st, w, n, block, remap, s);
+  }
+}
+
+int norm_block(int b) { return (b >= 64 && b <= 256 && b % 64 == 0) ? b : 256; }
+int mode_of(const sph_tuning& tu) { return (tu.nbr_lds ? 1 : 0) | (tu.nbr_compact ? 2 : 0); }
+
+}  // namespace
+
+hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0, md = mode_of(tu);
+  switch (tu.nbr_unroll) {
+    case 1: return nsA_u<1>(T, st, w, n, b, rm, md, s);
+    case 2: return nsA_u<2>(T, st, w, n, b, rm, md, s);
+    case 8: return nsA_u<8>(T, st, w, n, b, rm, md, s);
+    default: return nsA_u<4>(T, st, w, n, b, rm, md, s);
+  }
+}
+
+hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0, md = mode_of(tu);
+  switch (tu.nbr_unroll) {
+    case 1: return nsB_u<1>(T, st, w, n, b, rm, md, s);
+    case 2: return nsB_u<2>(T, st, w, n, b, rm, md, s);
+    case 8: return nsB_u<8>(T, st, w, n, b, rm, md, s);
+    default: return nsB_u<4>(T, st, w, n, b, rm, md, s);
+  }
+}
+
+}  // namespace sph

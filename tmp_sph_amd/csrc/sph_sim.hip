@@ -206,6 +206,8 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   s->tun.nbr_unroll = 4;
   s->tun.xcd_remap = 1;
   s->tun.flags = 0;
+  s->tun.nbr_lds = 0;
+  s->tun.nbr_compact = 0;
   int rc = SPH_OK;
   hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (he != hipSuccess) {
@@ -327,9 +329,13 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
   if (t->nbr_unroll != 1 && t->nbr_unroll != 2 && t->nbr_unroll != 4 && t->nbr_unroll != 8)
     return set_error(SPH_E_INVAL, "nbr_unroll must be 1, 2, 4 or 8");
   const bool shape_changed = t->nbr_block != s->tun.nbr_block || t->nbr_unroll != s->tun.nbr_unroll ||
-                             (t->xcd_remap != 0) != (s->tun.xcd_remap != 0);
+                             (t->xcd_remap != 0) != (s->tun.xcd_remap != 0) ||
+                             (t->nbr_lds != 0) != (s->tun.nbr_lds != 0) ||
+                             (t->nbr_compact != 0) != (s->tun.nbr_compact != 0);
   s->tun = *t;
   s->tun.xcd_remap = t->xcd_remap ? 1 : 0;
+  s->tun.nbr_lds = t->nbr_lds ? 1 : 0;
+  s->tun.nbr_compact = t->nbr_compact ? 1 : 0;
   if (shape_changed) drop_graph(s);  // a captured graph holds the old launch shapes
   std::fill(s->tsum_ms.begin(), s->tsum_ms.end(), 0.0);
   std::fill(s->tcount.begin(), s->tcount.end(), 0);
@@ -527,7 +533,8 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
 }
 
 // Autotuner: times kernels A and B on the live state for every launch shape
-// (block 64/128/256 x unroll 1/2/4/8 x XCD remap on/off) with HIP events on
+// (plain or compact loop x LDS-staged or global with block 64/128/256 x
+// unroll 1/2/4/8 x XCD remap on/off) with HIP events on
 // the context's stream and keeps the fastest.  The neighbour phase is
 // re-evaluated from the same sorted set each time, so the state is unchanged
 // and every candidate sees identical work.
@@ -544,17 +551,22 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   const sph_tuning saved = s->tun;
   sph_tuning pick = saved;
   double pick_us = 1e300;
-  const int blocks[] = {64, 128, 256};
+  const int blocks[] = {0, 64, 128, 256};
   const int unrolls[] = {1, 2, 4, 8};
   hipError_t err = hipSuccess;
+  // the LDS-staged variant always runs one-wave blocks: it is tried once per
+  // unroll/remap pair (bl == 0 below)
+  for (int compact = 0; compact < 2 && err == hipSuccess; ++compact)
   for (int remap = 0; remap < 2 && err == hipSuccess; ++remap)
     for (int bl : blocks)
       for (int un : unrolls) {
         if (err != hipSuccess) break;
         sph_tuning t = saved;
-        t.nbr_block = bl;
+        t.nbr_block = bl == 0 ? 64 : bl;
         t.nbr_unroll = un;
         t.xcd_remap = remap;
+        t.nbr_lds = bl == 0 ? 1 : 0;
+        t.nbr_compact = compact;
         s->tun = t;
         err = rates_phase(s, 1, w, false);  // warm-up
         if (err == hipSuccess) err = hipEventRecord(a, s->stream);

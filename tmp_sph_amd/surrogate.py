@@ -120,6 +120,21 @@ class Gnn:
         call("sph_gnn_features", self._h, _hp(xn), _hp(xe))
         return xn, xe
 
+    def tensor(self, kind: int, index: int = 0) -> np.ndarray:
+        """An activation of the last forward pass (see sph_gnn_get_tensor):
+        kinds 0/1 return BF16 bit patterns (uint16), the others FP32."""
+        _, s, _ = self.graph()
+        L = self.cfg.layers
+        if kind in (0, 1):
+            edges = index == 1 or (2 <= index < 2 + 2 * L and index % 2 == 0)
+            a = np.empty(((s.size if edges else self.n), W), np.uint16)
+        elif kind == 3:
+            a = np.empty((s.size, W), np.float32)
+        else:
+            a = np.empty((self.n, W), np.float32)
+        call("sph_gnn_get_tensor", self._h, int(kind), int(index), _hp(a), C.c_longlong(a.size))
+        return a
+
     def forward(self, want_out=True):
         out = np.empty((self.n, 3), np.float32) if want_out else None
         loss = C.c_double()
