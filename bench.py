@@ -65,6 +65,21 @@ def cpu_baseline(case, seconds_target: float) -> dict:
                       f"from rest, {case.n} particles ({case.n_fluid} fluid), {sk:.1f} s"}
 
 
+def copy_probe(mib: int) -> int:
+    """A plain device copy of known size, for converting the TCC byte
+    counters (FETCH_SIZE, WRITE_SIZE) of a kernel into HBM bytes."""
+    from tmp_sph_amd import _lib, sph
+    n = mib << 20
+    a = sph.DeviceArray(n, "u1")
+    b = sph.DeviceArray(n, "u1")
+    a.fill_bytes(1)
+    for _ in range(5):
+        _lib.call("sph_memcpy_dtod", b.c, a.c, n)
+    _lib.call("sph_device_synchronize")
+    print(json.dumps({"copy_bytes_per_launch": n, "launches": 5}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,8 +90,17 @@ def main() -> int:
     ap.add_argument("--timing-steps", type=int, default=10, help="eager steps with per-kernel HIP events")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--nbr-block", type=int, default=0)
+    ap.add_argument("--nbr-block", type=int, default=0, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--nbr-unroll", type=int, default=0)
+    ap.add_argument("--xcd-remap", type=int, default=-1)
+    ap.add_argument("--nbr-lds", type=int, default=-1)
+    ap.add_argument("--nbr-compact", type=int, default=-1)
+    ap.add_argument("--nbr-prefetch", type=int, default=-1)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only run a device-to-device copy of MB MiB 5 times (scales HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
               "the slab decomposition it would use is described in DESIGN.md", file=sys.stderr)
@@ -93,8 +117,13 @@ def main() -> int:
           f"(generated in {time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     sim = sph.Simulation(cases if len(cases) > 1 else case)
     tuned = None
-    if args.nbr_block:
-        sim.set_tuning(nbr_block=args.nbr_block)
+    fixed = {k: v for k, v in (("nbr_block", args.nbr_block), ("nbr_unroll", args.nbr_unroll)) if v > 0}
+    fixed.update({k: v for k, v in (("xcd_remap", args.xcd_remap), ("nbr_lds", args.nbr_lds),
+                                     ("nbr_compact", args.nbr_compact), ("nbr_prefetch", args.nbr_prefetch))
+                  if v >= 0})
+    if fixed:
+        sim.set_tuning(**fixed)
+        tuned = sim.tuning.as_dict()
     elif not args.no_autotune:
         tuned, tuned_us = sim.autotune(reps=3)
         print(f"# autotune: {tuned} ({tuned_us:.1f} us for kernels A+B)", file=sys.stderr, flush=True)

@@ -195,6 +195,9 @@ typedef struct sph_tuning {
                             (nbr_block is then ignored) */
     int nbr_compact;     /* 1: scan candidates into per-lane LDS lists, then run the
                             pair body densely over the accepted ones */
+    int nbr_prefetch;    /* 1: load a candidate's velocity/dofv together with its
+                            position instead of after the cut-off test */
+    int pad_;
 } sph_tuning;
 
 /* Creates a context on the current device from host arrays in the layout of

@@ -103,6 +103,8 @@ class SphTuning(C.Structure):
         ("flags", C.c_int),
         ("nbr_lds", C.c_int),
         ("nbr_compact", C.c_int),
+        ("nbr_prefetch", C.c_int),
+        ("pad_", C.c_int),
     ]
 
     def as_dict(self) -> dict:

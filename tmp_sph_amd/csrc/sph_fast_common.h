@@ -48,8 +48,16 @@ struct RowRange {
 // Sorted index ranges of the stencil rows of the cell with global key `key`,
 // in the reference order (z outer, y inner; the x-adjacent cells of a row are
 // contiguous in sorted order, so each row is one range).
+//
+// Cells whose box lies provably beyond the cut-off from the target p are left
+// out: a whole row when its y/z face distance exceeds it, or the end cell of
+// a row (the row stays one contiguous range).  Such cells hold no pair the
+// loops would accept, so the visited pairs and their order are unchanged; in
+// 3-D about a quarter of the candidates go.  Box distances are shrunk by
+// gridSize/1024, far above the rounding of the cell bounds, and nothing is
+// culled for a target outside the grid (its cell was clamped).
 template <int DIM>
-__device__ __forceinline__ void row_ranges(const sph_params& P, int coff, int key,
+__device__ __forceinline__ void row_ranges(const sph_params& P, int coff, int key, const float4& p,
                                            const int* __restrict__ cellStart,
                                            const int* __restrict__ cellEnd,
                                            RowRange (&rows)[DIM == 3 ? 9 : 3]) {
@@ -60,16 +68,31 @@ __device__ __forceinline__ void row_ranges(const sph_params& P, int coff, int ke
   const int rem = hash - cz * XY;
   const int cy = rem / X;
   const int cx = rem - cy * X;
-  const int x0 = cx > 0 ? cx - 1 : 0;
-  const int x1 = cx < X - 1 ? cx + 1 : X - 1;
+  const float gs = P.gridSize;
+  const float m = gs * (1.0f / 1024.0f);
+  const float cut = P.rr_cut;
+  // distance from p to the lower / upper face of its own cell along each axis
+  const float lx = P.gridxmin + (float)cx * gs, ly = P.gridymin + (float)cy * gs, lz = P.gridzmin + (float)cz * gs;
+  const bool inside = p.x >= P.gridxmin && p.x <= P.gridxmin + (float)X * gs && p.z >= P.gridzmin &&
+                      p.z <= P.gridzmin + (float)Z * gs &&
+                      (DIM == 2 || (p.y >= P.gridymin && p.y <= P.gridymin + (float)Y * gs));
+  auto face = [&](float d) { return inside ? fmaxf(d - m, 0.0f) : 0.0f; };
+  const float xl = face(p.x - lx), xr = face(lx + gs - p.x);
+  const float yl = face(p.y - ly), yr = face(ly + gs - p.y);
+  const float zl = face(p.z - lz), zr = face(lz + gs - p.z);
   int nr = 0;
 #pragma unroll
   for (int dz = -1; dz <= 1; ++dz) {
 #pragma unroll
     for (int dy = (DIM == 3 ? -1 : 0); dy <= (DIM == 3 ? 1 : 0); ++dy) {
       const int z = cz + dz, y = cy + dy;
+      const float fz = dz < 0 ? zl : (dz > 0 ? zr : 0.0f);
+      const float fy = dy < 0 ? yl : (dy > 0 ? yr : 0.0f);
+      const float dyz = fz * fz + fy * fy;
       int s = 0, e = 0;
-      if (z >= 0 && z < Z && y >= 0 && y < Y) {
+      if (z >= 0 && z < Z && y >= 0 && y < Y && dyz <= cut) {
+        const int x0 = (cx > 0 && xl * xl + dyz <= cut) ? cx - 1 : cx;
+        const int x1 = (cx < X - 1 && xr * xr + dyz <= cut) ? cx + 1 : cx;
         const int rowBase = coff + z * XY + y * X;
         s = -1;
         for (int c = rowBase + x0; c <= rowBase + x1; ++c) {
@@ -86,6 +109,16 @@ __device__ __forceinline__ void row_ranges(const sph_params& P, int coff, int ke
       ++nr;
     }
   }
+}
+
+// 16-byte and 4-byte loads at a 32-bit element offset from a wave-uniform
+// base: the address is base (SGPRs) + one 32-bit VGPR offset, and the unrolled
+// candidate loads share that offset through the instruction's immediate.
+__device__ __forceinline__ float4 ld16(const float4* base, unsigned byte_off) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ float ld4(const float* base, unsigned byte_off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
 // rows[r] for a run-time r without indexing the array dynamically (which

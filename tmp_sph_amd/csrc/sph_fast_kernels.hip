@@ -205,7 +205,7 @@ __global__ void pair_stats_kernel(InstTable T, const float4* __restrict__ PR, co
     fi = pi.w > 0.0f;
     with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
       RowRange rows[DIM == 3 ? 9 : 3];
-      row_ranges<DIM>(P, coff, keys[i], cellStart, cellEnd, rows);
+      row_ranges<DIM>(P, coff, keys[i], pi, cellStart, cellEnd, rows);
       for (int r = 0; r < (DIM == 3 ? 9 : 3); ++r)
         for (int j = rows[r].s; j < rows[r].e; ++j) {
           if (j == i) continue;

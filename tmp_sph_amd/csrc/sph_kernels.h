@@ -39,6 +39,10 @@ hipError_t launch_pc_aos(const sph_params& P, bool predictor, float* sPos, float
 // ------------------------------------------------------------- fast path
 // One sorted particle set.  PR = (x, y, z, rho signed: + fluid, - otherwise),
 // VP = (vx, vy, vz, p); X = FP64 positions (SoA); ID = original id; TY = type.
+// Spare entries after PR of a fast set: the neighbour loops issue their
+// unrolled candidate loads without clamping to the row end.
+constexpr int CAND_PAD = 8;
+
 struct FastSet {
   float4* PR;
   float4* VP;

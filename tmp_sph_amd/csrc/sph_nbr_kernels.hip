@@ -30,6 +30,7 @@ namespace {
 inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 constexpr int STAGE_CAP = 1024;   // float4 entries per staged wave (16 KB of LDS)
+constexpr int STAGE_PAD = 8;      // spare tile entries read by unrolled loads past a row's end
 constexpr int COMPACT_CAP = 16;   // accepted candidates per lane per dense phase
 
 template <int DIM>
@@ -49,11 +50,13 @@ struct LdsRows {  // in the thread's own LDS slots
 };
 
 // ------------------------------------------------------- candidate sources
-// src.row(r) is called at the start of row r; src.at(j) returns PR[j].
+// src.row(r) is called at the start of row r; src.at(j0, u) returns
+// PR[j0 + u] for u < UNROLL, reading up to UNROLL - 1 entries past a row's end
+// (PR is allocated with CAND_PAD spare entries; results there are discarded).
 struct GlobalSrc {
   const float4* PR;
   __device__ __forceinline__ void row(int) {}
-  __device__ __forceinline__ float4 at(int j) const { return PR[j]; }
+  __device__ __forceinline__ float4 at(int j0, int u) const { return ld16(PR, (unsigned)j0 * 16u + 16u * u); }
 };
 
 // Candidates of row r live at tile[j + shift[r]]; the shifts are wave-uniform.
@@ -68,52 +71,82 @@ struct StagedSrc {
     for (int k = 0; k < NR; ++k) v |= shift[k] & -(int)(r == k);
     cur = v;
   }
-  __device__ __forceinline__ float4 at(int j) const { return tile[j + cur]; }
+  __device__ __forceinline__ float4 at(int j0, int u) const { return tile[j0 + cur + u]; }
 };
 
 // ------------------------------------------------------------ visit loops
-// visit(accept, pair): accept(j, pj) is the candidate test, pair(j, pj) the
-// body for an accepted candidate; both see candidates in ascending order.
-template <int NR, int U, class Rows, class Src>
+// visit(accept, extra, pair): accept(j, pj) is the candidate test, extra(j)
+// loads what the pair body needs of candidate j beyond PR (velocity etc.),
+// pair(j, pj, e) is the body for an accepted candidate; candidates are seen
+// in ascending order.  With PF (prefetch) the extra loads of all UNROLL
+// candidates are issued together with their positions, so an accepted
+// candidate does not wait a full memory latency in the pair body; without it
+// they are issued only for accepted candidates (fewer loads, one wait each).
+template <int NR, int U, bool PF, class Rows, class Src>
 struct PlainVisit {
   Rows rows;
   Src src;
-  template <class Accept, class Pair>
-  __device__ __forceinline__ void operator()(Accept&& accept, Pair&& pair) {
+  template <class Accept, class Extra, class Pair>
+  __device__ __forceinline__ void operator()(Accept&& accept, Extra&& extra, Pair&& pair) {
 #pragma unroll 1
     for (int r = 0; r < NR; ++r) {
       const RowRange g = rows(r);
       src.row(r);
       for (int j0 = g.s; j0 < g.e; j0 += U) {
         float4 pjs[U];
+        decltype(extra(0)) es[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) pjs[u] = src.at(min(j0 + u, g.e - 1));  // issued together
+        for (int u = 0; u < U; ++u) pjs[u] = src.at(j0, u);  // issued together
+        if constexpr (PF) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) es[u] = extra(j0 + u);
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u)
-          if (j0 + u < g.e && accept(j0 + u, pjs[u])) pair(j0 + u, pjs[u]);
+          if (j0 + u < g.e && accept(j0 + u, pjs[u])) pair(j0 + u, pjs[u], PF ? es[u] : extra(j0 + u));
       }
     }
   }
 };
 
-template <int NR, int U, class Rows, class Src>
+template <int NR, int U, bool PF, class Rows, class Src>
 struct CompactVisit {
   Rows rows;
   Src src;
   int* list;  // this wave's COMPACT_CAP x 64 slots, [slot][lane]
   const float4* PR;
-  template <class Accept, class Pair>
-  __device__ __forceinline__ void operator()(Accept&& accept, Pair&& pair) {
+  template <class Accept, class Extra, class Pair>
+  __device__ __forceinline__ void operator()(Accept&& accept, Extra&& extra, Pair&& pair) {
     const int lane = threadIdx.x & 63;
     int cnt = 0;
     // Dense phase over every lane's list, in order.  Called by the whole
     // wave (the conditions are wave-uniform); lanes keep their own order, so
-    // flushing early changes nothing in the sums.
+    // flushing early changes nothing in the sums.  With PF the loads of four
+    // list entries are issued before the first of them is used.
     auto flush = [&]() {
+      if constexpr (PF) {
 #pragma unroll 1
-      for (int k = 0; k < cnt; ++k) {
-        const int jj = list[k * 64 + lane];  // [slot][lane]: conflict-free for any per-lane count
-        pair(jj, PR[jj]);
+        for (int k = 0; k < cnt; k += 4) {
+          int jj[4];
+          float4 pj[4];
+          decltype(extra(0)) e[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) jj[u] = list[min(k + u, cnt - 1) * 64 + lane];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            pj[u] = ld16(PR, (unsigned)jj[u] * 16u);
+            e[u] = extra(jj[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (k + u < cnt) pair(jj[u], pj[u], e[u]);
+        }
+      } else {
+#pragma unroll 1
+        for (int k = 0; k < cnt; ++k) {
+          const int jj = list[k * 64 + lane];  // [slot][lane]: conflict-free for any per-lane count
+          pair(jj, ld16(PR, (unsigned)jj * 16u), extra(jj));
+        }
       }
       cnt = 0;
     };
@@ -128,7 +161,7 @@ struct CompactVisit {
         if (j < g.e) {
           float4 pjs[U];
 #pragma unroll
-          for (int u = 0; u < U; ++u) pjs[u] = src.at(min(j + u, g.e - 1));
+          for (int u = 0; u < U; ++u) pjs[u] = src.at(j, u);
 #pragma unroll
           for (int u = 0; u < U; ++u)
             if (j + u < g.e && accept(j + u, pjs[u])) {
@@ -175,13 +208,13 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
         return rr <= cut && j != i && pj.w > 0.0f;
       },
-      [&](int j, const float4& pj) {
+      [&](int j) { return ld16(VP, (unsigned)j * 16u); },
+      [&](int j, const float4& pj, const float4& vj) {
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
         const float q = fast_sqrt(rr) * inv_h;
         const float t = fmaf(-0.5f, q, 1.0f);
         const float t3 = t * t * t;
-        const float4 vj = VP[j];
         if (!fi) {
           const float w = (t3 * t) * (fmaf(2.0f, q, 1.0f) * P.adh);
           sp = fmaf(fmaf(-pj.w, mg * dz, vj.w), w, sp);
@@ -200,6 +233,12 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
     reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
   }
 }
+
+// What kernel B reads of a neighbour beyond PR.
+struct NbrB {
+  float4 v;  // velocity, pressure
+  float dv;  // dofv
+};
 
 // Kernel B for fluid target i (all neighbours).
 template <int DIM, class Visit>
@@ -227,10 +266,11 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
         return rr <= cut && j != i;
       },
-      [&](int j, const float4& pj) {
+      [&](int j) { return NbrB{ld16(VP, (unsigned)j * 16u), ld4(DV, (unsigned)j * 4u)}; },
+      [&](int j, const float4& pj, const NbrB& nb) {
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        const float4 vj = VP[j];
+        const float4 vj = nb.v;
         const bool fj = pj.w > 0.0f;
         const float rho_j = fabsf(pj.w);
         const float q = fast_sqrt(rr) * inv_h;
@@ -243,7 +283,7 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         const float frj = fr * inv_rho_j;
         drho = fmaf(frj * m, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
         float vt = -(vi.w + vj.w) * pm_i;
-        if (fj) vt = fmaf(avc, fmaf(damper, DV[j], dofv_i) + afa * dvdx * inv_re, vt);
+        if (fj) vt = fmaf(avc, fmaf(damper, nb.dv, dofv_i) + afa * dvdx * inv_re, vt);
         vt *= frj;
         ax = fmaf(vt, dx, ax);
         if (DIM == 3) ay = fmaf(vt, dy, ay);
@@ -306,14 +346,15 @@ template <int DIM, int U, bool STAGED, bool COMPACT>
 struct Cfg {
   static constexpr int NR = nrows<DIM>();
   static constexpr int LIST = COMPACT ? COMPACT_CAP * (STAGED ? 64 : 256) : 1;
-  static constexpr int TILE = STAGED ? STAGE_CAP : 1;
+  static constexpr int TILE = STAGED ? STAGE_CAP + STAGE_PAD : 1;
   static constexpr int SLOTS = (!STAGED && !COMPACT) ? NR * 256 : 1;
 };
 
 // Runs body(P, visit) for this lane with a visit loop over its stencil.  The
 // global path serves lanes instance by instance (wave-uniform parameters).
-template <int DIM, int U, bool STAGED, bool COMPACT, class Body>
-__device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key, bool active, const float4* PR,
+template <int DIM, int U, bool STAGED, bool COMPACT, bool PF, class Body>
+__device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key, const float4& p, bool active,
+                                           const float4* PR,
                                            const int* __restrict__ cs, const int* __restrict__ ce, float4* tile,
                                            int* lists, RowRange* slots, Body&& body) {
   constexpr int NR = nrows<DIM>();
@@ -323,18 +364,18 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
       // row ranges parked in the thread's own LDS slots (no barrier needed):
       // keeps 18 VGPRs free in 3-D, which buys occupancy for the plain loop
       RowRange rows[NR];
-      row_ranges<DIM>(P, coff, key, cs, ce, rows);
+      row_ranges<DIM>(P, coff, key, p, cs, ce, rows);
       const int t = threadIdx.x;
 #pragma unroll
       for (int r = 0; r < NR; ++r) slots[r * 256 + t] = rows[r];
-      body(P, PlainVisit<NR, U, LdsRows, GlobalSrc>{LdsRows{slots + t, 256}, GlobalSrc{PR}});
+      body(P, PlainVisit<NR, U, PF, LdsRows, GlobalSrc>{LdsRows{slots + t, 256}, GlobalSrc{PR}});
     } else {
       RegRows<NR> rows;
-      row_ranges<DIM>(P, coff, key, cs, ce, rows.v);
+      row_ranges<DIM>(P, coff, key, p, cs, ce, rows.v);
       if constexpr (COMPACT)
-        body(P, CompactVisit<NR, U, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}, list, PR});
+        body(P, CompactVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}, list, PR});
       else
-        body(P, PlainVisit<NR, U, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}});
+        body(P, PlainVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}});
     }
   };
   if constexpr (STAGED) {
@@ -344,15 +385,15 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
     const sph_params& P = T.P[u];
     const int coff = T.cell_off[u];
     RegRows<NR> rows;
-    row_ranges<DIM>(P, coff, key, cs, ce, rows.v);
+    row_ranges<DIM>(P, coff, key, p, cs, ce, rows.v);
     StagedSrc<NR> src;
     const bool staged = stage_rows<DIM>(P, coff, key, same_inst, rows.v, PR, tile, src);
     if (!active) return;
     if (staged) {
       if constexpr (COMPACT)
-        body(P, CompactVisit<NR, U, RegRows<NR>, StagedSrc<NR>>{rows, src, list, PR});
+        body(P, CompactVisit<NR, U, PF, RegRows<NR>, StagedSrc<NR>>{rows, src, list, PR});
       else
-        body(P, PlainVisit<NR, U, RegRows<NR>, StagedSrc<NR>>{rows, src});
+        body(P, PlainVisit<NR, U, PF, RegRows<NR>, StagedSrc<NR>>{rows, src});
     } else {
       with_instance(T, inst, global_path);
     }
@@ -362,7 +403,7 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
   }
 }
 
-template <int DIM, int U, bool STAGED, bool COMPACT>
+template <int DIM, int U, bool STAGED, bool COMPACT, bool PF>
 __global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, float4* PR, float4* VP,
                                                                 const int* __restrict__ TY, float* __restrict__ DV,
                                                                 const int* __restrict__ keys,
@@ -379,13 +420,13 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, flo
   const float4 pi = PR[ic];
   const float4 vi = VP[ic];
   const int key = keys[ic];
-  with_visit<DIM, U, STAGED, COMPACT>(T, inst_of(TY[ic]), key, valid, PR, cs, ce, tile, lists, slots,
+  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid, PR, cs, ce, tile, lists, slots,
                                       [&](const sph_params& P, auto visit) {
                                         nsA_target<DIM>(P, i, pi, vi, visit, PR, VP, DV);
                                       });
 }
 
-template <int DIM, int U, bool STAGED, bool COMPACT>
+template <int DIM, int U, bool STAGED, bool COMPACT, bool PF>
 __global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, const float4* __restrict__ PR,
                                                                 const float4* __restrict__ VP,
                                                                 const int* __restrict__ TY,
@@ -409,38 +450,38 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, con
   const float4 vi = VP[ic];
   const float dvi = DV[ic];
   const int key = keys[ic];
-  with_visit<DIM, U, STAGED, COMPACT>(T, inst_of(TY[ic]), key, valid && fluid, PR, cs, ce, tile, lists, slots,
+  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid && fluid, PR, cs, ce, tile, lists, slots,
                                       [&](const sph_params& P, auto visit) {
                                         nsB_target<DIM>(P, i, pi, vi, dvi, visit, VP, DV, RATE);
                                       });
 }
 
 // ------------------------------------------------------------ launchers
-template <int U, bool STAGED, bool COMPACT>
+template <int U, bool STAGED, bool COMPACT, bool PF>
 hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
                  hipStream_t s) {
   const int b = STAGED ? 64 : block;
   const dim3 g(blocks_for(n, b)), bl(b);
   if (T.dim == 3)
-    hipLaunchKernelGGL((nsA_kernel<3, U, STAGED, COMPACT>), g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
+    hipLaunchKernelGGL((nsA_kernel<3, U, STAGED, COMPACT, PF>), g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
                        (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, n, remap);
   else
-    hipLaunchKernelGGL((nsA_kernel<2, U, STAGED, COMPACT>), g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
+    hipLaunchKernelGGL((nsA_kernel<2, U, STAGED, COMPACT, PF>), g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
                        (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, n, remap);
   return hipGetLastError();
 }
 
-template <int U, bool STAGED, bool COMPACT>
+template <int U, bool STAGED, bool COMPACT, bool PF>
 hipError_t nsB_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
                  hipStream_t s) {
   const int b = STAGED ? 64 : block;
   const dim3 g(blocks_for(n, b)), bl(b);
   if (T.dim == 3)
-    hipLaunchKernelGGL((nsB_kernel<3, U, STAGED, COMPACT>), g, bl, 0, s, T, (const float4*)st.PR,
+    hipLaunchKernelGGL((nsB_kernel<3, U, STAGED, COMPACT, PF>), g, bl, 0, s, T, (const float4*)st.PR,
                        (const float4*)st.VP, (const int*)st.TY, (const float*)w.DV, (const int*)w.keys,
                        (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap);
   else
-    hipLaunchKernelGGL((nsB_kernel<2, U, STAGED, COMPACT>), g, bl, 0, s, T, (const float4*)st.PR,
+    hipLaunchKernelGGL((nsB_kernel<2, U, STAGED, COMPACT, PF>), g, bl, 0, s, T, (const float4*)st.PR,
                        (const float4*)st.VP, (const int*)st.TY, (const float*)w.DV, (const int*)w.keys,
                        (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap);
   return hipGetLastError();
@@ -450,10 +491,14 @@ template <int U>
 hipError_t nsA_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap, int mode,
                  hipStream_t s) {
   switch (mode) {
-    case 1: return nsA_v<U, true, false>(T, st, w, n, block, remap, s);
-    case 2: return nsA_v<U, false, true>(T, st, w, n, block, remap, s);
-    case 3: return nsA_v<U, true, true>(T, st, w, n, block, remap, s);
-    default: return nsA_v<U, false, false>(T, st, w, n, block, remap, s);
+    case 1: return nsA_v<U, true, false, false>(T, st, w, n, block, remap, s);
+    case 2: return nsA_v<U, false, true, false>(T, st, w, n, block, remap, s);
+    case 3: return nsA_v<U, true, true, false>(T, st, w, n, block, remap, s);
+    case 4: return nsA_v<U, false, false, true>(T, st, w, n, block, remap, s);
+    case 5: return nsA_v<U, true, false, true>(T, st, w, n, block, remap, s);
+    case 6: return nsA_v<U, false, true, true>(T, st, w, n, block, remap, s);
+    case 7: return nsA_v<U, true, true, true>(T, st, w, n, block, remap, s);
+    default: return nsA_v<U, false, false, false>(T, st, w, n, block, remap, s);
   }
 }
 
@@ -461,15 +506,21 @@ template <int U>
 hipError_t nsB_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap, int mode,
                  hipStream_t s) {
   switch (mode) {
-    case 1: return nsB_v<U, true, false>(T, st, w, n, block, remap, s);
-    case 2: return nsB_v<U, false, true>(T, st, w, n, block, remap, s);
-    case 3: return nsB_v<U, true, true>(T, st, w, n, block, remap, s);
-    default: return nsB_v<U, false, false>(T, st, w, n, block, remap, s);
+    case 1: return nsB_v<U, true, false, false>(T, st, w, n, block, remap, s);
+    case 2: return nsB_v<U, false, true, false>(T, st, w, n, block, remap, s);
+    case 3: return nsB_v<U, true, true, false>(T, st, w, n, block, remap, s);
+    case 4: return nsB_v<U, false, false, true>(T, st, w, n, block, remap, s);
+    case 5: return nsB_v<U, true, false, true>(T, st, w, n, block, remap, s);
+    case 6: return nsB_v<U, false, true, true>(T, st, w, n, block, remap, s);
+    case 7: return nsB_v<U, true, true, true>(T, st, w, n, block, remap, s);
+    default: return nsB_v<U, false, false, false>(T, st, w, n, block, remap, s);
   }
 }
 
 int norm_block(int b) { return (b >= 64 && b <= 256 && b % 64 == 0) ? b : 256; }
-int mode_of(const sph_tuning& tu) { return (tu.nbr_lds ? 1 : 0) | (tu.nbr_compact ? 2 : 0); }
+int mode_of(const sph_tuning& tu) {
+  return (tu.nbr_lds ? 1 : 0) | (tu.nbr_compact ? 2 : 0) | (tu.nbr_prefetch ? 4 : 0);
+}
 
 }  // namespace
 

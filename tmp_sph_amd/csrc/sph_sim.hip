@@ -204,10 +204,12 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   s->bits = key_bits((int)ctot);
   s->tun.nbr_block = 256;
   s->tun.nbr_unroll = 4;
-  s->tun.xcd_remap = 1;
+  s->tun.xcd_remap = 0;
   s->tun.flags = 0;
   s->tun.nbr_lds = 0;
   s->tun.nbr_compact = 0;
+  s->tun.nbr_prefetch = 0;
+  s->tun.pad_ = 0;
   int rc = SPH_OK;
   hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (he != hipSuccess) {
@@ -218,8 +220,8 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   const bool d3 = s->dim == 3;
   for (int k = 0; k < 2 && !rc; ++k) {
     FastSet& f = s->set[k];
-    rc = rc ? rc : dalloc(s, &f.PR, N);
-    rc = rc ? rc : dalloc(s, &f.VP, N);
+    rc = rc ? rc : dalloc(s, &f.PR, N + CAND_PAD);
+    rc = rc ? rc : dalloc(s, &f.VP, N + CAND_PAD);
     rc = rc ? rc : dalloc(s, &f.X[0], N);
     if (d3) rc = rc ? rc : dalloc(s, &f.X[1], N);
     rc = rc ? rc : dalloc(s, &f.X[2], N);
@@ -235,7 +237,7 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   rc = rc ? rc : dalloc(s, &w.totals, 256);
   rc = rc ? rc : dalloc(s, &w.cellStart, (size_t)ctot);
   rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)ctot);
-  rc = rc ? rc : dalloc(s, &w.DV, N);
+  rc = rc ? rc : dalloc(s, &w.DV, N + CAND_PAD);
   rc = rc ? rc : dalloc(s, &w.RATE, N);
   rc = rc ? rc : dalloc(s, &w.TX[0], N);
   if (d3) rc = rc ? rc : dalloc(s, &w.TX[1], N);
@@ -331,11 +333,14 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
   const bool shape_changed = t->nbr_block != s->tun.nbr_block || t->nbr_unroll != s->tun.nbr_unroll ||
                              (t->xcd_remap != 0) != (s->tun.xcd_remap != 0) ||
                              (t->nbr_lds != 0) != (s->tun.nbr_lds != 0) ||
-                             (t->nbr_compact != 0) != (s->tun.nbr_compact != 0);
+                             (t->nbr_compact != 0) != (s->tun.nbr_compact != 0) ||
+                             (t->nbr_prefetch != 0) != (s->tun.nbr_prefetch != 0);
   s->tun = *t;
   s->tun.xcd_remap = t->xcd_remap ? 1 : 0;
   s->tun.nbr_lds = t->nbr_lds ? 1 : 0;
   s->tun.nbr_compact = t->nbr_compact ? 1 : 0;
+  s->tun.nbr_prefetch = t->nbr_prefetch ? 1 : 0;
+  s->tun.pad_ = 0;
   if (shape_changed) drop_graph(s);  // a captured graph holds the old launch shapes
   std::fill(s->tsum_ms.begin(), s->tsum_ms.end(), 0.0);
   std::fill(s->tcount.begin(), s->tcount.end(), 0);
@@ -533,8 +538,8 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
 }
 
 // Autotuner: times kernels A and B on the live state for every launch shape
-// (plain or compact loop x LDS-staged or global with block 64/128/256 x
-// unroll 1/2/4/8 x XCD remap on/off) with HIP events on
+// (neighbour loads prefetched or not x plain or compact loop x LDS-staged or
+// global with block 64/128/256 x unroll 1/2/4/8 x XCD remap on/off) with HIP events on
 // the context's stream and keeps the fastest.  The neighbour phase is
 // re-evaluated from the same sorted set each time, so the state is unchanged
 // and every candidate sees identical work.
@@ -556,6 +561,7 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   hipError_t err = hipSuccess;
   // the LDS-staged variant always runs one-wave blocks: it is tried once per
   // unroll/remap pair (bl == 0 below)
+  for (int prefetch = 0; prefetch < 2 && err == hipSuccess; ++prefetch)
   for (int compact = 0; compact < 2 && err == hipSuccess; ++compact)
   for (int remap = 0; remap < 2 && err == hipSuccess; ++remap)
     for (int bl : blocks)
@@ -567,6 +573,7 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
         t.xcd_remap = remap;
         t.nbr_lds = bl == 0 ? 1 : 0;
         t.nbr_compact = compact;
+        t.nbr_prefetch = prefetch;
         s->tun = t;
         err = rates_phase(s, 1, w, false);  // warm-up
         if (err == hipSuccess) err = hipEventRecord(a, s->stream);
