@@ -25,6 +25,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 FP32_VALU_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
 def _workload(name: str):
@@ -80,12 +81,96 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+def learned_bench(args) -> int:
+    """Surrogate against the plain GPU solver on a held-out 2-D dam break.
+
+    Training data: windows of one batched plain run of the sweep members with
+    dx = 0.004 m (velocity history of 5 frames 10 steps apart, target = change
+    over the next 10 steps).  Training: Adam on the GPU for a fixed budget.
+    Evaluation: a 0.8 x 0.5 m column at dx = 0.002 m (100k fluid particles), a
+    geometry outside the training set; plain solver to t = 2 s, timed; the
+    surrogate rollout (10 plain steps per network step, plain step as judge
+    and fallback at 1 % density deviation) to --learned-t, timed and scaled
+    to t = 2 s; surge-front error against the plain run at every interval."""
+    import numpy as np
+    from tmp_sph_amd import cases as cs
+    from tmp_sph_amd import sph
+    from tmp_sph_amd import surrogate as sg
+    k, history, layers = 10, 5, 10
+    t0 = time.time()
+    members = [c for c, m in zip(cs.sweep_cases(), cs.sweep_members()) if m.dx == 0.004][:12]
+    dts = sorted(float(c.params.timeStep) for c in members)
+    total = int(2.0 / dts[len(dts) // 2])
+    ds = sg.record_windows(members, total, k, history, samples=30)
+    print(f"# recorded {len(ds.snaps)} snapshots of {len(members)} instances over {total} steps "
+          f"({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(ds.snaps))
+    held = [ds.snaps[i] for i in order[:24]]
+    train = [ds.snaps[i] for i in order[24:]]
+    norm = sg.normalisation(train)
+    nmax = max(s.pos.shape[0] for s in ds.snaps)
+    kw = dict(layers=layers, history=history, v_scale=3.0, rho_scale=100.0, dv_scale=norm["dv_scale"],
+              drho_scale=norm["drho_scale"], out_mean=norm["out_mean"])
+    g = sg.Gnn(max_nodes=nmax, max_edges=30 * nmax, train=True, seed=11, **kw)
+    before = sg.evaluate(g, held, norm)
+    t1 = time.time()
+    losses = sg.train(g, train, norm, iters=1_000_000, lr=3e-4, lr_final=3e-5, seconds=args.train_seconds)
+    train_s = time.time() - t1
+    after = sg.evaluate(g, held, norm)
+    print(f"# trained {len(losses)} iterations in {train_s:.0f} s: held-out mse {before['network']:.3f} -> "
+          f"{after['network']:.3f} (constant-acceleration baseline {after['constant_acceleration']:.3f})",
+          file=sys.stderr, flush=True)
+    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.002, tank=(4.0, 1.0, 1.0), column=(0.8, 1.0, 0.5)))
+    gi = sg.Gnn(max_nodes=case.n, max_edges=30 * case.n, train=False, seed=11, **kw)
+    gi.set_params(g.params())
+    g.close()
+    res = sg.learned_vs_plain(gi, case, k, args.learned_t, 2.0, dev_tol=0.01)
+    # network throughput on the evaluation state
+    hist = np.zeros((history, case.n, 3), np.float32)
+    gi.load(case.params, case.pos, hist, case.rho, case.type)
+    off, send, _ = gi.graph()
+    gi.forward(want_out=False)
+    reps = 10
+    t2 = time.perf_counter()
+    for _ in range(reps):
+        gi.forward(want_out=False)
+    fwd_s = (time.perf_counter() - t2) / reps
+    flops = sg.forward_flops(layers, case.n, int(send.size))
+    gi.close()
+    out = {
+        "metric": "learned vs plain speedup at <=2% front error",
+        "value": res["speedup"],
+        "unit": "x (plain wall time / learned wall time to t = 2 s)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "bf16 MFMA (fp32 accumulate) network, f32 solver",
+        "data": "synthetic: trained on plain-solver trajectories of 12 sweep members (dx = 0.004 m)",
+        "config": {"workload": "held-out 2-D dam break, 0.8x0.5 m column in 4.0x1.0 m tank, dx=0.002 m",
+                   "particles_total": int(case.n), "particles_fluid": int(case.n_fluid), "k_steps_per_net_step": k,
+                   "layers": layers, "latent": 128, "fallback_density_deviation": 0.01},
+        "front_rel_error_max": res["front_rel_error_max"],
+        "front_within_2pct": bool(res["front_rel_error_max"] <= 0.02),
+        "evaluation": res,
+        "training": {"iterations": len(losses), "seconds": train_s, "snapshots": len(train),
+                     "held_out_mse_before": before, "held_out_mse_after": after},
+        "network_forward": {"seconds": fwd_s, "edges": int(send.size), "flops": flops,
+                            "tflops": flops / fwd_s / 1e12, "mfma_peak_tflops": BF16_MFMA_PEAK_TFLOPS,
+                            "mfma_frac": flops / fwd_s / 1e12 / BF16_MFMA_PEAK_TFLOPS},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned"])
+    ap.add_argument("--train-seconds", type=float, default=120.0, help="learned: GPU training budget")
+    ap.add_argument("--learned-t", type=float, default=0.25, help="learned: simulated time of the timed rollout")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=10, help="eager steps with per-kernel HIP events")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -101,6 +186,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.config == "learned":
+        return learned_bench(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
               "the slab decomposition it would use is described in DESIGN.md", file=sys.stderr)

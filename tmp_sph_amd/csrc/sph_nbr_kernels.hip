@@ -78,10 +78,11 @@ struct StagedSrc {
 // visit(accept, extra, pair): accept(j, pj) is the candidate test, extra(j)
 // loads what the pair body needs of candidate j beyond PR (velocity etc.),
 // pair(j, pj, e) is the body for an accepted candidate; candidates are seen
-// in ascending order.  With PF (prefetch) the extra loads of all UNROLL
-// candidates are issued together with their positions, so an accepted
-// candidate does not wait a full memory latency in the pair body; without it
-// they are issued only for accepted candidates (fewer loads, one wait each).
+// in ascending order.  The loads for the accepted candidates of one unrolled
+// group are issued together before the first pair body, so the group waits
+// one memory latency, not one per accepted candidate.  With PF (prefetch)
+// they are issued for every candidate together with its position (more
+// loads, no wait after the cut-off test).
 template <int NR, int U, bool PF, class Rows, class Src>
 struct PlainVisit {
   Rows rows;
@@ -101,9 +102,18 @@ struct PlainVisit {
 #pragma unroll
           for (int u = 0; u < U; ++u) es[u] = extra(j0 + u);
         }
+        bool acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = j0 + u < g.e && accept(j0 + u, pjs[u]);
+        if constexpr (!PF) {
+          // the accepted candidates' loads go out together: one wait per group
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (acc[u]) es[u] = extra(j0 + u);
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u)
-          if (j0 + u < g.e && accept(j0 + u, pjs[u])) pair(j0 + u, pjs[u], PF ? es[u] : extra(j0 + u));
+          if (acc[u]) pair(j0 + u, pjs[u], es[u]);
       }
     }
   }

@@ -237,7 +237,8 @@ def normalised_target(s: Snapshot, norm: dict) -> np.ndarray:
 
 def train(g: "Gnn", snaps: list, norm: dict, iters: int, lr: float = 3e-4, lr_final: float = 3e-5,
           clip: float = 1.0, seed: int = 0, seconds: float | None = None) -> list:
-    """Adam on single-snapshot batches; cosine learning-rate decay.  Stops
+    """Adam on single-snapshot batches; cosine learning-rate decay over the
+    iterations or, with a time budget, over the elapsed fraction of it.  Stops
     after `iters` steps or `seconds` of wall-clock, whichever comes first."""
     import time
     rng = np.random.default_rng(seed)
@@ -250,6 +251,8 @@ def train(g: "Gnn", snaps: list, norm: dict, iters: int, lr: float = 3e-4, lr_fi
         _, loss = g.forward(want_out=False)
         g.backward()
         frac = it / max(iters - 1, 1)
+        if seconds is not None:
+            frac = max(frac, min((time.time() - t0) / seconds, 1.0))
         g.adam(lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac)), clip)
         losses.append(loss)
         if seconds is not None and time.time() - t0 > seconds:
@@ -293,3 +296,94 @@ def record(case: Case, nsteps: int, k: int, history: int) -> Dataset:
                         nxt["rho"] - cur["rho"]], 1).astype(np.float32)
         ds.snaps.append(Snapshot(case.params, cur["pos"], hist.astype(np.float32), cur["rho"], cur["type"], tgt))
     return ds
+
+
+def record_windows(cases: list, total_steps: int, k: int, history: int, samples: int) -> Dataset:
+    """Snapshots of one batched plain run of `cases` over `total_steps` steps:
+    at `samples` evenly spaced sample steps T the frames T - (history-1) k,
+    ..., T give the velocity history and the frame T + k the target.  Only the
+    frames of the windows are downloaded, so long runs stay cheap."""
+    ds = Dataset(k, history)
+    span = history * k  # steps covered by one window
+    first = (history - 1) * k
+    if samples < 1 or total_steps < first + k:
+        raise ValueError("run too short for one window")
+    gap = max((total_steps - first - k) // max(samples - 1, 1), span)
+    with Simulation(cases) as sim:
+        sim.capture()
+        cur = 0
+        offs = sim.offsets
+        for s in range(samples):
+            T = first + s * gap
+            if T + k > total_steps:
+                break
+            start = T - first
+            if start > cur:
+                sim.replay(start - cur)
+                cur = start
+            frames = []
+            for h in range(history + 1):
+                frames.append(sim.state())
+                if h < history:
+                    sim.replay(k)
+                    cur += k
+            for c, case in enumerate(cases):
+                lo, hi = offs[c], offs[c + 1]
+                fr = [{key: v[lo:hi] for key, v in f.items()} for f in frames]
+                now, nxt = fr[history - 1], fr[history]
+                hist = np.stack([fr[history - 1 - j]["vel"] for j in range(history)])
+                tgt = np.stack([nxt["vel"][:, 0] - now["vel"][:, 0], nxt["vel"][:, 2] - now["vel"][:, 2],
+                                nxt["rho"] - now["rho"]], 1).astype(np.float32)
+                ds.snaps.append(Snapshot(case.params, now["pos"], hist.astype(np.float32), now["rho"], now["type"],
+                                         tgt))
+    return ds
+
+
+def forward_flops(layers: int, n: int, edges: int) -> float:
+    """Multiply-add work (2 flop each) of one forward pass: three 128-wide
+    linear layers per MLP; encoder and processor inputs are 128 (encoders),
+    384 (edge update) and 256 (node update) wide."""
+    per_row = lambda kin: 2.0 * W * (kin + W + W)
+    return (n * per_row(W) + edges * per_row(W) + layers * (edges * per_row(3 * W) + n * per_row(2 * W))
+            + n * per_row(W))
+
+
+def learned_vs_plain(g: "Gnn", case: Case, k: int, t_learned: float, t_plain: float, dev_tol: float = 0.01,
+                     band: float = 0.02) -> dict:
+    """Plain GPU solver against the surrogate rollout on one case.
+
+    The plain run to t_plain is timed as one graph replay; a second plain run
+    records the surge front every k steps up to t_learned, where the surrogate
+    rollout (k plain steps per network step, plain step as judge and
+    fallback) also records it.  The learned time to t_plain is the measured
+    rollout time scaled by t_plain / t_learned (the rollout cost per interval
+    does not depend on the time reached)."""
+    import time
+    dt = float(case.params.timeStep)
+    intervals = max(int(round(t_learned / (dt * k))), 1)
+    steps_plain = int(round(t_plain / dt))
+    with Simulation(case) as sim:
+        sim.capture()
+        sim.replay(k)  # warm-up
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.replay(steps_plain)
+        sim.synchronize()
+        plain_s = time.perf_counter() - t0
+    front_p = np.zeros(intervals)
+    with Simulation(case) as sim:
+        sim.capture()
+        for it in range(intervals):
+            sim.replay(k)
+            front_p[it], _ = sim.probe(band)
+    with Simulation(case) as sim:
+        st, front_l, height_l = g.rollout(sim, k, intervals, dev_tol=dev_tol, band=band)
+    learned_s = st["seconds"] * (steps_plain / (intervals * k))
+    rel = np.abs(front_l - front_p) / np.maximum(front_p, 1e-9)
+    return {"particles": int(case.n), "fluid": int(case.n_fluid), "dt": dt, "k": k,
+            "t_plain": steps_plain * dt, "plain_seconds": plain_s,
+            "t_learned": intervals * k * dt, "learned_seconds_measured": st["seconds"],
+            "learned_seconds_to_t_plain": learned_s, "speedup": plain_s / learned_s,
+            "front_rel_error_max": float(rel.max()), "front_rel_error_final": float(rel[-1]),
+            "front_plain_final": float(front_p[-1]), "front_learned_final": float(front_l[-1]),
+            "rollout": st}
