@@ -45,8 +45,11 @@ def _algorithmic_bytes(name: str, dim: int, n: int, nf: int) -> float:
     nsB: PR (16 B) of every target; VP (16 B), dofv (4 B) and key (4 B) of
     fluid targets; RATE (16 B) written for every target.  nsA: PR, VP, key read
     (36 B) and dofv written (4 B) for every target, wall p and rho (8 B).
+    nbrlist (neighbour-list mode): PR and key read, count written.
     """
     nw = n - nf
+    if name == "nbrlist":  # PR and key read, count written (the lists themselves are intermediate)
+        return 24.0 * n
     if name == "nsB":
         return 16.0 * n + 24.0 * nf + 16.0 * n
     if name == "nsA":
@@ -181,6 +184,7 @@ def main() -> int:
     ap.add_argument("--nbr-lds", type=int, default=-1)
     ap.add_argument("--nbr-compact", type=int, default=-1)
     ap.add_argument("--nbr-prefetch", type=int, default=-1)
+    ap.add_argument("--nbr-list", type=int, default=-1)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB MiB 5 times (scales HBM byte counters) and exit")
     args = ap.parse_args()
@@ -206,7 +210,8 @@ def main() -> int:
     tuned = None
     fixed = {k: v for k, v in (("nbr_block", args.nbr_block), ("nbr_unroll", args.nbr_unroll)) if v > 0}
     fixed.update({k: v for k, v in (("xcd_remap", args.xcd_remap), ("nbr_lds", args.nbr_lds),
-                                     ("nbr_compact", args.nbr_compact), ("nbr_prefetch", args.nbr_prefetch))
+                                     ("nbr_compact", args.nbr_compact), ("nbr_prefetch", args.nbr_prefetch),
+                                     ("nbr_list", args.nbr_list))
                   if v >= 0})
     if fixed:
         sim.set_tuning(**fixed)
@@ -233,7 +238,7 @@ def main() -> int:
     stats = sim.pair_stats()
     front, height = sim.probe(band=0.02)
     sim.close()
-    dom = max(("nsA", "nsB"), key=lambda k: kt.get(k, 0.0))
+    dom = max(("nsA", "nsB", "nbrlist"), key=lambda k: kt.get(k, 0.0))
     us = kt[dom]
     byts = _algorithmic_bytes(dom, prm.dim, n_total, n_fluid)
     achieved = byts / (us * 1e-6) / 1e9

@@ -197,7 +197,8 @@ typedef struct sph_tuning {
                             pair body densely over the accepted ones */
     int nbr_prefetch;    /* 1: load a candidate's velocity/dofv together with its
                             position instead of after the cut-off test */
-    int pad_;
+    int nbr_list;        /* 1: one scan records each target's neighbours, kernels A
+                            and B iterate the lists (block/unroll/remap apply) */
 } sph_tuning;
 
 /* Creates a context on the current device from host arrays in the layout of
@@ -290,10 +291,12 @@ typedef struct sph_gnn_cfg {
     int train;           /* 1: keep per-layer activations for backward */
     unsigned seed;       /* weight initialisation */
     float v_scale;       /* velocity normalisation */
-    float dv_scale;      /* output velocity-change normalisation */
-    float drho_scale;    /* output density-change normalisation */
+    float dv_scale;      /* output acceleration normalisation (mean dv/dt over k steps) */
+    float drho_scale;    /* output density-rate normalisation (mean drho/dt over k steps) */
     float rho_scale;     /* input density deviation scale ((rho/rho0 - 1) * rho_scale) */
-    float out_mean[3];   /* output = normalised prediction * scale + out_mean (dvx, dvz, drho) */
+    float out_mean[3];   /* rate = normalised prediction * scale + out_mean (dvx/dt, dvz/dt, drho/dt);
+                            the change over k steps is rate * k * dt, so one network serves
+                            cases of different resolution (dt) */
 } sph_gnn_cfg;
 
 typedef struct sph_rollout_stats {
@@ -310,8 +313,9 @@ typedef struct sph_rollout_stats {
    method: the SPH summation density (the kernel sum of kernel A's Wendland
    weights) of the predicted positions must change like the predicted density
    -- mean |d rho_sum - d rho_pred| / rho0 over fluid particles with a full
-   support <= dev_tol -- and no fluid particle may leave the tank; otherwise
-   the interval is redone with k plain steps.  The first `history` intervals
+   support <= dev_tol -- and no fluid particle may pass a tank face by more
+   than half a particle spacing (into the wall layer); otherwise the interval
+   is redone with k plain steps.  The first `history` intervals
    are plain (they fill the velocity history).  front/height (may be null)
    receive the probes after every interval. */
 int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, float dev_tol, double band,

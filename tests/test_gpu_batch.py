@@ -72,24 +72,25 @@ def test_autotune_keeps_results(gpu):
                 np.testing.assert_array_equal(a.pos64(), c.pos64())
 
 
-_VARIANTS = [(lds, compact, pf, un) for lds in (0, 1) for compact in (0, 1) for pf in (0, 1) for un in (1, 4, 8)]
+_VARIANTS = ([(lds, compact, pf, 0, un) for lds in (0, 1) for compact in (0, 1) for pf in (0, 1) for un in (1, 4, 8)]
+             + [(0, 0, 0, 1, un) for un in (1, 4, 8)])
 
 
 @pytest.mark.parametrize("dim", [2, 3])
 def test_neighbour_loop_variants_bit_identical(gpu, dim):
-    """LDS staging, the scan/dense (compact) loop and prefetched neighbour loads
-    visit the same pairs in the same order as the plain global loop: every
-    rate is bit-identical."""
+    """LDS staging, the scan/dense (compact) loop, prefetched neighbour loads and
+    neighbour lists visit the same pairs in the same order as the plain global
+    loop: every rate is bit-identical."""
     from tmp_sph_amd.sph import dambreak_cfg, make_case
     if dim == 3:
         case = make_case(dambreak_cfg(dim=3, dx=0.03, tank=(0.8, 0.4, 0.5), column=(0.3, 0.4, 0.3)))
     else:
         case = make_case(dambreak_cfg(dim=2, dx=0.01))
     ref = None
-    for lds, compact, pf, un in _VARIANTS:
+    for lds, compact, pf, lst, un in _VARIANTS:
         with gpu.Simulation(case) as s:
             s.step(3)  # default loop; leaves the lattice so every rate is non-trivial
-            s.set_tuning(nbr_lds=lds, nbr_compact=compact, nbr_prefetch=pf, nbr_unroll=un)
+            s.set_tuning(nbr_lds=lds, nbr_compact=compact, nbr_prefetch=pf, nbr_list=lst, nbr_unroll=un)
             r = s.eval_rates()
             s.step(2)
             r["pos"] = s.pos64()
@@ -98,7 +99,8 @@ def test_neighbour_loop_variants_bit_identical(gpu, dim):
             continue
         for k in ("drho", "acc", "dofv", "p", "rho", "pos"):
             np.testing.assert_array_equal(r[k], ref[k],
-                                          err_msg=f"lds={lds} compact={compact} prefetch={pf} unroll={un} {k}")
+                                          err_msg=f"lds={lds} compact={compact} prefetch={pf} list={lst} "
+                                                  f"unroll={un} {k}")
 
 
 def test_neighbour_loop_variants_batched(gpu):
@@ -106,12 +108,13 @@ def test_neighbour_loop_variants_batched(gpu):
     variant; results stay identical to the plain loop."""
     cases = _cases()
     ref = None
-    for lds, compact, pf, un in _VARIANTS:
+    for lds, compact, pf, lst, un in _VARIANTS:
         with gpu.Simulation(cases) as b:
-            b.set_tuning(nbr_lds=lds, nbr_compact=compact, nbr_prefetch=pf, nbr_unroll=un)
+            b.set_tuning(nbr_lds=lds, nbr_compact=compact, nbr_prefetch=pf, nbr_list=lst, nbr_unroll=un)
             b.step(6)
             r = b.pos64()
         if ref is None:
             ref = r
             continue
-        np.testing.assert_array_equal(r, ref, err_msg=f"lds={lds} compact={compact} prefetch={pf} unroll={un}")
+        np.testing.assert_array_equal(r, ref, err_msg=f"lds={lds} compact={compact} prefetch={pf} list={lst} "
+                                                     f"unroll={un}")

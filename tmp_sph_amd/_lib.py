@@ -104,7 +104,7 @@ class SphTuning(C.Structure):
         ("nbr_lds", C.c_int),
         ("nbr_compact", C.c_int),
         ("nbr_prefetch", C.c_int),
-        ("pad_", C.c_int),
+        ("nbr_list", C.c_int),
     ]
 
     def as_dict(self) -> dict:

@@ -760,8 +760,9 @@ __device__ __forceinline__ float wendland2d(const sph_params& P, float rr) {
   return P.adh * t * t * t * t * (2.0f * q + 1.0f);
 }
 
-// Applies the prediction (fluid only): v' = v + dv, x' = x + k dt (v + v')/2,
-// rho' = max(rho0, rho + drho), p' = EOS(rho'); walls keep their state.
+// Applies the prediction (fluid only): with T = k dt and the predicted rates,
+// v' = v + T dv/dt, x' = x + T (v + v')/2, rho' = max(rho0, rho + T drho/dt),
+// p' = EOS(rho'); walls keep their state.
 __global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* __restrict__ Y,
                              const float* __restrict__ pos, const float* __restrict__ vel,
                              const float* __restrict__ rho, const int* __restrict__ type, float* __restrict__ pos2,
@@ -773,10 +774,10 @@ __global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* 
   float nx = x, nz = z, nvx = vx, nvz = vz, nr = rho[i];
   if (type[i] == 1) {
     const float* y = Y + (size_t)i * W;
-    nvx = vx + y[0] * cfg.dv_scale + cfg.out_mean[0];
-    nvz = vz + y[1] * cfg.dv_scale + cfg.out_mean[1];
-    nr = fmaxf(rho[i] + y[2] * cfg.drho_scale + cfg.out_mean[2], P.restDensity);
     const float T = (float)k * P.timeStep;
+    nvx = vx + (y[0] * cfg.dv_scale + cfg.out_mean[0]) * T;
+    nvz = vz + (y[1] * cfg.dv_scale + cfg.out_mean[1]) * T;
+    nr = fmaxf(rho[i] + (y[2] * cfg.drho_scale + cfg.out_mean[2]) * T, P.restDensity);
     nx = x + 0.5f * T * (vx + nvx);
     nz = z + 0.5f * T * (vz + nvz);
   }
@@ -802,7 +803,10 @@ __global__ void judge_kernel(sph_params P, const int* __restrict__ off, const in
   float dev = 0.0f, cnt = 0.0f, esc = 0.0f;
   if (i < n && type[i] == 1) {
     const float x2 = pos2[3 * i], z2 = pos2[3 * i + 2];
-    if ((double)x2 < P.domain_min[0] || (double)x2 > P.domain_max[0] || (double)z2 < P.domain_min[2]) esc = 1.0f;
+    // penetration deeper than half a particle spacing into a wall layer
+    const double tol = 0.5 * (double)P.particleR;
+    if ((double)x2 < P.domain_min[0] - tol || (double)x2 > P.domain_max[0] + tol || (double)z2 < P.domain_min[2] - tol)
+      esc = 1.0f;
     const int b = off[i], e = off[i + 1];
     if (e - b >= full_deg) {
       const float W0 = wendland2d(P, 0.0f);

@@ -65,7 +65,15 @@ struct FastWork {
   float4* RATE;    // (drho/dt, ax, ay, az)
   double* TX[3];   // positions at the start of the step (predictor slot order)
   float4* TV;      // (vx, vy, vz, rho) at the start of the step
+  int* NL;         // neighbour lists, slot-major: NL[k * n + i] (k < nl_cap)
+  int* NC;         // neighbour count of each target (> nl_cap: list overflowed)
+  int nl_cap;
 };
+
+// Neighbour-list capacity per target: 3-D supports hold ~74 neighbours on a
+// lattice, 2-D ~21; targets with more use the scan path.
+constexpr int NL_CAP_3D = 128;
+constexpr int NL_CAP_2D = 48;
 
 // Device table of the instances of one context (K >= 1).  Instance k owns the
 // global cells [cell_off[k], cell_off[k] + P[k].numCells); all share `dim`.
@@ -82,6 +90,11 @@ hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const
                                     const int* idx, int* cellStart, int* cellEnd, int n, hipStream_t s);
 hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s);
+// Neighbour-list mode: one scan records each target's in-support neighbours
+// (cut-off and j != i, fluid and wall) in visit order; kernels A and B then
+// iterate the lists.  Same pairs in the same order as the scan kernels.
+hipError_t launch_nbrlist_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                               hipStream_t s);
 hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s);
 hipError_t launch_pc_fast(const InstTable& T, bool predictor, const FastSet& st, const FastWork& w,

@@ -186,6 +186,38 @@ struct CompactVisit {
   }
 };
 
+// Iterates target i's recorded neighbour list (slot-major, NL[k * stride + i])
+// in recorded order; the loads of four entries go out before the first is
+// used.
+struct ListVisit {
+  const int* NL;
+  int stride;
+  int i;
+  int cnt;
+  const float4* PR;
+  template <class Accept, class Extra, class Pair>
+  __device__ __forceinline__ void operator()(Accept&& accept, Extra&& extra, Pair&& pair) {
+#pragma unroll 1
+    for (int k = 0; k < cnt; k += 4) {
+      int jj[4];
+      float4 pj[4];
+      decltype(extra(0)) e[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jj[u] = NL[(size_t)min(k + u, cnt - 1) * stride + i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        pj[u] = ld16(PR, (unsigned)jj[u] * 16u);
+        e[u] = extra(jj[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k + u < cnt && accept(jj[u], pj[u])) pair(jj[u], pj[u], e[u]);
+    }
+  }
+};
+
+struct NoExtra {};
+
 // ------------------------------------------------------------ per target
 // The pair arithmetic below is written with explicit fmaf and compiled with
 // contraction off, so that every loop variant (whose code the compiler
@@ -466,6 +498,101 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, con
                                       });
 }
 
+// ------------------------------------------------------------ neighbour lists
+// Records every target's in-support neighbours (rr <= cut, j != i; fluid and
+// wall) in the order the scan kernels visit them.
+template <int DIM, int U>
+__global__ __launch_bounds__(256) void nbrlist_kernel(InstTable T, const float4* __restrict__ PR,
+                                                      const int* __restrict__ TY, const int* __restrict__ keys,
+                                                      const int* __restrict__ cs, const int* __restrict__ ce,
+                                                      int* __restrict__ NL, int* __restrict__ NC, int cap, int n,
+                                                      int remap) {
+  constexpr int NR = nrows<DIM>();
+  __shared__ RowRange slots[NR * 256];
+  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = PR[i];
+  const int key = keys[i];
+  int cnt = 0;
+  with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
+    RowRange rows[NR];
+    row_ranges<DIM>(P, coff, key, pi, cs, ce, rows);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) slots[r * 256 + t] = rows[r];
+    const float cut = P.rr_cut;
+    PlainVisit<NR, U, false, LdsRows, GlobalSrc>{LdsRows{slots + t, 256}, GlobalSrc{PR}}(
+        [&](int j, const float4& pj) {
+          float dx, dy, dz;
+          return geom<DIM>(pi, pj, dx, dy, dz) <= cut && j != i;
+        },
+        [&](int) { return NoExtra{}; },
+        [&](int j, const float4&, const NoExtra&) {
+          if (cnt < cap) NL[(size_t)cnt * n + i] = j;
+          ++cnt;
+        });
+  });
+  NC[i] = cnt;
+}
+
+// Kernels A and B over the recorded lists; a target whose list overflowed
+// scans its stencil instead.
+template <int DIM>
+__global__ __launch_bounds__(256) void nsA_list_kernel(InstTable T, float4* PR, float4* VP, const int* __restrict__ TY,
+                                                       float* __restrict__ DV, const int* __restrict__ keys,
+                                                       const int* __restrict__ cs, const int* __restrict__ ce,
+                                                       const int* __restrict__ NL, const int* __restrict__ NC,
+                                                       int cap, int n, int remap) {
+  constexpr int NR = nrows<DIM>();
+  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = PR[i];
+  const float4 vi = VP[i];
+  const int key = keys[i];
+  const int cnt = NC[i];
+  with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
+    if (cnt <= cap) {
+      nsA_target<DIM>(P, i, pi, vi, ListVisit{NL, n, i, cnt, PR}, PR, VP, DV);
+    } else {
+      RegRows<NR> rows;
+      row_ranges<DIM>(P, coff, key, pi, cs, ce, rows.v);
+      nsA_target<DIM>(P, i, pi, vi, PlainVisit<NR, 2, false, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}}, PR, VP,
+                      DV);
+    }
+  });
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void nsB_list_kernel(InstTable T, const float4* __restrict__ PR,
+                                                       const float4* __restrict__ VP, const int* __restrict__ TY,
+                                                       const float* __restrict__ DV, const int* __restrict__ keys,
+                                                       const int* __restrict__ cs, const int* __restrict__ ce,
+                                                       const int* __restrict__ NL, const int* __restrict__ NC,
+                                                       float4* __restrict__ RATE, int cap, int n, int remap) {
+  constexpr int NR = nrows<DIM>();
+  const int i = tile_id(remap) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 pi = PR[i];
+  if (!(pi.w > 0.0f)) {
+    RATE[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const float4 vi = VP[i];
+  const float dvi = DV[i];
+  const int key = keys[i];
+  const int cnt = NC[i];
+  with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
+    if (cnt <= cap) {
+      nsB_target<DIM>(P, i, pi, vi, dvi, ListVisit{NL, n, i, cnt, PR}, VP, DV, RATE);
+    } else {
+      RegRows<NR> rows;
+      row_ranges<DIM>(P, coff, key, pi, cs, ce, rows.v);
+      nsB_target<DIM>(P, i, pi, vi, dvi, PlainVisit<NR, 2, false, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}}, VP,
+                      DV, RATE);
+    }
+  });
+}
+
 // ------------------------------------------------------------ launchers
 template <int U, bool STAGED, bool COMPACT, bool PF>
 hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
@@ -532,12 +659,72 @@ int mode_of(const sph_tuning& tu) {
   return (tu.nbr_lds ? 1 : 0) | (tu.nbr_compact ? 2 : 0) | (tu.nbr_prefetch ? 4 : 0);
 }
 
+template <int U>
+hipError_t nbrlist_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
+                     hipStream_t s) {
+  const dim3 g(blocks_for(n, block)), bl(block);
+  if (T.dim == 3)
+    hipLaunchKernelGGL((nbrlist_kernel<3, U>), g, bl, 0, s, T, (const float4*)st.PR, (const int*)st.TY,
+                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, w.NL, w.NC, w.nl_cap, n,
+                       remap);
+  else
+    hipLaunchKernelGGL((nbrlist_kernel<2, U>), g, bl, 0, s, T, (const float4*)st.PR, (const int*)st.TY,
+                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, w.NL, w.NC, w.nl_cap, n,
+                       remap);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_nbrlist_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0;
+  switch (tu.nbr_unroll) {
+    case 1: return nbrlist_u<1>(T, st, w, n, b, rm, s);
+    case 2: return nbrlist_u<2>(T, st, w, n, b, rm, s);
+    case 8: return nbrlist_u<8>(T, st, w, n, b, rm, s);
+    default: return nbrlist_u<4>(T, st, w, n, b, rm, s);
+  }
+}
+
+namespace {
+
+hipError_t nsA_list(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
+                    hipStream_t s) {
+  const dim3 g(blocks_for(n, block)), bl(block);
+  if (T.dim == 3)
+    hipLaunchKernelGGL(nsA_list_kernel<3>, g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV, (const int*)w.keys,
+                       (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.NL, (const int*)w.NC, w.nl_cap,
+                       n, remap);
+  else
+    hipLaunchKernelGGL(nsA_list_kernel<2>, g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV, (const int*)w.keys,
+                       (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.NL, (const int*)w.NC, w.nl_cap,
+                       n, remap);
+  return hipGetLastError();
+}
+
+hipError_t nsB_list(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
+                    hipStream_t s) {
+  const dim3 g(blocks_for(n, block)), bl(block);
+  if (T.dim == 3)
+    hipLaunchKernelGGL(nsB_list_kernel<3>, g, bl, 0, s, T, (const float4*)st.PR, (const float4*)st.VP,
+                       (const int*)st.TY, (const float*)w.DV, (const int*)w.keys, (const int*)w.cellStart,
+                       (const int*)w.cellEnd, (const int*)w.NL, (const int*)w.NC, w.RATE, w.nl_cap, n, remap);
+  else
+    hipLaunchKernelGGL(nsB_list_kernel<2>, g, bl, 0, s, T, (const float4*)st.PR, (const float4*)st.VP,
+                       (const int*)st.TY, (const float*)w.DV, (const int*)w.keys, (const int*)w.cellStart,
+                       (const int*)w.cellEnd, (const int*)w.NL, (const int*)w.NC, w.RATE, w.nl_cap, n, remap);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0, md = mode_of(tu);
+  if (tu.nbr_list) return nsA_list(T, st, w, n, b, rm, s);
   switch (tu.nbr_unroll) {
     case 1: return nsA_u<1>(T, st, w, n, b, rm, md, s);
     case 2: return nsA_u<2>(T, st, w, n, b, rm, md, s);
@@ -550,6 +737,7 @@ hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0, md = mode_of(tu);
+  if (tu.nbr_list) return nsB_list(T, st, w, n, b, rm, s);
   switch (tu.nbr_unroll) {
     case 1: return nsB_u<1>(T, st, w, n, b, rm, md, s);
     case 2: return nsB_u<2>(T, st, w, n, b, rm, md, s);

@@ -120,6 +120,11 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
 }
 
 hipError_t rates_phase(sph_sim* s, int set, const FastWork& w, bool timing) {
+  if (s->tun.nbr_list) {
+    hipError_t e =
+        timed(s, timing, "nbrlist", [&] { return launch_nbrlist_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
+    if (e != hipSuccess) return e;
+  }
   hipError_t e = timed(s, timing, "nsA", [&] { return launch_nsA_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
   if (e != hipSuccess) return e;
   return timed(s, timing, "nsB", [&] { return launch_nsB_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
@@ -209,7 +214,7 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   s->tun.nbr_lds = 0;
   s->tun.nbr_compact = 0;
   s->tun.nbr_prefetch = 0;
-  s->tun.pad_ = 0;
+  s->tun.nbr_list = 0;
   int rc = SPH_OK;
   hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (he != hipSuccess) {
@@ -238,6 +243,9 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   rc = rc ? rc : dalloc(s, &w.cellStart, (size_t)ctot);
   rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)ctot);
   rc = rc ? rc : dalloc(s, &w.DV, N + CAND_PAD);
+  w.nl_cap = d3 ? NL_CAP_3D : NL_CAP_2D;
+  rc = rc ? rc : dalloc(s, &w.NL, N * (size_t)w.nl_cap);
+  rc = rc ? rc : dalloc(s, &w.NC, N);
   rc = rc ? rc : dalloc(s, &w.RATE, N);
   rc = rc ? rc : dalloc(s, &w.TX[0], N);
   if (d3) rc = rc ? rc : dalloc(s, &w.TX[1], N);
@@ -334,13 +342,14 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
                              (t->xcd_remap != 0) != (s->tun.xcd_remap != 0) ||
                              (t->nbr_lds != 0) != (s->tun.nbr_lds != 0) ||
                              (t->nbr_compact != 0) != (s->tun.nbr_compact != 0) ||
-                             (t->nbr_prefetch != 0) != (s->tun.nbr_prefetch != 0);
+                             (t->nbr_prefetch != 0) != (s->tun.nbr_prefetch != 0) ||
+                             (t->nbr_list != 0) != (s->tun.nbr_list != 0);
   s->tun = *t;
   s->tun.xcd_remap = t->xcd_remap ? 1 : 0;
   s->tun.nbr_lds = t->nbr_lds ? 1 : 0;
   s->tun.nbr_compact = t->nbr_compact ? 1 : 0;
   s->tun.nbr_prefetch = t->nbr_prefetch ? 1 : 0;
-  s->tun.pad_ = 0;
+  s->tun.nbr_list = t->nbr_list ? 1 : 0;
   if (shape_changed) drop_graph(s);  // a captured graph holds the old launch shapes
   std::fill(s->tsum_ms.begin(), s->tsum_ms.end(), 0.0);
   std::fill(s->tcount.begin(), s->tcount.end(), 0);
@@ -538,8 +547,9 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
 }
 
 // Autotuner: times kernels A and B on the live state for every launch shape
-// (neighbour loads prefetched or not x plain or compact loop x LDS-staged or
-// global with block 64/128/256 x unroll 1/2/4/8 x XCD remap on/off) with HIP events on
+// (scan kernels: neighbour loads prefetched or not x plain or compact loop x
+// LDS-staged or global with block 64/128/256; or neighbour lists with block
+// 64/128/256; each x unroll 1/2/4/8 x XCD remap on/off) with HIP events on
 // the context's stream and keeps the fastest.  The neighbour phase is
 // re-evaluated from the same sorted set each time, so the state is unchanged
 // and every candidate sees identical work.
@@ -561,8 +571,9 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   hipError_t err = hipSuccess;
   // the LDS-staged variant always runs one-wave blocks: it is tried once per
   // unroll/remap pair (bl == 0 below)
-  for (int prefetch = 0; prefetch < 2 && err == hipSuccess; ++prefetch)
-  for (int compact = 0; compact < 2 && err == hipSuccess; ++compact)
+  for (int list = 0; list < 2 && err == hipSuccess; ++list)
+  for (int prefetch = 0; prefetch < 2 - list && err == hipSuccess; ++prefetch)
+  for (int compact = 0; compact < 2 - list && err == hipSuccess; ++compact)
   for (int remap = 0; remap < 2 && err == hipSuccess; ++remap)
     for (int bl : blocks)
       for (int un : unrolls) {
@@ -574,6 +585,8 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
         t.nbr_lds = bl == 0 ? 1 : 0;
         t.nbr_compact = compact;
         t.nbr_prefetch = prefetch;
+        t.nbr_list = list;
+        if (list && bl == 0) continue;  // the list kernels have no LDS-staged form
         s->tun = t;
         err = rates_phase(s, 1, w, false);  // warm-up
         if (err == hipSuccess) err = hipEventRecord(a, s->stream);

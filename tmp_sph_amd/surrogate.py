@@ -4,9 +4,10 @@ trajectories of the plain solver and run with the library's BF16 MFMA kernels.
 * :class:`Gnn` -- ctypes wrapper of ``sph_gnn_*`` (forward, backward, Adam on
   the device).
 * :func:`gnn_layout` -- the flat parameter layout (also returned by the library).
-* :func:`make_dataset` / :func:`train` -- snapshots every k steps of plain 2-D
-  dam-break runs, inputs = velocity history + wall distances + type + density,
-  targets = velocity and density change over the next k steps.
+* :func:`record` / :func:`record_windows` / :func:`train` -- snapshots of
+  plain 2-D dam-break runs, inputs = velocity history + wall distances + type
+  + density, targets = mean acceleration and density rate over the next k
+  steps (rates, so that one network serves cases of different dt).
 """
 from __future__ import annotations
 
@@ -188,7 +189,7 @@ class Snapshot:
     hist: np.ndarray    # (history, n, 3), newest first
     rho: np.ndarray
     type: np.ndarray
-    target: np.ndarray  # (n, 3) raw: dvx, dvz, drho over the next k steps
+    target: np.ndarray  # (n, 3) raw mean rates over the next k steps: dvx/dt, dvz/dt, drho/dt
 
 
 @dataclass
@@ -196,6 +197,12 @@ class Dataset:
     k: int
     history: int
     snaps: list = field(default_factory=list)
+
+
+def _rates(cur: dict, nxt: dict, T: float) -> np.ndarray:
+    """Mean rates over an interval T between two frames: (dvx, dvz, drho) / T."""
+    return (np.stack([nxt["vel"][:, 0] - cur["vel"][:, 0], nxt["vel"][:, 2] - cur["vel"][:, 2],
+                      nxt["rho"] - cur["rho"]], 1).astype(np.float64) / T).astype(np.float32)
 
 
 def record_batch(cases: list, nsteps: int, k: int, history: int) -> Dataset:
@@ -214,8 +221,7 @@ def record_batch(cases: list, nsteps: int, k: int, history: int) -> Dataset:
         for t in range(history - 1, len(fr) - 1):
             cur, nxt = fr[t], fr[t + 1]
             hist = np.stack([fr[t - j]["vel"] for j in range(history)])
-            tgt = np.stack([nxt["vel"][:, 0] - cur["vel"][:, 0], nxt["vel"][:, 2] - cur["vel"][:, 2],
-                            nxt["rho"] - cur["rho"]], 1).astype(np.float32)
+            tgt = _rates(cur, nxt, k * float(case.params.timeStep))
             ds.snaps.append(Snapshot(case.params, cur["pos"], hist.astype(np.float32), cur["rho"], cur["type"], tgt))
     return ds
 
@@ -292,8 +298,7 @@ def record(case: Case, nsteps: int, k: int, history: int) -> Dataset:
     for t in range(history - 1, len(frames) - 1):
         cur, nxt = frames[t], frames[t + 1]
         hist = np.stack([frames[t - j]["vel"] for j in range(history)])
-        tgt = np.stack([nxt["vel"][:, 0] - cur["vel"][:, 0], nxt["vel"][:, 2] - cur["vel"][:, 2],
-                        nxt["rho"] - cur["rho"]], 1).astype(np.float32)
+        tgt = _rates(cur, nxt, k * float(case.params.timeStep))
         ds.snaps.append(Snapshot(case.params, cur["pos"], hist.astype(np.float32), cur["rho"], cur["type"], tgt))
     return ds
 
@@ -332,8 +337,7 @@ def record_windows(cases: list, total_steps: int, k: int, history: int, samples:
                 fr = [{key: v[lo:hi] for key, v in f.items()} for f in frames]
                 now, nxt = fr[history - 1], fr[history]
                 hist = np.stack([fr[history - 1 - j]["vel"] for j in range(history)])
-                tgt = np.stack([nxt["vel"][:, 0] - now["vel"][:, 0], nxt["vel"][:, 2] - now["vel"][:, 2],
-                                nxt["rho"] - now["rho"]], 1).astype(np.float32)
+                tgt = _rates(now, nxt, k * float(case.params.timeStep))
                 ds.snaps.append(Snapshot(case.params, now["pos"], hist.astype(np.float32), now["rho"], now["type"],
                                          tgt))
     return ds
