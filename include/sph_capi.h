@@ -250,6 +250,18 @@ int sph_sim_download_pos64(sph_sim* s, double* pos);
 /* Surge front (max x over in-domain fluid) and column height (max z over fluid
    with x <= band) computed on the device; values are written to host doubles. */
 int sph_sim_probe(sph_sim* s, double band, double* front, double* height);
+/* Measuring probes of the reference's output step (src/io/io.cu:236-279),
+   on the device.  Height gauges: for gauge g the largest z of a fluid
+   particle of instance 0 with |x - gx[g]| < halfw (and |y - gy[g]| < halfw
+   in 3-D) and z < zcap[g], 0 if there is none (the reference uses bands of
+   half width 1.5 dx at x = 2.228 and 0.582 m, y = 0.5 m, the first capped at
+   z < 0.4 m).  Point probes: the pressure of the particles with the given
+   original indices (the reference's measuring nodes).  At most
+   SPH_MAX_PROBES of each per call. */
+#define SPH_MAX_PROBES 64
+int sph_sim_gauges(sph_sim* s, int ngauges, const float* gx, const float* gy, const float* zcap, float halfw,
+                   float* heights);
+int sph_sim_pressure_at(sph_sim* s, int nids, const int* ids, float* p);
 /* Average device time of each kernel launched by the last timed sph_sim_step /
    sph_sim_replay when timing is enabled (flags bit 0 of sph_tuning); names are
    written as a ';'-separated list.  Microseconds per launch. */

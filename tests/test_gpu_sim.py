@@ -153,3 +153,31 @@ def test_full_size_properties_c3(gpu, orc):
     # at rest the interior is hydrostatic: |a_z| small against g away from walls and surface
     assert np.isfinite(got["acc"]).all() and np.isfinite(got["drho"]).all()
     assert np.all(got["rho"][~fl] >= case.params.restDensity)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gauges_and_pressure_probes_match_numpy(gpu, dim):
+    """The reference's measuring probes on the device: height gauges (largest
+    fluid z in an x (and y) band under a cap) and point pressures by id."""
+    case = case2d(0.01) if dim == 2 else case3d(0.03)
+    with gpu.Simulation(case) as sim:
+        sim.step(30)
+        st = sim.state()
+        dx = float(case.params.particleR)
+        hw = 1.5 * dx
+        gx = np.array([0.05, 0.2, 0.5, 3.9], np.float32)
+        gy = np.full(4, 0.5 * float(case.params.domain_max[1]) if dim == 3 else 0.0, np.float32)
+        zc = np.array([10.0, 0.2, 10.0, 10.0], np.float32)
+        got = sim.gauges(gx, gy, zc, hw)
+        rng = np.random.default_rng(0)
+        ids = rng.choice(case.n, 8, replace=False)
+        p = sim.pressure_at(ids)
+    pos, fl = st["pos"], st["type"] == 1
+    for g in range(4):
+        sel = fl & (np.abs(pos[:, 0] - gx[g]) < np.float32(hw)) & (pos[:, 2] < zc[g])
+        if dim == 3:
+            sel &= np.abs(pos[:, 1] - gy[g]) < np.float32(hw)
+        want = max(float(pos[sel, 2].max()), 0.0) if sel.any() else 0.0
+        assert got[g] == np.float32(want), (g, got[g], want)
+    assert got[0] > 0 and got[3] == 0  # a band through the column, one beyond the water
+    np.testing.assert_array_equal(p, st["p"][ids])

@@ -153,6 +153,8 @@ _PROTOS = {
     "sph_sim_download": [P, P, P, P, P, P],
     "sph_sim_download_pos64": [P, P],
     "sph_sim_probe": [P, C.c_double, DP, DP],
+    "sph_sim_gauges": [P, I, P, P, P, C.c_float, P],
+    "sph_sim_pressure_at": [P, I, P, P],
     "sph_sim_kernel_times": [P, C.c_char_p, I, DP, IP, I],
     "sph_sim_steps_done": [P, C.POINTER(C.c_longlong)],
     "sph_sim_eval": [P, P, P, P, P, P, P, P],

@@ -55,6 +55,59 @@ __device__ __forceinline__ bf16x8 load_a8(const AOperand& A, int row, int k, int
   return load_seg8(A.seg[2], row, k - w0 - w1);
 }
 
+// Raw (unconverted) 8-element A fragment: two 16-byte words holding 8 FP32
+// values, or 8 BF16 values in the first word.  Loading raw and converting one
+// k-step later keeps the loads in flight across the MFMAs of the current step.
+struct ARaw {
+  uint4 a, b;
+  int f32;
+};
+
+__device__ __forceinline__ ARaw load_raw_seg(const ASeg& g, int row, int kk) {
+  ARaw r;
+  const int rr = g.rows ? g.rows[row] : row;
+  if (g.is_f32) {
+    const uint4* p = reinterpret_cast<const uint4*>(static_cast<const float*>(g.ptr) + (size_t)rr * g.ld + kk);
+    r.a = p[0];
+    r.b = p[1];
+    r.f32 = 1;
+  } else {
+    r.a = *reinterpret_cast<const uint4*>(static_cast<const bf16*>(g.ptr) + (size_t)rr * g.ld + kk);
+    r.b = make_uint4(0u, 0u, 0u, 0u);
+    r.f32 = 0;
+  }
+  return r;
+}
+
+// (segments picked with constant indices: a pointer into the kernel-argument
+// struct would move it to scratch)
+__device__ __forceinline__ ARaw load_raw8(const AOperand& A, int row, int k, int M) {
+  if (row >= M) {
+    ARaw r;
+    r.a = make_uint4(0u, 0u, 0u, 0u);
+    r.b = r.a;
+    r.f32 = 0;
+    return r;
+  }
+  const int w0 = A.seg[0].width;
+  if (A.nseg == 1 || k < w0) return load_raw_seg(A.seg[0], row, k);
+  const int w1 = A.seg[1].width;
+  if (A.nseg == 2 || k < w0 + w1) return load_raw_seg(A.seg[1], row, k - w0);
+  return load_raw_seg(A.seg[2], row, k - w0 - w1);
+}
+
+__device__ __forceinline__ bf16x8 convert8(const ARaw& r) {
+  if (r.f32) {
+    float4 x, y;
+    __builtin_memcpy(&x, &r.a, 16);
+    __builtin_memcpy(&y, &r.b, 16);
+    return cvt8(x, y);
+  }
+  bf16x8 v;
+  __builtin_memcpy(&v, &r.a, 16);
+  return v;
+}
+
 __device__ __forceinline__ float load_a1(const AOperand& A, int row, int k) {
   const ASeg* g = &A.seg[0];
   int kk = k;
@@ -71,8 +124,12 @@ __device__ __forceinline__ float load_a1(const AOperand& A, int row, int k) {
   return (float)static_cast<const bf16*>(g->ptr)[(size_t)r * g->ld + kk];
 }
 
+// Main loop, software-pipelined: the A fragments and the B piece of step
+// k + 1 are loaded while the MFMAs of step k run; the B tile is double
+// buffered in LDS, so one barrier per step suffices (a buffer is rewritten two
+// steps after it was read, and every wave has passed the barrier in between).
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-  __shared__ bf16 ldsB[BN * LDB];
+  __shared__ bf16 ldsB[2][BN * LDB];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lq = lane >> 4;
   const int m0 = blockIdx.x * BM + w * 32;
@@ -87,27 +144,39 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   f32x4 acc[2][8];
   for (int t = 0; t < 2; ++t)
     for (int c = 0; c < 8; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  // this thread's share of the B tile: row n of B, 16 elements at 16 * half
+  const int bn = tid >> 1, bhalf = tid & 1;
+  const bf16* bsrc = g.B + (size_t)(n0 + bn) * g.K + 16 * bhalf;
+  uint4 bq0 = make_uint4(0u, 0u, 0u, 0u), bq1 = bq0;
+  ARaw ar0{}, ar1{};
+  if (kbeg < kend) {
+    bq0 = *reinterpret_cast<const uint4*>(bsrc + kbeg);
+    bq1 = *reinterpret_cast<const uint4*>(bsrc + kbeg + 8);
+    ar0 = load_raw8(g.A, m0 + li, kbeg + 8 * lq, g.M);
+    ar1 = load_raw8(g.A, m0 + 16 + li, kbeg + 8 * lq, g.M);
+  }
+  int buf = 0;
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    {
-      // stage B[n0 .. n0+127][k0 .. k0+31]: two 16-byte pieces per thread
-      const int n = tid >> 1, half = tid & 1;
-      const bf16* src = g.B + (size_t)(n0 + n) * g.K + k0 + 16 * half;
-      const uint4 a = *reinterpret_cast<const uint4*>(src);
-      const uint4 b = *reinterpret_cast<const uint4*>(src + 8);
-      *reinterpret_cast<uint4*>(&ldsB[n * LDB + 16 * half]) = a;
-      *reinterpret_cast<uint4*>(&ldsB[n * LDB + 16 * half + 8]) = b;
-    }
-    __syncthreads();
+    *reinterpret_cast<uint4*>(&ldsB[buf][bn * LDB + 16 * bhalf]) = bq0;
+    *reinterpret_cast<uint4*>(&ldsB[buf][bn * LDB + 16 * bhalf + 8]) = bq1;
     bf16x8 af[2];
-    af[0] = load_a8(g.A, m0 + li, k0 + 8 * lq, g.M);
-    af[1] = load_a8(g.A, m0 + 16 + li, k0 + 8 * lq, g.M);
+    af[0] = convert8(ar0);
+    af[1] = convert8(ar1);
+    __syncthreads();
+    if (k0 + BK < kend) {  // next step's operands, in flight during the MFMAs
+      bq0 = *reinterpret_cast<const uint4*>(bsrc + k0 + BK);
+      bq1 = *reinterpret_cast<const uint4*>(bsrc + k0 + BK + 8);
+      ar0 = load_raw8(g.A, m0 + li, k0 + BK + 8 * lq, g.M);
+      ar1 = load_raw8(g.A, m0 + 16 + li, k0 + BK + 8 * lq, g.M);
+    }
+    const bf16* tileB = ldsB[buf];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      const bf16x8 bf = *reinterpret_cast<const bf16x8*>(&ldsB[(16 * c + li) * LDB + 8 * lq]);
+      const bf16x8 bf = *reinterpret_cast<const bf16x8*>(&tileB[(16 * c + li) * LDB + 8 * lq]);
       acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf, acc[0][c], 0, 0, 0);
       acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf, acc[1][c], 0, 0, 0);
     }
-    __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int t = 0; t < 2; ++t)
@@ -157,11 +226,49 @@ __global__ void reduce_partials_kernel(const float* __restrict__ part, int npart
 }
 
 // partial column sums: block b sums rows b, b + G, b + 2G, ... for all N columns
-__global__ void column_partial_kernel(const float* __restrict__ X, int M, int N, int ld, float* __restrict__ part) {
-  for (int n = threadIdx.x; n < N; n += blockDim.x) {
-    float s = 0.0f;
-    for (int m = blockIdx.x; m < M; m += gridDim.x) s += X[(size_t)m * ld + n];
-    part[(size_t)blockIdx.x * N + n] = s;
+// Partial column sums (N <= 128, N % 4 == 0): 256 threads = 8 row lanes x 32
+// float4 column groups; each thread keeps four independent accumulators so
+// several row loads are in flight; the 8 row lanes are combined in LDS.
+__global__ __launch_bounds__(256) void column_partial_kernel(const float* __restrict__ X, int M, int N, int ld,
+                                                             float* __restrict__ part) {
+  __shared__ float4 red[8][32];
+  const int cg = threadIdx.x & 31, rl = threadIdx.x >> 5;
+  const bool active = 4 * cg < N;
+  float4 a[4];
+  for (int u = 0; u < 4; ++u) a[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const long long stride = 8LL * gridDim.x;
+  long long m = (long long)blockIdx.x * 8 + rl;
+  if (active) {
+    for (; m + 3 * stride < M; m += 4 * stride) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4 v = *reinterpret_cast<const float4*>(X + (size_t)(m + u * stride) * ld + 4 * cg);
+        a[u].x += v.x;
+        a[u].y += v.y;
+        a[u].z += v.z;
+        a[u].w += v.w;
+      }
+    }
+    for (; m < M; m += stride) {
+      const float4 v = *reinterpret_cast<const float4*>(X + (size_t)m * ld + 4 * cg);
+      a[0].x += v.x;
+      a[0].y += v.y;
+      a[0].z += v.z;
+      a[0].w += v.w;
+    }
+  }
+  red[rl][cg] = make_float4(a[0].x + a[1].x + a[2].x + a[3].x, a[0].y + a[1].y + a[2].y + a[3].y,
+                            a[0].z + a[1].z + a[2].z + a[3].z, a[0].w + a[1].w + a[2].w + a[3].w);
+  __syncthreads();
+  if (rl == 0 && active) {
+    float4 t = red[0][cg];
+    for (int r = 1; r < 8; ++r) {
+      t.x += red[r][cg].x;
+      t.y += red[r][cg].y;
+      t.z += red[r][cg].z;
+      t.w += red[r][cg].w;
+    }
+    *reinterpret_cast<float4*>(part + (size_t)blockIdx.x * N + 4 * cg) = t;
   }
 }
 
@@ -296,7 +403,8 @@ hipError_t column_sum(const float* X, int M, int N, int ld, float* colsum, float
                       hipStream_t s) {
   if (M <= 0) return hipSuccess;
   const int G = 256;
-  hipLaunchKernelGGL(column_partial_kernel, dim3(G), dim3(128), 0, s, X, M, N, ld, scratch);
+  if (N > 128 || N % 4 != 0 || ld % 4 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(column_partial_kernel, dim3(G), dim3(256), 0, s, X, M, N, ld, scratch);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return reduce_partials(scratch, G, N, colsum, accumulate, s);

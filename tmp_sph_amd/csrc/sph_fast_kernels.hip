@@ -348,6 +348,48 @@ hipError_t launch_pair_stats(const InstTable& T, const FastSet& st, const FastWo
                     (const int*)w.cellEnd, out4, n);
 }
 
+// Height gauges (the reference's measuring bands, src/io/io.cu:236-255):
+// gauge g reports the largest z of an instance-0 fluid particle with
+// |x - gx[g]| < hw (and |y - gy[g]| < hw in 3-D) and z < zcap[g].  Heights
+// are stored as the bits of non-negative floats, whose integer order is the
+// float order.
+__global__ void gauge_kernel(InstTable T, FastSet st, const float* __restrict__ gx, const float* __restrict__ gy,
+                             const float* __restrict__ zcap, int ng, float hw, unsigned* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ty = st.TY[i];
+  if (inst_of(ty) != 0 || type_of(ty) != 1) return;
+  const float4 p = st.PR[i];
+  for (int g = 0; g < ng; ++g) {
+    const bool in_x = fabsf(p.x - gx[g]) < hw;
+    const bool in_y = T.dim == 2 || fabsf(p.y - gy[g]) < hw;
+    if (in_x && in_y && p.z < zcap[g]) atomicMax(&out[g], __float_as_uint(fmaxf(p.z, 0.0f)));
+  }
+}
+
+// Pressure of the particles with the given original ids (the reference's
+// measuring nodes, src/io/io.cu:268-276).
+__global__ void pressure_at_kernel(FastSet st, const int* __restrict__ ids, int nid, float* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int id = st.ID[i];
+  for (int k = 0; k < nid; ++k)
+    if (ids[k] == id) out[k] = st.VP[i].w;
+}
+
+hipError_t launch_gauges_fast(const InstTable& T, const FastSet& st, const float* gx, const float* gy,
+                              const float* zcap, int ng, float hw, unsigned* out, int n, hipStream_t s) {
+  if (n <= 0 || ng <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gauge_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, st, gx, gy, zcap, ng, hw, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pressure_at_fast(const FastSet& st, const int* ids, int nid, float* out, int n, hipStream_t s) {
+  if (n <= 0 || nid <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pressure_at_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, st, ids, nid, out, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_probe_fast(const InstTable& T, const FastSet& st, double band, unsigned long long* out, int n,
                              hipStream_t s) {
   if (n <= 0) return hipSuccess;

@@ -107,6 +107,9 @@ hipError_t launch_download_rates(const FastSet& st, const FastWork& w, float* dr
                                  hipStream_t s);
 hipError_t launch_pair_stats(const InstTable& T, const FastSet& st, const FastWork& w, unsigned long long* out4,
                              int n, hipStream_t s);
+hipError_t launch_gauges_fast(const InstTable& T, const FastSet& st, const float* gx, const float* gy,
+                              const float* zcap, int ng, float hw, unsigned* out, int n, hipStream_t s);
+hipError_t launch_pressure_at_fast(const FastSet& st, const int* ids, int nid, float* out, int n, hipStream_t s);
 hipError_t launch_probe_fast(const InstTable& T, const FastSet& st, double band, unsigned long long* out, int n,
                              hipStream_t s);
 

@@ -30,6 +30,7 @@ struct sph_sim {
   hipGraphExec_t exec = nullptr;
   long long steps = 0;
   unsigned long long* probe = nullptr;  // 2K words (also 4 for pair statistics)
+  unsigned* gbuf = nullptr;             // gauges / point probes: 5 x SPH_MAX_PROBES words
   std::vector<void*> allocs;
   // kernel timing (eager steps with tuning.flags bit 0)
   std::vector<std::string> tnames;
@@ -252,6 +253,7 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   rc = rc ? rc : dalloc(s, &w.TX[2], N);
   rc = rc ? rc : dalloc(s, &w.TV, N);
   rc = rc ? rc : dalloc(s, &s->probe, (size_t)std::max(2 * K, 4));
+  rc = rc ? rc : dalloc(s, &s->gbuf, (size_t)5 * SPH_MAX_PROBES);
   sph_params* dP = nullptr;
   int* dC = nullptr;
   rc = rc ? rc : dalloc(s, &dP, (size_t)K);
@@ -476,6 +478,39 @@ extern "C" int sph_sim_probe(sph_sim* s, double band, double* front, double* hei
   if (rc) return rc;
   *front = *std::max_element(f.begin(), f.end());
   *height = *std::max_element(h.begin(), h.end());
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_gauges(sph_sim* s, int ng, const float* gx, const float* gy, const float* zcap, float halfw,
+                              float* heights) {
+  if (!s || !gx || !gy || !zcap || !heights) return set_error(SPH_E_INVAL, "null argument");
+  if (ng < 1 || ng > SPH_MAX_PROBES) return set_error(SPH_E_INVAL, "sph_sim_gauges: 1 <= ngauges <= SPH_MAX_PROBES");
+  if (!(halfw > 0.0f)) return set_error(SPH_E_INVAL, "sph_sim_gauges: half width must be positive");
+  std::vector<float> h(3 * (size_t)ng);
+  std::copy(gx, gx + ng, h.begin());
+  std::copy(gy, gy + ng, h.begin() + ng);
+  std::copy(zcap, zcap + ng, h.begin() + 2 * ng);
+  float* dg = reinterpret_cast<float*>(s->gbuf);
+  unsigned* dout = s->gbuf + 3 * SPH_MAX_PROBES;
+  SPH_HIP(hipMemcpyAsync(dg, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s->stream), "gauges");
+  SPH_HIP(hipMemsetAsync(dout, 0, (size_t)ng * sizeof(unsigned), s->stream), "gauges clear");
+  SPH_HIP(launch_gauges_fast(s->T, s->set[0], dg, dg + ng, dg + 2 * ng, ng, halfw, dout, s->n, s->stream), "gauges");
+  SPH_HIP(hipMemcpyAsync(heights, dout, (size_t)ng * sizeof(float), hipMemcpyDeviceToHost, s->stream), "gauges copy");
+  SPH_HIP(hipStreamSynchronize(s->stream), "gauges sync");
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_pressure_at(sph_sim* s, int nid, const int* ids, float* p) {
+  if (!s || !ids || !p) return set_error(SPH_E_INVAL, "null argument");
+  if (nid < 1 || nid > SPH_MAX_PROBES) return set_error(SPH_E_INVAL, "sph_sim_pressure_at: 1 <= nids <= SPH_MAX_PROBES");
+  for (int k = 0; k < nid; ++k)
+    if (ids[k] < 0 || ids[k] >= s->n) return set_error(SPH_E_INVAL, "sph_sim_pressure_at: id out of range");
+  int* dids = reinterpret_cast<int*>(s->gbuf);
+  float* dout = reinterpret_cast<float*>(s->gbuf + SPH_MAX_PROBES);
+  SPH_HIP(hipMemcpyAsync(dids, ids, (size_t)nid * sizeof(int), hipMemcpyHostToDevice, s->stream), "pressure ids");
+  SPH_HIP(launch_pressure_at_fast(s->set[0], dids, nid, dout, s->n, s->stream), "pressure probe");
+  SPH_HIP(hipMemcpyAsync(p, dout, (size_t)nid * sizeof(float), hipMemcpyDeviceToHost, s->stream), "pressure copy");
+  SPH_HIP(hipStreamSynchronize(s->stream), "pressure sync");
   return SPH_OK;
 }
 

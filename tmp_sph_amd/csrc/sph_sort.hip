@@ -18,8 +18,8 @@ namespace {
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
-constexpr int RS_IPT = 16;
-constexpr int RS_TILE = RS_THREADS * RS_IPT;
+constexpr int RS_IPT_MAX = 16;  // keys per thread and tile of the largest tile
+constexpr int RS_MIN_TILES = 512;  // small sorts use smaller tiles to fill the GPU
 
 static_assert(RS_THREADS == 256, "one thread per digit in the per-digit phases");
 
@@ -35,6 +35,7 @@ __device__ __forceinline__ unsigned long long match_digit(unsigned d, int nbits,
   return m;
 }
 
+template <int RS_IPT>
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const int* __restrict__ keys, int n,
                                                          int shift, int nbits,
                                                          unsigned* __restrict__ hist,
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const int* __restrict__
   cnt[tid] = 0;
   __syncthreads();
   const unsigned mask = (1u << nbits) - 1u;
-  const long long base = (long long)blockIdx.x * RS_TILE + (long long)w * (64 * RS_IPT);
+  const long long base = (long long)blockIdx.x * (RS_THREADS * RS_IPT) + (long long)w * (64 * RS_IPT);
   for (int r = 0; r < RS_IPT; ++r) {
     const long long i = base + r * 64 + lane;
     const bool valid = i < n;
@@ -85,6 +86,7 @@ __global__ __launch_bounds__(256) void rs_scan_rows(unsigned* __restrict__ hist,
   if (threadIdx.x == 255) totals[blockIdx.x] = s[255];
 }
 
+template <int RS_IPT>
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     const int* __restrict__ keys_in, const int* __restrict__ vals_in, int* __restrict__ keys_out,
     int* __restrict__ vals_out, int n, int shift, int nbits, const unsigned* __restrict__ hist,
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
 
   const unsigned mask = (1u << nbits) - 1u;
   const unsigned long long ltmask = (1ull << lane) - 1ull;
-  const long long base = (long long)blockIdx.x * RS_TILE + (long long)w * (64 * RS_IPT);
+  const long long base = (long long)blockIdx.x * (RS_THREADS * RS_IPT) + (long long)w * (64 * RS_IPT);
   int kk[RS_IPT], vv[RS_IPT];
   unsigned rk[RS_IPT];
 #pragma unroll
@@ -159,8 +161,20 @@ __global__ void fill_int_kernel(int* __restrict__ p, int value, long long n) {
 
 }  // namespace
 
+namespace {
+
+// Keys per thread: the largest of 16, 8, 4, 2, 1 that still gives
+// RS_MIN_TILES tiles (small sorts would otherwise run on a few dozen CUs).
+int radix_ipt(int n) {
+  int ipt = RS_IPT_MAX;
+  while (ipt > 1 && (n + RS_THREADS * ipt - 1) / (RS_THREADS * ipt) < RS_MIN_TILES) ipt >>= 1;
+  return ipt;
+}
+
+}  // namespace
+
 int radix_hist_words(int n) {
-  int tiles = (n + RS_TILE - 1) / RS_TILE;
+  int tiles = (n + RS_THREADS - 1) / RS_THREADS;  // the smallest tile
   if (tiles < 1) tiles = 1;
   return 256 * tiles;
 }
@@ -173,12 +187,12 @@ hipError_t fill_int(int* p, int value, long long n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t radix_sort_pairs(int* keys, int* vals, int* keys_alt, int* vals_alt, unsigned* hist,
-                            unsigned* totals, int n, int bits, hipStream_t s,
-                            bool* result_in_alt) {
-  *result_in_alt = false;
-  if (n <= 1) return hipSuccess;
-  const int numTiles = (n + RS_TILE - 1) / RS_TILE;
+namespace {
+
+template <int IPT>
+hipError_t radix_passes(int* keys, int* vals, int* keys_alt, int* vals_alt, unsigned* hist, unsigned* totals, int n,
+                        int bits, hipStream_t s, bool* result_in_alt) {
+  const int numTiles = (n + RS_THREADS * IPT - 1) / (RS_THREADS * IPT);
   int* kin = keys;
   int* vin = vals;
   int* kout = keys_alt;
@@ -186,11 +200,10 @@ hipError_t radix_sort_pairs(int* keys, int* vals, int* keys_alt, int* vals_alt, 
   bool in_alt = false;
   for (int shift = 0; shift < bits; shift += 8) {
     const int nb = bits - shift < 8 ? bits - shift : 8;
-    hipLaunchKernelGGL(rs_upsweep, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, n, shift, nb,
-                       hist, numTiles);
+    hipLaunchKernelGGL(rs_upsweep<IPT>, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, n, shift, nb, hist, numTiles);
     hipLaunchKernelGGL(rs_scan_rows, dim3(256), dim3(256), 0, s, hist, numTiles, totals);
-    hipLaunchKernelGGL(rs_downsweep, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, vin, kout, vout,
-                       n, shift, nb, hist, totals, numTiles);
+    hipLaunchKernelGGL(rs_downsweep<IPT>, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, vin, kout, vout, n, shift, nb,
+                       hist, totals, numTiles);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     int* t = kin; kin = kout; kout = t;
@@ -199,6 +212,22 @@ hipError_t radix_sort_pairs(int* keys, int* vals, int* keys_alt, int* vals_alt, 
   }
   *result_in_alt = in_alt;
   return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t radix_sort_pairs(int* keys, int* vals, int* keys_alt, int* vals_alt, unsigned* hist,
+                            unsigned* totals, int n, int bits, hipStream_t s,
+                            bool* result_in_alt) {
+  *result_in_alt = false;
+  if (n <= 1) return hipSuccess;
+  switch (radix_ipt(n)) {
+    case 16: return radix_passes<16>(keys, vals, keys_alt, vals_alt, hist, totals, n, bits, s, result_in_alt);
+    case 8: return radix_passes<8>(keys, vals, keys_alt, vals_alt, hist, totals, n, bits, s, result_in_alt);
+    case 4: return radix_passes<4>(keys, vals, keys_alt, vals_alt, hist, totals, n, bits, s, result_in_alt);
+    case 2: return radix_passes<2>(keys, vals, keys_alt, vals_alt, hist, totals, n, bits, s, result_in_alt);
+    default: return radix_passes<1>(keys, vals, keys_alt, vals_alt, hist, totals, n, bits, s, result_in_alt);
+  }
 }
 
 }  // namespace sph

@@ -339,6 +339,22 @@ class Simulation:
              _hp(out["rho"]), _hp(out["keys"]), _hp(out["perm"]))
         return out
 
+    def gauges(self, gx, gy, zcap, halfw: float) -> np.ndarray:
+        """Height gauges (largest fluid z in each band, instance 0)."""
+        gx = np.ascontiguousarray(gx, np.float32)
+        gy = np.ascontiguousarray(gy, np.float32)
+        zc = np.ascontiguousarray(zcap, np.float32)
+        out = np.empty(gx.size, np.float32)
+        call("sph_sim_gauges", self._h, int(gx.size), _hp(gx), _hp(gy), _hp(zc), C.c_float(halfw), _hp(out))
+        return out
+
+    def pressure_at(self, ids) -> np.ndarray:
+        """Pressure of the particles with the given original indices."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        out = np.empty(ids.size, np.float32)
+        call("sph_sim_pressure_at", self._h, int(ids.size), _hp(ids), _hp(out))
+        return out
+
     def kernel_times(self) -> dict[str, float]:
         names = C.create_string_buffer(4096)
         us = (C.c_double * 64)()
