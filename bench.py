@@ -118,7 +118,9 @@ def learned_bench(args) -> int:
     g = sg.Gnn(max_nodes=nmax, max_edges=30 * nmax, train=True, seed=11, **kw)
     before = sg.evaluate(g, held, norm)
     t1 = time.time()
-    losses = sg.train(g, train, norm, iters=1_000_000, lr=3e-4, lr_final=3e-5, seconds=args.train_seconds)
+    losses = sg.train(g, train, norm, iters=1_000_000, lr=3e-4, lr_final=3e-5, seconds=args.train_seconds,
+                      progress=lambda it, l: print(f"# training: {it} iterations, loss {l:.3f}", file=sys.stderr,
+                                                   flush=True))
     train_s = time.time() - t1
     after = sg.evaluate(g, held, norm)
     print(f"# trained {len(losses)} iterations in {train_s:.0f} s: held-out mse {before['network']:.3f} -> "

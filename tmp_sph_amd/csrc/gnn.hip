@@ -74,7 +74,7 @@ struct sph_gnn {
   float *dY = nullptr, *dV = nullptr, *dE = nullptr, *dX = nullptr, *dZf[2] = {nullptr, nullptr};
   bf16 *dZb[2] = {nullptr, nullptr}, *Tz = nullptr, *Tx = nullptr;
   float *part = nullptr, *colpart = nullptr, *scal = nullptr;
-  int split_cap = 64;
+  int split_cap = 512;  // K chunks of the weight-gradient GEMMs (enough blocks to fill the GPU)
 };
 
 #define SPH_HIP(call, where)                            \
@@ -377,7 +377,7 @@ hipError_t weight_grad(sph_gnn* g, const float* dZf, const bf16* dZb, const AOpe
   // transposes with the reduction (row) dimension padded with zeros to Mp
   HIPCHK(transpose_to_bf16(op1(segb(dZb, W, W)), M, W, g->Tz, Mp, g->stream));
   HIPCHK(transpose_to_bf16(X, M, l.K, g->Tx, Mp, g->stream));
-  int ks = Mp / 2048;
+  int ks = Mp / 1024;
   if (ks < 1) ks = 1;
   if (ks > g->split_cap) ks = g->split_cap;
   GemmArgs a{};
@@ -1076,11 +1076,11 @@ extern "C" int sph_gnn_zero_grad(sph_gnn* g) {
 
 extern "C" int sph_gnn_adam(sph_gnn* g, float lr, float clip) {
   if (!g) return set_error(SPH_E_INVAL, "null argument");
+  if (!g->cfg.train) return set_error(SPH_E_STATE, "sph_gnn_adam: needs train = 1");
   hipStream_t s = g->stream;
   float scale = 1.0f;
   if (clip > 0.0f) {
-    SPH_HIP(hipMemsetAsync(g->scal + 2, 0, sizeof(float), s), "clip");
-    SPH_HIP(sum_squares(g->G, (long long)g->nparam, g->scal + 2, s), "clip");
+    SPH_HIP(sum_squares(g->G, (long long)g->nparam, g->scal + 2, g->colpart, s), "clip");
     float sq = 0.0f;
     SPH_HIP(hipMemcpyAsync(&sq, g->scal + 2, sizeof(float), hipMemcpyDeviceToHost, s), "clip");
     SPH_HIP(hipStreamSynchronize(s), "clip");

@@ -72,7 +72,8 @@ hipError_t pack_weights(const float* W, int N, int K, bf16* Wb, bf16* Wt, hipStr
 hipError_t adam(float* w, const float* g, float* m, float* v, long long count, float lr, float b1, float b2,
                 float eps, int t, float clip_scale, hipStream_t s);
 // sum of squares of g into *out (FP32 atomics; used for gradient-norm clipping)
-hipError_t sum_squares(const float* g, long long count, float* out, hipStream_t s);
+// *out = sum of g[i]^2 in a fixed order; scratch holds 1024 floats.
+hipError_t sum_squares(const float* g, long long count, float* out, float* scratch, hipStream_t s);
 
 // MFMA operand-layout probe: D(16x16) = A(16x32) * B(32x16), one wave.
 hipError_t mfma_tile(const bf16* A, const bf16* B, float* D, hipStream_t s);

@@ -349,13 +349,32 @@ __global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g, 
   }
 }
 
-__global__ void sum_squares_kernel(const float* __restrict__ g, long long count, float* __restrict__ out) {
+// Sum of squares in a fixed order (block partials, then one block), so the
+// gradient-clipping factor -- and with it training -- is reproducible.
+constexpr int SQ_BLOCKS = 1024;
+
+__global__ __launch_bounds__(256) void sum_squares_partial_kernel(const float* __restrict__ g, long long count,
+                                                                  float* __restrict__ part) {
+  __shared__ float red[4];
   float s = 0.0f;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
        i += (long long)gridDim.x * blockDim.x)
     s += g[i] * g[i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(256) void sum_squares_final_kernel(const float* __restrict__ part, int nparts,
+                                                                float* __restrict__ out) {
+  __shared__ float red[4];
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 __global__ void mfma_tile_kernel(const bf16* __restrict__ A, const bf16* __restrict__ B, float* __restrict__ D) {
@@ -457,8 +476,9 @@ hipError_t adam(float* w, const float* g, float* m, float* v, long long count, f
   return hipGetLastError();
 }
 
-hipError_t sum_squares(const float* g, long long count, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(sum_squares_kernel, dim3(grid_for(count, 256, 1024)), dim3(256), 0, s, g, count, out);
+hipError_t sum_squares(const float* g, long long count, float* out, float* scratch, hipStream_t s) {
+  hipLaunchKernelGGL(sum_squares_partial_kernel, dim3(SQ_BLOCKS), dim3(256), 0, s, g, count, scratch);
+  hipLaunchKernelGGL(sum_squares_final_kernel, dim3(1), dim3(256), 0, s, scratch, SQ_BLOCKS, out);
   return hipGetLastError();
 }
 

@@ -56,10 +56,16 @@ struct RowRange {
 // 3-D about a quarter of the candidates go.  Box distances are shrunk by
 // gridSize/1024, far above the rounding of the cell bounds, and nothing is
 // culled for a target outside the grid (its cell was clamped).
+//
+// With fluidcells (not null), each row is also trimmed to its first..last
+// cell that holds a fluid particle: kernel A only pairs with fluid
+// neighbours, so the cells cut away hold no pair it accepts, and a wall target
+// with no fluid in its stencil skips its scan altogether.
 template <int DIM>
 __device__ __forceinline__ void row_ranges(const sph_params& P, int coff, int key, const float4& p,
                                            const int* __restrict__ cellStart,
                                            const int* __restrict__ cellEnd,
+                                           const int* __restrict__ fluidcells,
                                            RowRange (&rows)[DIM == 3 ? 9 : 3]) {
   const int X = P.gridxdim, Y = P.gridydim, Z = P.gridzdim;
   const int XY = X * Y;
@@ -97,7 +103,7 @@ __device__ __forceinline__ void row_ranges(const sph_params& P, int coff, int ke
         s = -1;
         for (int c = rowBase + x0; c <= rowBase + x1; ++c) {
           const int st = cellStart[c];
-          if (st != -1) {
+          if (st != -1 && (!fluidcells || fluidcells[c])) {
             if (s < 0) s = st;
             e = cellEnd[c];
           }

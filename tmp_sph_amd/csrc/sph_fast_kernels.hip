@@ -47,7 +47,7 @@ __global__ void hash_fast_kernel(InstTable T, const float4* __restrict__ PR, con
 template <int DIM>
 __global__ void cells_gather_kernel(FastSet in, FastSet out, const int* __restrict__ keys,
                                     const int* __restrict__ idx, int* __restrict__ cellStart,
-                                    int* __restrict__ cellEnd, int n) {
+                                    int* __restrict__ cellEnd, int* __restrict__ cellFluid, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int h = keys[i];
@@ -62,7 +62,9 @@ __global__ void cells_gather_kernel(FastSet in, FastSet out, const int* __restri
   }
   if (i == n - 1) cellEnd[h] = n;
   const int src = idx[i];
-  out.PR[i] = in.PR[src];
+  const float4 pr = in.PR[src];
+  out.PR[i] = pr;
+  if (pr.w > 0.0f) cellFluid[h] = 1;  // fluid particle (every writer stores the same value)
   out.VP[i] = in.VP[src];
   out.X[0][i] = in.X[0][src];
   if (DIM == 3) out.X[1][i] = in.X[1][src];
@@ -195,17 +197,20 @@ __global__ void download_kernel(FastSet st, float* __restrict__ pos, float* __re
 template <int DIM>
 __global__ void pair_stats_kernel(InstTable T, const float4* __restrict__ PR, const int* __restrict__ TY,
                                   const int* __restrict__ keys, const int* __restrict__ cellStart,
-                                  const int* __restrict__ cellEnd, unsigned long long* __restrict__ out,
-                                  int n) {
+                                  const int* __restrict__ cellEnd, const int* __restrict__ cellFluid,
+                                  unsigned long long* __restrict__ out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long c = 0, hA = 0, hB = 0;
+  unsigned long long c = 0, cA = 0, hA = 0, hB = 0;
   bool fi = false;
   if (i < n) {
     const float4 pi = PR[i];
     fi = pi.w > 0.0f;
     with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
-      RowRange rows[DIM == 3 ? 9 : 3];
-      row_ranges<DIM>(P, coff, keys[i], pi, cellStart, cellEnd, rows);
+      RowRange rows[DIM == 3 ? 9 : 3], rowsA[DIM == 3 ? 9 : 3];
+      row_ranges<DIM>(P, coff, keys[i], pi, cellStart, cellEnd, nullptr, rows);
+      row_ranges<DIM>(P, coff, keys[i], pi, cellStart, cellEnd, cellFluid, rowsA);
+      for (int r = 0; r < (DIM == 3 ? 9 : 3); ++r)
+        for (int j = rowsA[r].s; j < rowsA[r].e; ++j) cA += j != i;
       for (int r = 0; r < (DIM == 3 ? 9 : 3); ++r)
         for (int j = rows[r].s; j < rows[r].e; ++j) {
           if (j == i) continue;
@@ -220,7 +225,7 @@ __global__ void pair_stats_kernel(InstTable T, const float4* __restrict__ PR, co
         }
     });
   }
-  unsigned long long v[4] = {c, hA, fi ? c : 0ull, fi ? hB : 0ull};
+  unsigned long long v[4] = {cA, hA, fi ? c : 0ull, fi ? hB : 0ull};
   for (int k = 0; k < 4; ++k) {
     unsigned long long x = v[k];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
@@ -301,11 +306,13 @@ hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, in
 }
 
 hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
-                                    const int* idx, int* cellStart, int* cellEnd, int n, hipStream_t s) {
+                                    const int* idx, int* cellStart, int* cellEnd, int* cellFluid, int n,
+                                    hipStream_t s) {
   hipError_t e = fill_int(cellStart, -1, T.total_cells, s);
+  if (e == hipSuccess) e = fill_int(cellFluid, 0, T.total_cells, s);
   if (e != hipSuccess || n <= 0) return e;
   return launch_dim(T.dim, cells_gather_kernel<2>, cells_gather_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), s,
-                    in, out, keys, idx, cellStart, cellEnd, n);
+                    in, out, keys, idx, cellStart, cellEnd, cellFluid, n);
 }
 
 hipError_t launch_pc_fast(const InstTable& T, bool predictor, const FastSet& st, const FastWork& w,
@@ -345,7 +352,7 @@ hipError_t launch_pair_stats(const InstTable& T, const FastSet& st, const FastWo
   if (n <= 0) return hipSuccess;
   return launch_dim(T.dim, pair_stats_kernel<2>, pair_stats_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), s, T,
                     (const float4*)st.PR, (const int*)st.TY, (const int*)w.keys, (const int*)w.cellStart,
-                    (const int*)w.cellEnd, out4, n);
+                    (const int*)w.cellEnd, (const int*)w.cellFluid, out4, n);
 }
 
 // Height gauges (the reference's measuring bands, src/io/io.cu:236-255):

@@ -65,6 +65,7 @@ struct FastWork {
   float4* RATE;    // (drho/dt, ax, ay, az)
   double* TX[3];   // positions at the start of the step (predictor slot order)
   float4* TV;      // (vx, vy, vz, rho) at the start of the step
+  int* cellFluid;  // 1 for cells holding a fluid particle (rebuilt with the cell ranges)
   int* NL;         // neighbour lists, slot-major: NL[k * n + i] (k < nl_cap)
   int* NC;         // neighbour count of each target (> nl_cap: list overflowed)
   int nl_cap;
@@ -87,7 +88,8 @@ struct InstTable {
 
 hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s);
 hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
-                                    const int* idx, int* cellStart, int* cellEnd, int n, hipStream_t s);
+                                    const int* idx, int* cellStart, int* cellEnd, int* cellFluid, int n,
+                                    hipStream_t s);
 hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s);
 // Neighbour-list mode: one scan records each target's in-support neighbours

@@ -397,8 +397,9 @@ struct Cfg {
 template <int DIM, int U, bool STAGED, bool COMPACT, bool PF, class Body>
 __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key, const float4& p, bool active,
                                            const float4* PR,
-                                           const int* __restrict__ cs, const int* __restrict__ ce, float4* tile,
-                                           int* lists, RowRange* slots, Body&& body) {
+                                           const int* __restrict__ cs, const int* __restrict__ ce,
+                                           const int* __restrict__ fc, float4* tile, int* lists, RowRange* slots,
+                                           Body&& body) {
   constexpr int NR = nrows<DIM>();
   int* list = lists + (threadIdx.x >> 6) * COMPACT_CAP * 64;
   auto global_path = [&](const sph_params& P, int coff) {
@@ -406,14 +407,14 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
       // row ranges parked in the thread's own LDS slots (no barrier needed):
       // keeps 18 VGPRs free in 3-D, which buys occupancy for the plain loop
       RowRange rows[NR];
-      row_ranges<DIM>(P, coff, key, p, cs, ce, rows);
+      row_ranges<DIM>(P, coff, key, p, cs, ce, fc, rows);
       const int t = threadIdx.x;
 #pragma unroll
       for (int r = 0; r < NR; ++r) slots[r * 256 + t] = rows[r];
       body(P, PlainVisit<NR, U, PF, LdsRows, GlobalSrc>{LdsRows{slots + t, 256}, GlobalSrc{PR}});
     } else {
       RegRows<NR> rows;
-      row_ranges<DIM>(P, coff, key, p, cs, ce, rows.v);
+      row_ranges<DIM>(P, coff, key, p, cs, ce, fc, rows.v);
       if constexpr (COMPACT)
         body(P, CompactVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}, list, PR});
       else
@@ -427,7 +428,7 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
     const sph_params& P = T.P[u];
     const int coff = T.cell_off[u];
     RegRows<NR> rows;
-    row_ranges<DIM>(P, coff, key, p, cs, ce, rows.v);
+    row_ranges<DIM>(P, coff, key, p, cs, ce, fc, rows.v);
     StagedSrc<NR> src;
     const bool staged = stage_rows<DIM>(P, coff, key, same_inst, rows.v, PR, tile, src);
     if (!active) return;
@@ -450,7 +451,8 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, flo
                                                                 const int* __restrict__ TY, float* __restrict__ DV,
                                                                 const int* __restrict__ keys,
                                                                 const int* __restrict__ cs,
-                                                                const int* __restrict__ ce, int n, int remap) {
+                                                                const int* __restrict__ ce,
+                                                                const int* __restrict__ fc, int n, int remap) {
   using C = Cfg<DIM, U, STAGED, COMPACT>;
   __shared__ float4 tile[C::TILE];
   __shared__ int lists[C::LIST];
@@ -462,7 +464,7 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, flo
   const float4 pi = PR[ic];
   const float4 vi = VP[ic];
   const int key = keys[ic];
-  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid, PR, cs, ce, tile, lists, slots,
+  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid, PR, cs, ce, fc, tile, lists, slots,
                                       [&](const sph_params& P, auto visit) {
                                         nsA_target<DIM>(P, i, pi, vi, visit, PR, VP, DV);
                                       });
@@ -492,7 +494,8 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, con
   const float4 vi = VP[ic];
   const float dvi = DV[ic];
   const int key = keys[ic];
-  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid && fluid, PR, cs, ce, tile, lists, slots,
+  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid && fluid, PR, cs, ce, nullptr, tile,
+                                          lists, slots,
                                       [&](const sph_params& P, auto visit) {
                                         nsB_target<DIM>(P, i, pi, vi, dvi, visit, VP, DV, RATE);
                                       });
@@ -516,7 +519,7 @@ __global__ __launch_bounds__(256) void nbrlist_kernel(InstTable T, const float4*
   int cnt = 0;
   with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
     RowRange rows[NR];
-    row_ranges<DIM>(P, coff, key, pi, cs, ce, rows);
+    row_ranges<DIM>(P, coff, key, pi, cs, ce, nullptr, rows);
     const int t = threadIdx.x;
 #pragma unroll
     for (int r = 0; r < NR; ++r) slots[r * 256 + t] = rows[r];
@@ -541,8 +544,8 @@ template <int DIM>
 __global__ __launch_bounds__(256) void nsA_list_kernel(InstTable T, float4* PR, float4* VP, const int* __restrict__ TY,
                                                        float* __restrict__ DV, const int* __restrict__ keys,
                                                        const int* __restrict__ cs, const int* __restrict__ ce,
-                                                       const int* __restrict__ NL, const int* __restrict__ NC,
-                                                       int cap, int n, int remap) {
+                                                       const int* __restrict__ fc, const int* __restrict__ NL,
+                                                       const int* __restrict__ NC, int cap, int n, int remap) {
   constexpr int NR = nrows<DIM>();
   const int i = tile_id(remap) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -555,7 +558,7 @@ __global__ __launch_bounds__(256) void nsA_list_kernel(InstTable T, float4* PR, 
       nsA_target<DIM>(P, i, pi, vi, ListVisit{NL, n, i, cnt, PR}, PR, VP, DV);
     } else {
       RegRows<NR> rows;
-      row_ranges<DIM>(P, coff, key, pi, cs, ce, rows.v);
+      row_ranges<DIM>(P, coff, key, pi, cs, ce, fc, rows.v);
       nsA_target<DIM>(P, i, pi, vi, PlainVisit<NR, 2, false, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}}, PR, VP,
                       DV);
     }
@@ -586,7 +589,7 @@ __global__ __launch_bounds__(256) void nsB_list_kernel(InstTable T, const float4
       nsB_target<DIM>(P, i, pi, vi, dvi, ListVisit{NL, n, i, cnt, PR}, VP, DV, RATE);
     } else {
       RegRows<NR> rows;
-      row_ranges<DIM>(P, coff, key, pi, cs, ce, rows.v);
+      row_ranges<DIM>(P, coff, key, pi, cs, ce, nullptr, rows.v);
       nsB_target<DIM>(P, i, pi, vi, dvi, PlainVisit<NR, 2, false, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}}, VP,
                       DV, RATE);
     }
@@ -601,10 +604,12 @@ hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n
   const dim3 g(blocks_for(n, b)), bl(b);
   if (T.dim == 3)
     hipLaunchKernelGGL((nsA_kernel<3, U, STAGED, COMPACT, PF>), g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
-                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, n, remap);
+                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.cellFluid,
+                       n, remap);
   else
     hipLaunchKernelGGL((nsA_kernel<2, U, STAGED, COMPACT, PF>), g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
-                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, n, remap);
+                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.cellFluid,
+                       n, remap);
   return hipGetLastError();
 }
 
@@ -695,12 +700,12 @@ hipError_t nsA_list(const InstTable& T, const FastSet& st, const FastWork& w, in
   const dim3 g(blocks_for(n, block)), bl(block);
   if (T.dim == 3)
     hipLaunchKernelGGL(nsA_list_kernel<3>, g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV, (const int*)w.keys,
-                       (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.NL, (const int*)w.NC, w.nl_cap,
-                       n, remap);
+                       (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.cellFluid, (const int*)w.NL,
+                       (const int*)w.NC, w.nl_cap, n, remap);
   else
     hipLaunchKernelGGL(nsA_list_kernel<2>, g, bl, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV, (const int*)w.keys,
-                       (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.NL, (const int*)w.NC, w.nl_cap,
-                       n, remap);
+                       (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.cellFluid, (const int*)w.NL,
+                       (const int*)w.NC, w.nl_cap, n, remap);
   return hipGetLastError();
 }
 

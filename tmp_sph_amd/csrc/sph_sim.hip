@@ -10,40 +10,10 @@
 #include <string>
 #include <vector>
 
-#include "sph_kernels.h"
+#include "sph_sim_ctx.h"
 
 using namespace sph;
 
-struct sph_sim {
-  std::vector<sph_params> P;  // per instance (host copy)
-  std::vector<int> cell_off;  // per instance
-  InstTable T{};
-  int K = 1;
-  int dim = 3;
-  int n = 0;
-  int bits = 1;
-  hipStream_t stream = nullptr;
-  FastSet set[2]{};
-  FastWork w{};
-  sph_tuning tun{};
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  long long steps = 0;
-  unsigned long long* probe = nullptr;  // 2K words (also 4 for pair statistics)
-  unsigned* gbuf = nullptr;             // gauges / point probes: 5 x SPH_MAX_PROBES words
-  std::vector<void*> allocs;
-  // kernel timing (eager steps with tuning.flags bit 0)
-  std::vector<std::string> tnames;
-  std::vector<double> tsum_ms;
-  std::vector<long long> tcount;
-  struct Mark {
-    int slot;
-    hipEvent_t a, b;
-  };
-  std::vector<Mark> marks;
-  std::vector<hipEvent_t> event_pool;
-  size_t event_next = 0;
-};
 
 #define SPH_HIP(call, where)                            \
   do {                                                  \
@@ -113,7 +83,8 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
     w.idx_alt = s->w.idx;
   }
   e = timed(s, timing, "cells_gather", [&] {
-    return launch_cells_gather_fast(s->T, s->set[from], s->set[to], w.keys, w.idx, w.cellStart, w.cellEnd, n,
+    return launch_cells_gather_fast(s->T, s->set[from], s->set[to], w.keys, w.idx, w.cellStart, w.cellEnd,
+                                    w.cellFluid, n,
                                      s->stream);
   });
   *out = w;
@@ -155,6 +126,7 @@ void drop_graph(sph_sim* s) {
 }
 
 void release(sph_sim* s) {
+  output_release(s);
   drop_graph(s);
   for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
   for (void* p : s->allocs) hipFree(p);
@@ -243,6 +215,7 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   rc = rc ? rc : dalloc(s, &w.totals, 256);
   rc = rc ? rc : dalloc(s, &w.cellStart, (size_t)ctot);
   rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)ctot);
+  rc = rc ? rc : dalloc(s, &w.cellFluid, (size_t)ctot);
   rc = rc ? rc : dalloc(s, &w.DV, N + CAND_PAD);
   w.nl_cap = d3 ? NL_CAP_3D : NL_CAP_2D;
   rc = rc ? rc : dalloc(s, &w.NL, N * (size_t)w.nl_cap);

@@ -15,6 +15,7 @@ Cases: :func:`dambreak_cfg` builds the lattice dam-break configurations and
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -253,6 +254,52 @@ class Simulation:
              _hp(cat("pos", np.float32)), _hp(cat("vel", np.float32)), _hp(cat("rho", np.float32)),
              _hp(cat("p", np.float32)), _hp(cat("type", np.int32)), C.byref(h))
         self._h = h
+
+    @classmethod
+    def restore(cls, path) -> "Simulation":
+        """A context continuing bit-identically from a :meth:`checkpoint` file."""
+        load()
+        h = C.c_void_p()
+        call("sph_sim_restore", os.fsencode(os.fspath(path)), C.byref(h))
+        self = cls.__new__(cls)
+        self._h = h
+        K = C.c_int()
+        call("sph_sim_num_instances", h, C.byref(K))
+        self.K = K.value
+        self.cases_params = []
+        for k in range(self.K):
+            p = SphParams()
+            call("sph_sim_instance_params", h, k, C.byref(p))
+            self.cases_params.append(p)
+        self.params = self.cases_params[0]
+        counts = np.array([p.numParticles for p in self.cases_params], np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.n = int(counts.sum())
+        self.n_fluid = int(np.count_nonzero(self.state()["type"] == 1))
+        return self
+
+    def checkpoint(self, path) -> None:
+        """Writes the complete state (sorted order, FP64 positions, step count)."""
+        call("sph_sim_checkpoint", self._h, os.fsencode(os.fspath(path)))
+
+    def output_open(self, directory, binary: bool = True, rho: bool = False, type: bool = False) -> None:
+        """Starts the asynchronous frame writer (files directory/fluid_t=<frame>.vtk)."""
+        os.makedirs(directory, exist_ok=True)
+        call("sph_sim_output_open", self._h, os.fsencode(os.fspath(directory)), int(bool(binary)),
+             int(bool(rho)) | (int(bool(type)) << 1))
+
+    def output_frame(self, frame: int) -> None:
+        """Queues the current state as frame ``frame``; returns before it is written."""
+        call("sph_sim_output_frame", self._h, int(frame))
+
+    def output_flush(self) -> int:
+        """Waits for every queued frame; returns the number written so far."""
+        n = C.c_longlong()
+        call("sph_sim_output_flush", self._h, C.byref(n))
+        return n.value
+
+    def output_close(self) -> None:
+        call("sph_sim_output_close", self._h)
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:

@@ -242,14 +242,15 @@ def normalised_target(s: Snapshot, norm: dict) -> np.ndarray:
 
 
 def train(g: "Gnn", snaps: list, norm: dict, iters: int, lr: float = 3e-4, lr_final: float = 3e-5,
-          clip: float = 1.0, seed: int = 0, seconds: float | None = None) -> list:
+          clip: float = 1.0, seed: int = 0, seconds: float | None = None, progress=None) -> list:
     """Adam on single-snapshot batches; cosine learning-rate decay over the
     iterations or, with a time budget, over the elapsed fraction of it.  Stops
-    after `iters` steps or `seconds` of wall-clock, whichever comes first."""
+    after `iters` steps or `seconds` of wall-clock, whichever comes first;
+    progress(iterations, recent mean loss) is called every 30 s."""
     import time
     rng = np.random.default_rng(seed)
     losses = []
-    t0 = time.time()
+    t0 = t_last = time.time()
     for it in range(iters):
         s = snaps[rng.integers(len(snaps))]
         g.load(s.params, s.pos, s.hist, s.rho, s.type, normalised_target(s, norm))
@@ -261,6 +262,9 @@ def train(g: "Gnn", snaps: list, norm: dict, iters: int, lr: float = 3e-4, lr_fi
             frac = max(frac, min((time.time() - t0) / seconds, 1.0))
         g.adam(lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac)), clip)
         losses.append(loss)
+        if progress is not None and time.time() - t_last > 30.0:
+            t_last = time.time()
+            progress(it + 1, float(np.mean(losses[-50:])))
         if seconds is not None and time.time() - t0 > seconds:
             break
     return losses
