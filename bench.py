@@ -84,6 +84,45 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+def output_bench(sim, every: int, frames: int, ms_plain: float) -> dict:
+    """Wall time of `frames` x (`every` replayed steps + one binary VTK frame):
+    with the asynchronous writer (the frame's copy and file write overlap the
+    next steps) and with a blocking download followed by the same write."""
+    import shutil
+    import tempfile
+    from tmp_sph_amd import files
+    tmp = Path(tempfile.mkdtemp(prefix="sph_frames_"))
+    try:
+        sim.output_open(tmp / "async", binary=True)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            sim.replay(every)
+            sim.output_frame(f)
+        sim.output_flush()
+        sim.synchronize()
+        t_async = time.perf_counter() - t0
+        sim.output_close()
+        (tmp / "sync").mkdir()
+        t0 = time.perf_counter()
+        for f in range(frames):
+            sim.replay(every)
+            st = sim.state()
+            files.write_vtk(tmp / "sync" / f"fluid_t={f}.vtk", st["pos"], st["vel"], st["p"], binary=True)
+        sim.synchronize()
+        t_sync = time.perf_counter() - t0
+        frame_bytes = (tmp / "sync" / "fluid_t=0.vtk").stat().st_size
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    steps = every * frames
+    res = {"every": every, "frames": frames, "frame_bytes": frame_bytes,
+           "ms_per_step_no_output": ms_plain,
+           "ms_per_step_async": 1000.0 * t_async / steps, "ms_per_step_blocking": 1000.0 * t_sync / steps}
+    print(f"# output every {every} steps: {res['ms_per_step_async']:.3f} ms/step asynchronous, "
+          f"{res['ms_per_step_blocking']:.3f} blocking, {ms_plain:.3f} without frames", file=sys.stderr, flush=True)
+    return res
+
+
 def learned_bench(args) -> int:
     """Surrogate against the plain GPU solver on a held-out 2-D dam break.
 
@@ -187,6 +226,9 @@ def main() -> int:
     ap.add_argument("--nbr-compact", type=int, default=-1)
     ap.add_argument("--nbr-prefetch", type=int, default=-1)
     ap.add_argument("--nbr-list", type=int, default=-1)
+    ap.add_argument("--output-every", type=int, default=0, metavar="M",
+                    help="also time VTK frame output every M steps: asynchronous writer vs download-and-write")
+    ap.add_argument("--output-frames", type=int, default=4)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device-to-device copy of MB MiB 5 times (scales HBM byte counters) and exit")
     args = ap.parse_args()
@@ -231,6 +273,8 @@ def main() -> int:
     ms_per_step = 1000.0 * elapsed / args.steps
     value = n_fluid * args.steps / elapsed
     print(f"# timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    output = output_bench(sim, args.output_every, args.output_frames, ms_per_step) if args.output_every > 0 else None
 
     # per-kernel device time (HIP events on the simulation stream, eager steps)
     sim.set_tuning(flags=1)
@@ -278,6 +322,8 @@ def main() -> int:
         "pair_stats": stats,
         "probe_after_run": {"front_m": front, "column_height_m": height},
     }
+    if output:
+        out["output"] = output
     if not args.no_cpu_baseline and len(cases) == 1:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -38,6 +38,7 @@ int set_error(int code, const char* msg) {
 
 int hip_status(int hip_err, const char* where) {
   if (hip_err == hipSuccess) return SPH_OK;
+  (void)hipGetLastError();  // clear it, so a later launch check does not report it again
   std::string m = std::string(where) + ": " + hipGetErrorString((hipError_t)hip_err);
   g_last_error = m;
   return hip_err > 0 ? hip_err : 1000 - hip_err;

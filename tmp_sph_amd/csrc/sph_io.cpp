@@ -9,6 +9,7 @@
 //    optional fields the files are byte-identical to the reference's; the
 //    optional density and type arrays are written as extra SCALARS sections
 //    between them.
+#include <algorithm>
 #include <cerrno>
 #include <cstdint>
 #include <cstdio>
@@ -75,12 +76,22 @@ int write_values(const std::string& path, const T* v, long long count, int per_l
   return ok ? SPH_OK : set_error(SPH_E_INVAL, "write failed: " + path);
 }
 
-void put_be32(FILE* f, const void* p) {
-  uint32_t u;
-  std::memcpy(&u, p, 4);
-  const unsigned char b[4] = {(unsigned char)(u >> 24), (unsigned char)(u >> 16), (unsigned char)(u >> 8),
-                              (unsigned char)u};
-  std::fwrite(b, 1, 4, f);
+// Writes `count` 32-bit words as big-endian bytes, in chunks.
+void put_be32(FILE* f, const void* src, size_t count) {
+  const unsigned char* p = (const unsigned char*)src;
+  unsigned char buf[1 << 16];
+  while (count) {
+    const size_t k = std::min(count, sizeof(buf) / 4);
+    for (size_t i = 0; i < k; ++i) {
+      buf[4 * i + 0] = p[4 * i + 3];
+      buf[4 * i + 1] = p[4 * i + 2];
+      buf[4 * i + 2] = p[4 * i + 1];
+      buf[4 * i + 3] = p[4 * i + 0];
+    }
+    std::fwrite(buf, 1, 4 * k, f);
+    p += 4 * k;
+    count -= k;
+  }
 }
 
 }  // namespace
@@ -139,21 +150,17 @@ extern "C" int sph_write_vtk(const char* path, int n, const float* pos, const fl
   // precision 6); binary values as big-endian 32-bit words with no separator
   // before the next keyword line, as the reference writes them.
   auto vec3 = [&](const float* v) {
-    for (int i = 0; i < n; ++i) {
-      if (binary) {
-        for (int c = 0; c < 3; ++c) put_be32(f, &v[3 * i + c]);
-      } else {
+    if (binary)
+      put_be32(f, v, 3 * (size_t)n);
+    else
+      for (int i = 0; i < n; ++i)
         std::fprintf(f, "%.6e %.6e %.6e\n", (double)v[3 * i], (double)v[3 * i + 1], (double)v[3 * i + 2]);
-      }
-    }
   };
   auto scal = [&](const float* v) {
-    for (int i = 0; i < n; ++i) {
-      if (binary)
-        put_be32(f, &v[i]);
-      else
-        std::fprintf(f, "%.6e\n", (double)v[i]);
-    }
+    if (binary)
+      put_be32(f, v, (size_t)n);
+    else
+      for (int i = 0; i < n; ++i) std::fprintf(f, "%.6e\n", (double)v[i]);
   };
   vec3(pos);
   std::fprintf(f, "POINT_DATA %d\n", n);
@@ -165,12 +172,10 @@ extern "C" int sph_write_vtk(const char* path, int n, const float* pos, const fl
   }
   if (type) {
     std::fprintf(f, "SCALARS type int 1\nLOOKUP_TABLE DEFAULT\n");
-    for (int i = 0; i < n; ++i) {
-      if (binary)
-        put_be32(f, &type[i]);
-      else
-        std::fprintf(f, "%d\n", type[i]);
-    }
+    if (binary)
+      put_be32(f, type, (size_t)n);
+    else
+      for (int i = 0; i < n; ++i) std::fprintf(f, "%d\n", type[i]);
   }
   std::fprintf(f, "VECTORS velocity %s\n", ty);
   vec3(vel);

@@ -50,15 +50,16 @@ struct Fnv {
   }
 };
 
-// The particle arrays of the current set, in sorted-slot order.
+// The particle arrays of the current set, in sorted-slot order (2-D
+// contexts keep no y coordinates).
 struct Arrays {
   std::vector<float> PR, VP;  // 4 per particle
   std::vector<double> X[3];
   std::vector<int> ID, TY;
-  void resize(size_t n) {
+  void resize(size_t n, int dim) {
     PR.resize(4 * n);
     VP.resize(4 * n);
-    for (auto& x : X) x.resize(n);
+    for (int c = 0; c < 3; ++c) X[c].resize(c == 1 && dim == 2 ? 0 : n);
     ID.resize(n);
     TY.resize(n);
   }
@@ -79,7 +80,8 @@ int copy_set(const FastSet& st, Arrays& a, size_t n, hipMemcpyKind kind) {
   };
   hipError_t e = cp(st.PR, a.PR.data(), 16 * n);
   if (e == hipSuccess) e = cp(st.VP, a.VP.data(), 16 * n);
-  for (int c = 0; c < 3 && e == hipSuccess; ++c) e = cp(st.X[c], a.X[c].data(), 8 * n);
+  for (int c = 0; c < 3 && e == hipSuccess; ++c)
+    if (!a.X[c].empty()) e = cp(st.X[c], a.X[c].data(), 8 * n);
   if (e == hipSuccess) e = cp(st.ID, a.ID.data(), 4 * n);
   if (e == hipSuccess) e = cp(st.TY, a.TY.data(), 4 * n);
   return e == hipSuccess ? SPH_OK : hip_status(e, "checkpoint copy");
@@ -182,7 +184,7 @@ extern "C" int sph_sim_checkpoint(sph_sim* s, const char* path) {
   if (!s || !path) return set_error(SPH_E_INVAL, "sph_sim_checkpoint: null argument");
   const size_t n = (size_t)s->n;
   Arrays a;
-  a.resize(n);
+  a.resize(n, s->dim);
   SPH_HIP(hipStreamSynchronize(s->stream), "sph_sim_checkpoint");
   int rc = copy_set(s->set[0], a, n, hipMemcpyDeviceToHost);
   if (rc) return rc;
@@ -233,7 +235,7 @@ extern "C" int sph_sim_restore(const char* path, sph_sim** out) {
   if (total != h.n) return fail("instance sizes do not add up");
   const size_t n = (size_t)h.n;
   Arrays a;
-  a.resize(n);
+  a.resize(n, h.dim);
   bool ok = true;
   a.each([&](void* p, size_t len) { ok = ok && std::fread(p, 1, len, f) == len; });
   uint64_t want = 0;
@@ -254,7 +256,7 @@ extern "C" int sph_sim_restore(const char* path, sph_sim** out) {
   std::vector<float> pos(3 * n), vel(3 * n), rho(n), prs(n);
   std::vector<int> type(n);
   for (size_t i = 0; i < n; ++i) {
-    for (int c = 0; c < 3; ++c) pos[3 * i + c] = (float)a.X[c][i];
+    for (int c = 0; c < 3; ++c) pos[3 * i + c] = a.X[c].empty() ? 0.0f : (float)a.X[c][i];
     type[i] = a.TY[i] & 255;
     rho[i] = 1.0f;
   }
