@@ -123,6 +123,79 @@ def output_bench(sim, every: int, frames: int, ms_plain: float) -> dict:
     return res
 
 
+def surrogate_bench(args) -> int:
+    """Throughput of the surrogate network's BF16 MFMA kernels on the graph of
+    the learned benchmark's evaluation case (0.8 x 0.5 m column, dx = 0.002
+    m: 109 k particles, 2.1 M edges): inference forward with the fused MLP
+    kernel and with three GEMM launches per MLP, and one training step
+    (forward, backward, Adam) on the dx = 0.004 m version (29.5 k particles)."""
+    import numpy as np
+    from tmp_sph_amd import sph
+    from tmp_sph_amd import surrogate as sg
+    layers = 10
+    reps = max(1, args.steps)
+
+    def timed(f, n):
+        f()
+        sph.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            f()
+        sph.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.002, tank=(4.0, 1.0, 1.0), column=(0.8, 1.0, 0.5)))
+    hist = (np.random.default_rng(0).standard_normal((5, case.n, 3)) * 0.1).astype(np.float32)
+    g = sg.Gnn(layers=layers, max_nodes=case.n, max_edges=30 * case.n, train=False)
+    g.load(case.params, case.pos, hist, case.rho, case.type)
+    _, send, _ = g.graph()
+    ne = int(send.size)
+    flops = sg.forward_flops(layers, case.n, ne)
+    res = {}
+    for depth in (4, 2, 1, 0):
+        g.set_fused(depth)
+        dt = timed(lambda: g.forward(want_out=False), reps)
+        name = f"fused_depth{depth}" if depth else "three_gemms"
+        res[name] = {"ms": 1e3 * dt, "tflops": flops / dt / 1e12}
+        print(f"# forward ({name}): {1e3 * dt:.2f} ms, {flops / dt / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
+    g.close()
+    small = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.004, tank=(4.0, 1.0, 1.0), column=(0.8, 1.0, 0.5)))
+    h2 = (np.random.default_rng(1).standard_normal((5, small.n, 3)) * 0.1).astype(np.float32)
+    tg = np.random.default_rng(2).standard_normal((small.n, 3)).astype(np.float32)
+    gt = sg.Gnn(layers=layers, max_nodes=small.n, max_edges=30 * small.n, train=True)
+    gt.load(small.params, small.pos, h2, small.rho, small.type, tg)
+    _, s2, _ = gt.graph()
+
+    def train_step():
+        gt.zero_grad()
+        gt.forward(want_out=False)
+        gt.backward()
+        gt.adam(1e-5)
+
+    dtt = timed(train_step, max(1, reps // 2))
+    tflops_train = 3 * sg.forward_flops(layers, small.n, int(s2.size)) / dtt / 1e12
+    print(f"# training step: {1e3 * dtt:.2f} ms (~{tflops_train:.1f} TFLOP/s counting 3x forward flops)",
+          file=sys.stderr, flush=True)
+    gt.close()
+    best = res["fused_depth2"]
+    out = {
+        "metric": "surrogate forward throughput (edges x layers / s)",
+        "value": ne * layers / (best["ms"] * 1e-3),
+        "unit": "edge-layers/s",
+        "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": best["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 MFMA, fp32 accumulate",
+        "data": "synthetic: random velocity history, random-init weights",
+        "config": {"workload": "message-passing network, 10 blocks, latent 128, on the 2.1 M-edge graph of a "
+                               "0.8x0.5 m column at dx=0.002 m", "nodes": case.n, "edges": ne},
+        "roofline": {"bound": "mfma", "achieved": best["tflops"], "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": best["tflops"] / BF16_MFMA_PEAK_TFLOPS, "traffic": None},
+        "forward": res,
+        "training_step": {"nodes": small.n, "edges": int(s2.size), "ms": 1e3 * dtt, "tflops_3x_forward": tflops_train},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def learned_bench(args) -> int:
     """Surrogate against the plain GPU solver on a held-out 2-D dam break.
 
@@ -212,7 +285,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned", "surrogate"])
     ap.add_argument("--train-seconds", type=float, default=120.0, help="learned: GPU training budget")
     ap.add_argument("--learned-t", type=float, default=0.25, help="learned: simulated time of the timed rollout")
     ap.add_argument("--no-autotune", action="store_true")
@@ -236,6 +309,8 @@ def main() -> int:
         return copy_probe(args.copy_probe)
     if args.config == "learned":
         return learned_bench(args)
+    if args.config == "surrogate":
+        return surrogate_bench(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
               "the slab decomposition it would use is described in DESIGN.md", file=sys.stderr)

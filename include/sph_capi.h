@@ -425,6 +425,11 @@ int sph_gnn_backward(sph_gnn* g);
 int sph_gnn_zero_grad(sph_gnn* g);
 /* Adam step (beta 0.9 / 0.999, eps 1e-8) with global-norm clipping at clip (<= 0: none). */
 int sph_gnn_adam(sph_gnn* g, float lr, float clip);
+/* Forward MLP calls as one fused kernel each (fused = 1, 2 or 4: hidden
+   activations stay in LDS, operand loads run `fused` k-steps ahead of the
+   MFMAs; default 2) or as three GEMM launches (fused = 0).  Every setting
+   gives bit-identical results. */
+int sph_gnn_set_fused(sph_gnn* g, int fused);
 /* One MFMA tile D(16x16) = A(16x32) B(32x16) in BF16 (host arrays as raw
    uint16 BF16 bits), for checking the operand layout. */
 int sph_gnn_mfma_probe(const unsigned short* A, const unsigned short* B, float* D);

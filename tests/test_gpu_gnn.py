@@ -115,6 +115,36 @@ def test_loss_and_gradient_match_oracle(gpu, orc, layers):
     g.close()
 
 
+@pytest.mark.parametrize("train", [True, False])
+def test_fused_mlp_kernel_equals_three_gemm_launches(gpu, train):
+    """The fused MLP kernel (hidden activations in LDS) rounds and accumulates
+    exactly as the three separate GEMM launches: outputs, loss and the stored
+    hidden activations are bit-identical, on a graph whose node and edge
+    counts are not multiples of the 128-row tile."""
+    from tmp_sph_amd import sph
+    from tmp_sph_amd.surrogate import Gnn
+    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.02))
+    hist, target = _state(case)
+    layers = 3
+    g = Gnn(layers=layers, max_nodes=case.n, max_edges=40 * case.n, seed=5, train=train)
+    g.load(case.params, case.pos, hist, case.rho, case.type, target)
+    _, send, _ = g.graph()
+    assert case.n % 128 and send.size % 128
+    res = {}
+    for depth in (0, 1, 2, 4):  # 0: three GEMM launches
+        g.set_fused(depth)
+        out, loss = g.forward(want_out=True)
+        H = [g.tensor(k, c) for c in range(3 + 2 * layers) for k in (0, 1)] if train else []
+        res[depth] = (out, loss, H, g.tensor(5) if train else out)
+    for depth in (1, 2, 4):
+        np.testing.assert_array_equal(res[depth][0], res[0][0])
+        assert res[depth][1] == res[0][1]
+        np.testing.assert_array_equal(res[depth][3], res[0][3])
+        for a, b in zip(res[depth][2], res[0][2]):
+            np.testing.assert_array_equal(a, b)
+    g.close()
+
+
 def test_rollout_fallback_reproduces_plain_run(gpu):
     """With a judge that rejects every surrogate step the rollout is exactly the
     plain solver; with one that accepts everything the surrogate advances."""

@@ -174,6 +174,7 @@ _PROTOS = {
     "sph_sim_probe_batch": [P, C.c_double, P, P],
     "sph_sim_autotune": [P, I, C.POINTER(SphTuning), DP],
     "sph_gnn_create": [P, C.POINTER(C.c_void_p)],
+    "sph_gnn_set_fused": [P, I],
     "sph_gnn_destroy": [P],
     "sph_gnn_num_params": [P, C.POINTER(C.c_longlong)],
     "sph_gnn_get_params": [P, P],

@@ -74,7 +74,9 @@ struct sph_gnn {
   float *dY = nullptr, *dV = nullptr, *dE = nullptr, *dX = nullptr, *dZf[2] = {nullptr, nullptr};
   bf16 *dZb[2] = {nullptr, nullptr}, *Tz = nullptr, *Tx = nullptr;
   float *part = nullptr, *colpart = nullptr, *scal = nullptr;
+  double* lpart = nullptr;  // per-block loss partials
   int split_cap = 512;  // K chunks of the weight-gradient GEMMs (enough blocks to fill the GPU)
+  int fused = 2;        // forward MLP calls: fused kernel with this load depth (1, 2, 4), or three GEMMs (0)
 };
 
 #define SPH_HIP(call, where)                            \
@@ -279,8 +281,24 @@ __global__ void loss_kernel(const float* __restrict__ Y, const float* __restrict
       l += (double)r * r;
     }
   }
+  // block partial in a fixed order (the sum over blocks is loss_final_kernel's)
   for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
-  if ((threadIdx.x & 63) == 0) atomicAdd(loss, l * inv_count);
+  __shared__ double wsum[4];
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) loss[blockIdx.x] = ((wsum[0] + wsum[1]) + (wsum[2] + wsum[3])) * inv_count;
+}
+
+// *loss = sum of the nb block partials, in a fixed order
+__global__ __launch_bounds__(256) void loss_final_kernel(const double* __restrict__ part, int nb,
+                                                         double* __restrict__ loss) {
+  double l = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) l += part[i];
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+  __shared__ double wsum[4];
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) *loss = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
 __global__ void gather_out_kernel(const float* __restrict__ Y, float* __restrict__ out, int n) {
@@ -342,6 +360,26 @@ hipError_t pack_all(sph_gnn* g) {
 // out (f32, M x 128) = MLP(in) (+ resid); h1, h2 receive the hidden activations
 hipError_t mlp_forward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, bf16* h1, bf16* h2, float* out,
                        const float* resid) {
+  if (g->fused) {
+    Mlp3Args f{};
+    f.A = in;
+    f.K1 = m.kin;
+    f.W1 = g->Wb + m.l[0].wb;
+    f.W2 = g->Wb + m.l[1].wb;
+    f.W3 = g->Wb + m.l[2].wb;
+    f.b1 = g->P + m.l[0].b;
+    f.b2 = g->P + m.l[1].b;
+    f.b3 = g->P + m.l[2].b;
+    f.resid = resid;
+    f.out = out;
+    f.ldo = W;
+    // the hidden activations are only read back by the backward pass
+    f.h1 = g->cfg.train ? h1 : nullptr;
+    f.h2 = g->cfg.train ? h2 : nullptr;
+    f.M = M;
+    f.depth = g->fused;
+    return mlp3(f, g->stream);
+  }
   GemmArgs a{};
   a.A = in;
   a.B = g->Wb + m.l[0].wb;
@@ -592,6 +630,7 @@ extern "C" int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out) {
   rc = rc ? rc : galloc(g, &g->Y, Nn * W);
   rc = rc ? rc : galloc(g, &g->H, (size_t)calls * h_slot(g));
   rc = rc ? rc : galloc(g, &g->scal, 64);
+  rc = rc ? rc : galloc(g, &g->lpart, (size_t)nblk(cfg->max_nodes));
   if (tr) {
     rc = rc ? rc : galloc(g, &g->dY, Nn * W);
     rc = rc ? rc : galloc(g, &g->dV, Nn * W);
@@ -1035,10 +1074,10 @@ extern "C" int sph_gnn_forward(sph_gnn* g, float* out, double* loss) {
   SPH_HIP(forward(g), "gnn forward");
   if (g->has_target && g->cfg.train) {
     SPH_HIP(fill_f32(g->dY, 0.0f, (long long)g->n * W, s), "loss");
-    SPH_HIP(hipMemsetAsync(g->scal, 0, sizeof(double), s), "loss");
     const float inv = g->nfluid > 0 ? 1.0f / (3.0f * (float)g->nfluid) : 0.0f;
     hipLaunchKernelGGL(loss_kernel, dim3(nblk(g->n)), dim3(256), 0, s, g->Y, g->target, g->type, inv, g->dY,
-                       (double*)g->scal, g->n);
+                       g->lpart, g->n);
+    hipLaunchKernelGGL(loss_final_kernel, dim3(1), dim3(256), 0, s, g->lpart, (int)nblk(g->n), (double*)g->scal);
     SPH_HIP(hipGetLastError(), "loss");
   }
   if (out) {
@@ -1091,6 +1130,12 @@ extern "C" int sph_gnn_adam(sph_gnn* g, float lr, float clip) {
   SPH_HIP(adam(g->P, g->G, g->M1, g->M2, (long long)g->nparam, lr, 0.9f, 0.999f, 1e-8f, g->adam_t, scale, s), "adam");
   SPH_HIP(pack_all(g), "pack");
   SPH_HIP(hipStreamSynchronize(s), "adam sync");
+  return SPH_OK;
+}
+
+extern "C" int sph_gnn_set_fused(sph_gnn* g, int fused) {
+  if (!g || fused < 0 || fused > 4 || fused == 3) return set_error(SPH_E_INVAL, "sph_gnn_set_fused: bad argument");
+  g->fused = fused;
   return SPH_OK;
 }
 

@@ -47,6 +47,33 @@ struct GemmArgs {
 
 hipError_t gemm(const GemmArgs& g, hipStream_t s);
 
+// One whole MLP call (three linear layers of width 128, ReLU after the first
+// two) in one kernel: out[M x 128] = L3(relu(L2(relu(L1(A))))) + bias3 (+
+// resid), the hidden activations kept in LDS as BF16 (and also stored to
+// h1 / h2 when those are given, for the backward pass).  Same operand
+// rounding and accumulation order as three gemm() calls, so the results are
+// bit-identical to them.  W1 is 128 x K1, W2 and W3 are 128 x 128 (BF16, row
+// n contiguous in k); resid may alias out (row m is read before it is written).
+struct Mlp3Args {
+  AOperand A;
+  int K1;
+  const bf16* W1;
+  const bf16* W2;
+  const bf16* W3;
+  const float* b1;
+  const float* b2;
+  const float* b3;
+  const float* resid;
+  float* out;
+  int ldo;
+  bf16* h1;
+  bf16* h2;
+  int M;
+  int depth;  // k-steps of operand loads in flight (1, 2 or 4; divides K1 / 32)
+};
+
+hipError_t mlp3(const Mlp3Args& g, hipStream_t s);
+
 // out[c * ldo + r] (BF16) = element (r, c) of the R x C matrix given as an A
 // operand (the gathered / concatenated input), for r < ldo; columns
 // R <= r < ldo are zero (padding of the reduction to a multiple of 32).

@@ -32,29 +32,6 @@ __device__ __forceinline__ bf16x8 cvt8(float4 x, float4 y) {
   return v;
 }
 
-__device__ __forceinline__ bf16x8 load_seg8(const ASeg& g, int row, int kk) {
-  const int r = g.rows ? g.rows[row] : row;
-  if (g.is_f32) {
-    const float4* p = reinterpret_cast<const float4*>(static_cast<const float*>(g.ptr) + (size_t)r * g.ld + kk);
-    return cvt8(p[0], p[1]);
-  }
-  return *reinterpret_cast<const bf16x8*>(static_cast<const bf16*>(g.ptr) + (size_t)r * g.ld + kk);
-}
-
-// 8 consecutive K elements of row `row` of the concatenated operand.
-__device__ __forceinline__ bf16x8 load_a8(const AOperand& A, int row, int k, int M) {
-  if (row >= M) {
-    bf16x8 z;
-    for (int q = 0; q < 8; ++q) z[q] = (bf16)0.0f;
-    return z;
-  }
-  const int w0 = A.seg[0].width;
-  if (A.nseg == 1 || k < w0) return load_seg8(A.seg[0], row, k);
-  const int w1 = A.seg[1].width;
-  if (A.nseg == 2 || k < w0 + w1) return load_seg8(A.seg[1], row, k - w0);
-  return load_seg8(A.seg[2], row, k - w0 - w1);
-}
-
 // Raw (unconverted) 8-element A fragment: two 16-byte words holding 8 FP32
 // values, or 8 BF16 values in the first word.  Loading raw and converting one
 // k-step later keeps the loads in flight across the MFMAs of the current step.
@@ -63,37 +40,91 @@ struct ARaw {
   int f32;
 };
 
-__device__ __forceinline__ ARaw load_raw_seg(const ASeg& g, int row, int kk) {
+// The row of each segment that operand row `row` reads (its gathered index
+// for a gathered segment), computed once per kernel.  Rows past M read row
+// M - 1: their products land only in output rows that are never stored.
+struct SegRows {
+  int r[3];
+};
+
+__device__ __forceinline__ SegRows seg_rows(const AOperand& A, int row, int M) {
+  const int rw = min(row, M - 1);
+  SegRows s;
+  s.r[0] = A.seg[0].rows ? A.seg[0].rows[rw] : rw;
+  s.r[1] = A.nseg > 1 ? (A.seg[1].rows ? A.seg[1].rows[rw] : rw) : 0;
+  s.r[2] = A.nseg > 2 ? (A.seg[2].rows ? A.seg[2].rows[rw] : rw) : 0;
+  return s;
+}
+
+// The segment holding the k-step that starts at k0.  k0 is the same for all
+// lanes (segment widths are multiples of the 32-wide k-step), so the choice
+// stays in scalar registers; fields are picked with constant indices (a
+// pointer into the kernel-argument struct would move it to memory).
+struct SegSel {
+  const void* ptr;
+  int ld, is_f32, i, kk;
+};
+
+__device__ __forceinline__ SegSel seg_at(const AOperand& A, int k0) {
+  const int w0 = A.seg[0].width, w1 = A.seg[1].width;
+  SegSel s;
+  s.i = (A.nseg == 1 || k0 < w0) ? 0 : (A.nseg == 2 || k0 < w0 + w1) ? 1 : 2;
+  s.kk = s.i == 0 ? k0 : s.i == 1 ? k0 - w0 : k0 - w0 - w1;
+  s.ptr = s.i == 0 ? A.seg[0].ptr : s.i == 1 ? A.seg[1].ptr : A.seg[2].ptr;
+  s.ld = s.i == 0 ? A.seg[0].ld : s.i == 1 ? A.seg[1].ld : A.seg[2].ld;
+  s.is_f32 = s.i == 0 ? A.seg[0].is_f32 : s.i == 1 ? A.seg[1].is_f32 : A.seg[2].is_f32;
+  return s;
+}
+
+// K order inside a k-step.  BF16 operands: lane group lq (= lane / 16) holds
+// k = 8 lq .. 8 lq + 7 (one 16-byte load per lane, 64 contiguous bytes per
+// row).  FP32 operands: lane group lq holds k = 4 lq .. 4 lq + 3 and
+// 16 + 4 lq .. 16 + 4 lq + 3, so each of its two 16-byte loads reads 64
+// contiguous bytes of the row across the four lane groups (whole cache
+// lines); the B tile is staged in LDS in the same order (store_b).  A
+// product's k order only permutes the 32 terms of one MFMA.
+__device__ __forceinline__ ARaw load_raw(const SegSel& sel, const SegRows& rows, int lq) {
+  const int rr = sel.i == 0 ? rows.r[0] : sel.i == 1 ? rows.r[1] : rows.r[2];
   ARaw r;
-  const int rr = g.rows ? g.rows[row] : row;
-  if (g.is_f32) {
-    const uint4* p = reinterpret_cast<const uint4*>(static_cast<const float*>(g.ptr) + (size_t)rr * g.ld + kk);
-    r.a = p[0];
-    r.b = p[1];
+  if (sel.is_f32) {
+    const float* p = static_cast<const float*>(sel.ptr) + (size_t)rr * sel.ld + sel.kk + 4 * lq;
+    r.a = *reinterpret_cast<const uint4*>(p);
+    r.b = *reinterpret_cast<const uint4*>(p + 16);
     r.f32 = 1;
   } else {
-    r.a = *reinterpret_cast<const uint4*>(static_cast<const bf16*>(g.ptr) + (size_t)rr * g.ld + kk);
+    r.a = *reinterpret_cast<const uint4*>(static_cast<const bf16*>(sel.ptr) + (size_t)rr * sel.ld + sel.kk + 8 * lq);
     r.b = make_uint4(0u, 0u, 0u, 0u);
     r.f32 = 0;
   }
   return r;
 }
 
-// (segments picked with constant indices: a pointer into the kernel-argument
-// struct would move it to scratch)
-__device__ __forceinline__ ARaw load_raw8(const AOperand& A, int row, int k, int M) {
-  if (row >= M) {
-    ARaw r;
-    r.a = make_uint4(0u, 0u, 0u, 0u);
-    r.b = r.a;
-    r.f32 = 0;
-    return r;
+// The 128 x 32 piece [k0, k0 + 32) of B (row n contiguous in k, row stride
+// ldb): thread tid loads the 16-byte parts tid and tid + 256 (row part / 4,
+// quarter part % 4), four lanes per row.
+__device__ __forceinline__ void load_b(const bf16* __restrict__ B, int ldb, int k0, int tid, uint4& q0, uint4& q1) {
+  q0 = *reinterpret_cast<const uint4*>(B + (size_t)(tid >> 2) * ldb + k0 + 8 * (tid & 3));
+  q1 = *reinterpret_cast<const uint4*>(B + (size_t)((tid + 256) >> 2) * ldb + k0 + 8 * (tid & 3));
+}
+
+// Stores the two parts of load_b into the LDS tile [n][LDB] in the k order
+// of the A operand: natural (perm = 0, BF16 A) or the FP32 order (perm = 1),
+// where k = 8 p .. 8 p + 3 of quarter p go to slots lo(p) .. lo(p) + 3 and
+// k = 8 p + 4 .. 8 p + 7 to lo(p) + 8 .., lo(p) = 16 p (p < 2), 16 (p - 2) + 4.
+__device__ __forceinline__ void store_b(bf16* dst, uint4 q0, uint4 q1, int tid, int perm) {
+  const int part = tid & 3;
+  bf16* r0 = dst + (tid >> 2) * LDB;
+  bf16* r1 = dst + ((tid + 256) >> 2) * LDB;
+  if (!perm) {
+    *reinterpret_cast<uint4*>(r0 + 8 * part) = q0;
+    *reinterpret_cast<uint4*>(r1 + 8 * part) = q1;
+  } else {
+    const int lo = part < 2 ? 16 * part : 16 * (part - 2) + 4;
+    *reinterpret_cast<uint2*>(r0 + lo) = make_uint2(q0.x, q0.y);
+    *reinterpret_cast<uint2*>(r0 + lo + 8) = make_uint2(q0.z, q0.w);
+    *reinterpret_cast<uint2*>(r1 + lo) = make_uint2(q1.x, q1.y);
+    *reinterpret_cast<uint2*>(r1 + lo + 8) = make_uint2(q1.z, q1.w);
   }
-  const int w0 = A.seg[0].width;
-  if (A.nseg == 1 || k < w0) return load_raw_seg(A.seg[0], row, k);
-  const int w1 = A.seg[1].width;
-  if (A.nseg == 2 || k < w0 + w1) return load_raw_seg(A.seg[1], row, k - w0);
-  return load_raw_seg(A.seg[2], row, k - w0 - w1);
 }
 
 __device__ __forceinline__ bf16x8 convert8(const ARaw& r) {
@@ -144,30 +175,31 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   f32x4 acc[2][8];
   for (int t = 0; t < 2; ++t)
     for (int c = 0; c < 8; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  // this thread's share of the B tile: row n of B, 16 elements at 16 * half
-  const int bn = tid >> 1, bhalf = tid & 1;
-  const bf16* bsrc = g.B + (size_t)(n0 + bn) * g.K + 16 * bhalf;
+  const bf16* Bn = g.B + (size_t)n0 * g.K;
   uint4 bq0 = make_uint4(0u, 0u, 0u, 0u), bq1 = bq0;
   ARaw ar0{}, ar1{};
+  int perm = 0;  // k order of the step whose operands are in flight (FP32 A: 1)
+  const SegRows rows0 = seg_rows(g.A, m0 + li, g.M), rows1 = seg_rows(g.A, m0 + 16 + li, g.M);
   if (kbeg < kend) {
-    bq0 = *reinterpret_cast<const uint4*>(bsrc + kbeg);
-    bq1 = *reinterpret_cast<const uint4*>(bsrc + kbeg + 8);
-    ar0 = load_raw8(g.A, m0 + li, kbeg + 8 * lq, g.M);
-    ar1 = load_raw8(g.A, m0 + 16 + li, kbeg + 8 * lq, g.M);
+    load_b(Bn, g.K, kbeg, tid, bq0, bq1);
+    const SegSel sel = seg_at(g.A, kbeg);
+    perm = sel.is_f32;
+    ar0 = load_raw(sel, rows0, lq);
+    ar1 = load_raw(sel, rows1, lq);
   }
   int buf = 0;
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    *reinterpret_cast<uint4*>(&ldsB[buf][bn * LDB + 16 * bhalf]) = bq0;
-    *reinterpret_cast<uint4*>(&ldsB[buf][bn * LDB + 16 * bhalf + 8]) = bq1;
+    store_b(ldsB[buf], bq0, bq1, tid, perm);
     bf16x8 af[2];
     af[0] = convert8(ar0);
     af[1] = convert8(ar1);
     __syncthreads();
     if (k0 + BK < kend) {  // next step's operands, in flight during the MFMAs
-      bq0 = *reinterpret_cast<const uint4*>(bsrc + k0 + BK);
-      bq1 = *reinterpret_cast<const uint4*>(bsrc + k0 + BK + 8);
-      ar0 = load_raw8(g.A, m0 + li, k0 + BK + 8 * lq, g.M);
-      ar1 = load_raw8(g.A, m0 + 16 + li, k0 + BK + 8 * lq, g.M);
+      load_b(Bn, g.K, k0 + BK, tid, bq0, bq1);
+      const SegSel sel = seg_at(g.A, k0 + BK);
+      perm = sel.is_f32;
+      ar0 = load_raw(sel, rows0, lq);
+      ar1 = load_raw(sel, rows1, lq);
     }
     const bf16* tileB = ldsB[buf];
 #pragma unroll
@@ -200,6 +232,154 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         if (g.out_f32) g.out_f32[o] = v;
         if (g.out_bf16) g.out_bf16[o] = (bf16)v;
       }
+}
+
+// ---------------------------------------------------------------- fused MLP
+constexpr int LDH = 128 + 8;  // LDS row stride of a wave's hidden activations (bf16 elements)
+
+__device__ __forceinline__ void mfma_step(f32x4 (&acc)[2][8], const bf16x8 (&af)[2], const bf16* tileB, int li,
+                                          int lq) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const bf16x8 bf = *reinterpret_cast<const bf16x8*>(&tileB[(16 * c + li) * LDB + 8 * lq]);
+    acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf, acc[0][c], 0, 0, 0);
+    acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf, acc[1][c], 0, 0, 0);
+  }
+}
+
+// relu(acc + bias) as BF16 into the wave's LDS rows (and to global h if given)
+__device__ __forceinline__ void hidden_out(const f32x4 (&acc)[2][8], const float* __restrict__ bias, bf16* H,
+                                           bf16* __restrict__ h, int m0, int M, int li, int lq) {
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int n = 16 * c + li;
+      const float b = bias[n];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * t + 4 * lq + r;
+        const bf16 v = (bf16)fmaxf(acc[t][c][r] + b, 0.0f);
+        H[row * LDH + n] = v;
+        if (h && m0 + row < M) h[(size_t)(m0 + row) * 128 + n] = v;
+      }
+    }
+}
+
+// Operand loads run D k-steps ahead of the MFMAs (a ring of D register
+// sets, indexed with compile-time constants): with one step of 16 MFMAs per
+// wave far shorter than a gather's latency, deeper rings keep more bytes in
+// flight per wave.
+
+// One 128 x 128 layer whose A operand is the wave's LDS rows; continues the
+// block-wide double-buffered B staging (buf toggles once per k-step).
+template <int D>
+__device__ __forceinline__ void lds_layer(f32x4 (&acc)[2][8], const bf16* H, const bf16* __restrict__ Wt,
+                                          bf16 (*ldsB)[BN * LDB], int& buf, int tid, int li, int lq) {
+  constexpr int steps = 128 / BK;
+  for (int t = 0; t < 2; ++t)
+    for (int c = 0; c < 8; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  uint4 q[D][2];
+#pragma unroll
+  for (int d = 0; d < D; ++d) load_b(Wt, 128, d * BK, tid, q[d][0], q[d][1]);
+  static_assert(steps % D == 0, "ring depth must divide the k-steps");
+#pragma unroll 1
+  for (int st0 = 0; st0 < steps; st0 += D)
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int st = st0 + d;
+      store_b(ldsB[buf], q[d][0], q[d][1], tid, 0);
+      __syncthreads();
+      load_b(Wt, 128, min(st + D, steps - 1) * BK, tid, q[d][0], q[d][1]);
+      bf16x8 af[2];
+      af[0] = *reinterpret_cast<const bf16x8*>(&H[li * LDH + st * BK + 8 * lq]);
+      af[1] = *reinterpret_cast<const bf16x8*>(&H[(16 + li) * LDH + st * BK + 8 * lq]);
+      mfma_step(acc, af, ldsB[buf], li, lq);
+      buf ^= 1;
+    }
+}
+
+// The FP32 A fragment of lane group lq in the FP32 k order (see load_raw)
+__device__ __forceinline__ void load_f32x8(const SegSel& sel, const SegRows& rows, int lq, uint4& a, uint4& b) {
+  const int rr = sel.i == 0 ? rows.r[0] : sel.i == 1 ? rows.r[1] : rows.r[2];
+  const float* p = static_cast<const float*>(sel.ptr) + (size_t)rr * sel.ld + sel.kk + 4 * lq;
+  a = *reinterpret_cast<const uint4*>(p);
+  b = *reinterpret_cast<const uint4*>(p + 16);
+}
+
+__device__ __forceinline__ bf16x8 cvt_raw(uint4 a, uint4 b) {
+  float4 x, y;
+  __builtin_memcpy(&x, &a, 16);
+  __builtin_memcpy(&y, &b, 16);
+  return cvt8(x, y);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void mlp3_kernel(Mlp3Args g) {
+  __shared__ bf16 ldsB[2][BN * LDB];
+  __shared__ bf16 ldsH[4][32 * LDH];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lq = lane >> 4;
+  const int m0 = blockIdx.x * BM + w * 32;
+  bf16* H = ldsH[w];
+  f32x4 acc[2][8];
+  for (int t = 0; t < 2; ++t)
+    for (int c = 0; c < 8; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  int buf = 0;
+  // layer 1: A gathered from global
+  {
+    const int steps = g.K1 / BK;  // uniform over the block: the barrier below is reached by all threads
+
+    const SegRows rows0 = seg_rows(g.A, m0 + li, g.M), rows1 = seg_rows(g.A, m0 + 16 + li, g.M);
+    uint4 q[D][2], a0[D][2], a1[D][2];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {  // D <= steps (checked by the launcher)
+        load_b(g.W1, g.K1, d * BK, tid, q[d][0], q[d][1]);
+        const SegSel sel = seg_at(g.A, d * BK);
+        load_f32x8(sel, rows0, lq, a0[d][0], a0[d][1]);
+        load_f32x8(sel, rows1, lq, a1[d][0], a1[d][1]);
+      }
+#pragma unroll 1
+    for (int st0 = 0; st0 < steps; st0 += D)  // steps % D == 0 (checked by the launcher)
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int st = st0 + d;
+        store_b(ldsB[buf], q[d][0], q[d][1], tid, 1);
+        bf16x8 af[2];
+        af[0] = cvt_raw(a0[d][0], a0[d][1]);
+        af[1] = cvt_raw(a1[d][0], a1[d][1]);
+        __syncthreads();
+        {  // unconditional (the last steps reload the last k-step): no register copies
+          const int k = min(st + D, steps - 1) * BK;
+          load_b(g.W1, g.K1, k, tid, q[d][0], q[d][1]);
+          const SegSel sel = seg_at(g.A, k);
+          load_f32x8(sel, rows0, lq, a0[d][0], a0[d][1]);
+          load_f32x8(sel, rows1, lq, a1[d][0], a1[d][1]);
+        }
+        mfma_step(acc, af, ldsB[buf], li, lq);
+        buf ^= 1;
+      }
+  }
+  hidden_out(acc, g.b1, H, g.h1, m0, g.M, li, lq);
+  lds_layer<D>(acc, H, g.W2, ldsB, buf, tid, li, lq);
+  hidden_out(acc, g.b2, H, g.h2, m0, g.M, li, lq);
+  lds_layer<D>(acc, H, g.W3, ldsB, buf, tid, li, lq);
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int n = 16 * c + li;
+      const float b = g.b3[n];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 16 * t + 4 * lq + r;
+        if (m >= g.M) continue;
+        const size_t o = (size_t)m * g.ldo + n;
+        float v = acc[t][c][r] + b;
+        if (g.resid) v += g.resid[o];
+        g.out[o] = v;
+      }
+    }
 }
 
 __global__ void transpose_kernel(AOperand A, int R, int C, bf16* __restrict__ out, int ldo) {
@@ -402,6 +582,26 @@ hipError_t gemm(const GemmArgs& g, hipStream_t s) {
   if (g.N % BN != 0 || g.K % BK != 0 || g.K <= 0) return hipErrorInvalidValue;
   const int ks = g.ksplit > 1 ? g.ksplit : 1;
   hipLaunchKernelGGL(gemm_kernel, dim3(cdiv(g.M, BM), g.N / BN, ks), dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t mlp3(const Mlp3Args& g, hipStream_t s) {
+  if (g.M <= 0) return hipSuccess;
+  if (g.K1 % BK != 0 || g.K1 <= 0 || !g.W1 || !g.W2 || !g.W3 || !g.out) return hipErrorInvalidValue;
+  int width = 0;
+  for (int i = 0; i < g.A.nseg; ++i) {
+    if (!g.A.seg[i].is_f32 || g.A.seg[i].width % BK != 0) return hipErrorInvalidValue;  // FP32 operands only
+    width += g.A.seg[i].width;
+  }
+  if (width != g.K1) return hipErrorInvalidValue;
+  const dim3 grid(cdiv(g.M, BM));
+  if ((g.K1 / BK) % g.depth != 0) return hipErrorInvalidValue;
+  switch (g.depth) {
+    case 1: hipLaunchKernelGGL(mlp3_kernel<1>, grid, dim3(256), 0, s, g); break;
+    case 2: hipLaunchKernelGGL(mlp3_kernel<2>, grid, dim3(256), 0, s, g); break;
+    case 4: hipLaunchKernelGGL(mlp3_kernel<4>, grid, dim3(256), 0, s, g); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

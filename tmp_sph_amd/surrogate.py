@@ -121,6 +121,11 @@ class Gnn:
         call("sph_gnn_features", self._h, _hp(xn), _hp(xe))
         return xn, xe
 
+    def set_fused(self, depth: int) -> None:
+        """One fused kernel per MLP call with `depth` k-steps of loads in
+        flight (1, 2 or 4; default 2), or three GEMM launches (0)."""
+        call("sph_gnn_set_fused", self._h, int(depth))
+
     def tensor(self, kind: int, index: int = 0) -> np.ndarray:
         """An activation of the last forward pass (see sph_gnn_get_tensor):
         kinds 0/1 return BF16 bit patterns (uint16), the others FP32."""
