@@ -75,8 +75,11 @@ struct sph_gnn {
   bf16 *dZb[2] = {nullptr, nullptr}, *Tz = nullptr, *Tx = nullptr;
   float *part = nullptr, *colpart = nullptr, *scal = nullptr;
   double* lpart = nullptr;  // per-block loss partials
+  float* Pn = nullptr;      // inference: node projections of the edge MLPs' first layer (nodes x 256)
   int split_cap = 512;  // K chunks of the weight-gradient GEMMs (enough blocks to fill the GPU)
-  int fused = 2;        // forward MLP calls: fused kernel with this load depth (1, 2, 4), or three GEMMs (0)
+  // forward MLP calls: fused kernel with this load depth (1, 2, 4), three GEMMs (0); 8: edge MLPs by the
+  // persistent kernel with node projections (inference contexts), other calls fused with depth 2
+  int fused = 2;
 };
 
 #define SPH_HIP(call, where)                            \
@@ -377,7 +380,7 @@ hipError_t mlp_forward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, bf16
     f.h1 = g->cfg.train ? h1 : nullptr;
     f.h2 = g->cfg.train ? h2 : nullptr;
     f.M = M;
-    f.depth = g->fused;
+    f.depth = g->fused == 8 ? 2 : g->fused;
     return mlp3(f, g->stream);
   }
   GemmArgs a{};
@@ -476,6 +479,39 @@ hipError_t mlp_backward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, con
   return gemm(b, g->stream);
 }
 
+// Edge update of an inference context in place: E += MLP_e([E, V[recv], V[send]]),
+// the first layer's receiver and sender blocks applied to the nodes first
+// (P = V W1r^T | V W1s^T), then one persistent kernel per edge set.
+hipError_t edge_forward_split(sph_gnn* g, const Mlp& m, const float* V, float* E) {
+  for (int half = 0; half < 2; ++half) {
+    GemmArgs a{};
+    a.A = op1(segf(V, W, W));
+    a.B = g->Wb + m.l[0].wb + (size_t)W * (1 + half);  // columns [128, 256) / [256, 384) of W1
+    a.ldb = 3 * W;
+    a.out_f32 = g->Pn + (size_t)W * half;
+    a.M = g->n;
+    a.N = W;
+    a.K = W;
+    a.ldo = 2 * W;
+    a.ksplit = 1;
+    HIPCHK(gemm(a, g->stream));
+  }
+  EdgeMlpArgs e{};
+  e.E = E;
+  e.ne = g->ne;
+  e.P = g->Pn;
+  e.recv = g->recv;
+  e.send = g->send;
+  e.W1 = g->Wb + m.l[0].wb;
+  e.ldw1 = 3 * W;
+  e.W2 = g->Wb + m.l[1].wb;
+  e.W3 = g->Wb + m.l[2].wb;
+  e.b1 = g->P + m.l[0].b;
+  e.b2 = g->P + m.l[1].b;
+  e.b3 = g->P + m.l[2].b;
+  return edge_mlp(e, g->stream);
+}
+
 // hidden-activation slots: call c of the forward pass uses H + c * slot
 size_t h_slot(const sph_gnn* g) { return (size_t)2 * W * (size_t)std::max(g->cfg.max_nodes, g->cfg.max_edges); }
 
@@ -502,8 +538,12 @@ hipError_t forward(sph_gnn* g) {
     float* El1 = E_of(g, tr ? l + 1 : 0);
     float* Vl1 = V_of(g, tr ? l + 1 : 0);
     float* Al = A_of(g, l);
-    AOperand ein = op3(segf(El, W, W), segf(Vl, W, W, g->recv), segf(Vl, W, W, g->send));
-    HIPCHK(mlp_forward(g, g->pe[l], ein, ne, h1_of(g, call), h2_of(g, call), El1, El));
+    if (g->fused == 8 && !tr) {
+      HIPCHK(edge_forward_split(g, g->pe[l], Vl, El));
+    } else {
+      AOperand ein = op3(segf(El, W, W), segf(Vl, W, W, g->recv), segf(Vl, W, W, g->send));
+      HIPCHK(mlp_forward(g, g->pe[l], ein, ne, h1_of(g, call), h2_of(g, call), El1, El));
+    }
     ++call;
     HIPCHK(segment_sum(El1, W, g->off, nullptr, n, Al, W, 0, g->stream));
     AOperand nin = op2(segf(Vl, W, W), segf(Al, W, W));
@@ -579,6 +619,7 @@ extern "C" int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out) {
   sph_gnn* g = new sph_gnn();
   g->cfg = *cfg;
   g->L = cfg->layers;
+  g->fused = cfg->train ? 2 : 8;
   g->enc_n = make_mlp(g, W);
   g->enc_e = make_mlp(g, W);
   for (int l = 0; l < g->L; ++l) {
@@ -627,6 +668,7 @@ extern "C" int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out) {
   rc = rc ? rc : galloc(g, &g->V, (tr ? (size_t)g->L + 1 : 1) * Nn * W);
   rc = rc ? rc : galloc(g, &g->E, (tr ? (size_t)g->L + 1 : 1) * Ne * W);
   rc = rc ? rc : galloc(g, &g->Agg, (tr ? (size_t)g->L : 1) * Nn * W);
+  if (!tr) rc = rc ? rc : galloc(g, &g->Pn, Nn * 2 * W);
   rc = rc ? rc : galloc(g, &g->Y, Nn * W);
   rc = rc ? rc : galloc(g, &g->H, (size_t)calls * h_slot(g));
   rc = rc ? rc : galloc(g, &g->scal, 64);
@@ -1134,7 +1176,9 @@ extern "C" int sph_gnn_adam(sph_gnn* g, float lr, float clip) {
 }
 
 extern "C" int sph_gnn_set_fused(sph_gnn* g, int fused) {
-  if (!g || fused < 0 || fused > 4 || fused == 3) return set_error(SPH_E_INVAL, "sph_gnn_set_fused: bad argument");
+  if (!g || !(fused == 0 || fused == 1 || fused == 2 || fused == 4 || fused == 8))
+    return set_error(SPH_E_INVAL, "sph_gnn_set_fused: bad argument");
+  if (fused == 8 && g->cfg.train) return set_error(SPH_E_STATE, "sph_gnn_set_fused: mode 8 needs train = 0");
   g->fused = fused;
   return SPH_OK;
 }

@@ -43,9 +43,34 @@ struct GemmArgs {
   int ldo;
   int relu;
   int ksplit;
+  int ldb;  // row stride of B in elements (0: K)
 };
 
 hipError_t gemm(const GemmArgs& g, hipStream_t s);
+
+// Inference form of an edge-update MLP call, persistent (one block of 8
+// waves per CU, the three weight matrices resident in LDS, no barriers after
+// staging them): E[e] += L3(relu(L2(relu(E[e] W1e^T + P[recv[e]][0:128] +
+// P[send[e]][128:256] + b1)))), where P (nodes x 256) holds the node
+// projections V W1r^T | V W1s^T of the first layer's receiver and sender
+// blocks (W1 = [W1e | W1r | W1s], 128 x 384, row stride ldw1).  The same
+// products as the concatenated first layer, summed in a different order.
+struct EdgeMlpArgs {
+  float* E;
+  int ne;
+  const float* P;
+  const int* recv;
+  const int* send;
+  const bf16* W1;
+  int ldw1;
+  const bf16* W2;
+  const bf16* W3;
+  const float* b1;
+  const float* b2;
+  const float* b3;
+};
+
+hipError_t edge_mlp(const EdgeMlpArgs& g, hipStream_t s);
 
 // One whole MLP call (three linear layers of width 128, ReLU after the first
 // two) in one kernel: out[M x 128] = L3(relu(L2(relu(L1(A))))) + bias3 (+

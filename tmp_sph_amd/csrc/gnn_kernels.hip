@@ -10,6 +10,8 @@
 // The tests check it against numpy through mfma_tile().
 #include "gnn_kernels.h"
 
+#include <algorithm>
+
 namespace sph {
 namespace gnn {
 
@@ -175,13 +177,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   f32x4 acc[2][8];
   for (int t = 0; t < 2; ++t)
     for (int c = 0; c < 8; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  const bf16* Bn = g.B + (size_t)n0 * g.K;
+  const int ldb = g.ldb > 0 ? g.ldb : g.K;
+  const bf16* Bn = g.B + (size_t)n0 * ldb;
   uint4 bq0 = make_uint4(0u, 0u, 0u, 0u), bq1 = bq0;
   ARaw ar0{}, ar1{};
   int perm = 0;  // k order of the step whose operands are in flight (FP32 A: 1)
   const SegRows rows0 = seg_rows(g.A, m0 + li, g.M), rows1 = seg_rows(g.A, m0 + 16 + li, g.M);
   if (kbeg < kend) {
-    load_b(Bn, g.K, kbeg, tid, bq0, bq1);
+    load_b(Bn, ldb, kbeg, tid, bq0, bq1);
     const SegSel sel = seg_at(g.A, kbeg);
     perm = sel.is_f32;
     ar0 = load_raw(sel, rows0, lq);
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     af[1] = convert8(ar1);
     __syncthreads();
     if (k0 + BK < kend) {  // next step's operands, in flight during the MFMAs
-      load_b(Bn, g.K, k0 + BK, tid, bq0, bq1);
+      load_b(Bn, ldb, k0 + BK, tid, bq0, bq1);
       const SegSel sel = seg_at(g.A, k0 + BK);
       perm = sel.is_f32;
       ar0 = load_raw(sel, rows0, lq);
@@ -210,28 +213,46 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     }
     buf ^= 1;
   }
+  // Epilogue, one 16-row half at a time: the operands it reads (bias,
+  // residual, mask) are all loaded before any is used -- rows past M read
+  // row M - 1 -- so their latencies overlap; stores are predicated per row.
+  const bool fin = g.ksplit <= 1;
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
+  for (int t = 0; t < 2; ++t) {
+    float rs[8][4], bs[8];
+    bool keep[8][4];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int n = n0 + 16 * c + li;
+      bs[c] = (fin && g.bias) ? g.bias[n] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const size_t o = (size_t)min(m0 + 16 * t + 4 * lq + r, g.M - 1) * g.ldo + n;
+        rs[c][r] = (fin && g.resid) ? g.resid[o] : 0.0f;
+        keep[c][r] = (fin && g.mask) ? ((float)g.mask[o] > 0.0f) : true;
+      }
+    }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + 16 * t + 4 * lq + r;
-        const int n = n0 + 16 * c + li;
-        if (m >= g.M) continue;
+        const size_t o = (size_t)m * g.ldo + n0 + 16 * c + li;
         float v = acc[t][c][r];
-        const size_t o = (size_t)m * g.ldo + n;
-        if (g.ksplit > 1) {
-          g.out_f32[(size_t)z * g.M * g.ldo + o] = v;
+        if (!fin) {
+          if (m < g.M) g.out_f32[(size_t)z * g.M * g.ldo + o] = v;
           continue;
         }
-        if (g.bias) v += g.bias[n];
-        if (g.resid) v += g.resid[o];
+        if (g.bias) v += bs[c];
+        if (g.resid) v += rs[c][r];
         if (g.relu) v = fmaxf(v, 0.0f);
-        if (g.mask && !((float)g.mask[o] > 0.0f)) v = 0.0f;
-        if (g.out_f32) g.out_f32[o] = v;
-        if (g.out_bf16) g.out_bf16[o] = (bf16)v;
+        if (!keep[c][r]) v = 0.0f;
+        if (m < g.M) {
+          if (g.out_f32) g.out_f32[o] = v;
+          if (g.out_bf16) g.out_bf16[o] = (bf16)v;
+        }
       }
+  }
 }
 
 // ---------------------------------------------------------------- fused MLP
@@ -364,8 +385,15 @@ __global__ __launch_bounds__(256) void mlp3_kernel(Mlp3Args g) {
   lds_layer<D>(acc, H, g.W2, ldsB, buf, tid, li, lq);
   hidden_out(acc, g.b2, H, g.h2, m0, g.M, li, lq);
   lds_layer<D>(acc, H, g.W3, ldsB, buf, tid, li, lq);
+  // residuals loaded in a batch per row half (rows past M read row M - 1)
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
+  for (int t = 0; t < 2; ++t) {
+    float rs[8][4];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        rs[c][r] = g.resid ? g.resid[(size_t)min(m0 + 16 * t + 4 * lq + r, g.M - 1) * g.ldo + 16 * c + li] : 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int n = 16 * c + li;
@@ -373,13 +401,167 @@ __global__ __launch_bounds__(256) void mlp3_kernel(Mlp3Args g) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + 16 * t + 4 * lq + r;
-        if (m >= g.M) continue;
-        const size_t o = (size_t)m * g.ldo + n;
         float v = acc[t][c][r] + b;
-        if (g.resid) v += g.resid[o];
-        g.out[o] = v;
+        if (g.resid) v += rs[c][r];
+        if (m < g.M) g.out[(size_t)m * g.ldo + n] = v;
       }
     }
+  }
+}
+
+// ------------------------------------------------- persistent edge MLP (inference)
+// One block of EW waves per CU holds the three 128 x 128 weight matrices in
+// LDS (96 KB, 16-byte chunks XOR-swizzled by row: the 16 rows of an MFMA
+// fragment read hit 16 different chunks, free of bank conflicts) and one
+// padded 32 x 128 hidden-activation tile per wave (row stride LDH); the LDS
+// holds 7 such tiles.  Waves walk their tiles independently.
+constexpr int EW = 7;
+
+__device__ __forceinline__ int swz(int row, int col) { return row * 128 + ((((col >> 3) ^ (row & 15))) << 3) + (col & 7); }
+
+__global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 Wl[3][128 * 128];
+  __shared__ __attribute__((aligned(16))) bf16 Hs[EW][32 * LDH];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lq = lane >> 4;
+  // weights: 3 x 2048 chunks of 8; W1e in the FP32 k order of the A operand
+  // (chunk 4 s + j of row n holds k = 32 s + 4 j .. + 3 and 32 s + 16 + 4 j .. + 3)
+  for (int c = tid; c < 3 * 2048; c += 64 * EW) {
+    const int mtx = c >> 11, n = (c >> 4) & 127, ch = c & 15;
+    uint4 v;
+    if (mtx == 0) {
+      const bf16* src = g.W1 + (size_t)n * g.ldw1 + 32 * (ch >> 2) + 4 * (ch & 3);
+      const uint2 lo = *reinterpret_cast<const uint2*>(src);
+      const uint2 hi = *reinterpret_cast<const uint2*>(src + 16);
+      v = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    } else {
+      v = *reinterpret_cast<const uint4*>((mtx == 1 ? g.W2 : g.W3) + (size_t)n * 128 + 8 * ch);
+    }
+    *reinterpret_cast<uint4*>(&Wl[mtx][swz(n, 8 * ch)]) = v;
+  }
+  __syncthreads();
+  bf16* H = Hs[w];
+  // B fragment of lane (li, lq), output block c, k-step s: row 16 c + li,
+  // chunk (4 s + lq) ^ li = 4 (s ^ (li >> 2)) + (lq ^ (li & 3)); the
+  // 16 c rows are an immediate offset
+  const int wlane = li * 128 + 8 * (lq ^ (li & 3));
+  const int sx = li >> 2;
+  const int ntiles = (g.ne + 31) / 32;
+  const int stride = gridDim.x * EW;
+  for (int tile = blockIdx.x * EW + w; tile < ntiles; tile += stride) {  // wave-uniform: every wave exits
+    const int m0 = tile * 32;
+    f32x4 acc[2][8];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // layer 1 (E block of W1); A = E rows in the FP32 k order
+    {
+      const float* pa = g.E + (size_t)min(m0 + li, g.ne - 1) * 128 + 4 * lq;
+      const float* pb = g.E + (size_t)min(m0 + 16 + li, g.ne - 1) * 128 + 4 * lq;
+      uint4 ea[4][2][2];
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        ea[s4][0][0] = *reinterpret_cast<const uint4*>(pa + 32 * s4);
+        ea[s4][0][1] = *reinterpret_cast<const uint4*>(pa + 32 * s4 + 16);
+        ea[s4][1][0] = *reinterpret_cast<const uint4*>(pb + 32 * s4);
+        ea[s4][1][1] = *reinterpret_cast<const uint4*>(pb + 32 * s4 + 16);
+      }
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        bf16x8 af[2];
+        af[0] = cvt_raw(ea[s4][0][0], ea[s4][0][1]);
+        af[1] = cvt_raw(ea[s4][1][0], ea[s4][1][1]);
+        const bf16* wb = &Wl[0][wlane + 32 * (s4 ^ sx)];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const bf16x8 bf = *reinterpret_cast<const bf16x8*>(wb + 2048 * c);
+          acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf, acc[0][c], 0, 0, 0);
+          acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf, acc[1][c], 0, 0, 0);
+        }
+      }
+    }
+    // + bias + receiver and sender projections, ReLU, into H
+    // (all gathers of a row half are issued before any is used)
+    int ri[2][4], si[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = min(m0 + 16 * t + 4 * lq + r, g.ne - 1);
+        ri[t][r] = g.recv[m];
+        si[t][r] = g.send[m];
+      }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      float pv[4][8];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float* pr = g.P + (size_t)ri[t][r] * 256 + li;
+        const float* ps = g.P + (size_t)si[t][r] * 256 + 128 + li;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) pv[r][c] = pr[16 * c] + ps[16 * c];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          H[(16 * t + 4 * lq + r) * LDH + 16 * c + li] =
+              (bf16)fmaxf(acc[t][c][r] + g.b1[16 * c + li] + pv[r][c], 0.0f);
+    }
+    // layers 2 and 3 (A = H, rows 16 t + li at k 32 s + 8 lq)
+#pragma unroll
+    for (int L = 1; L < 3; ++L) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        bf16x8 af[2];
+        af[0] = *reinterpret_cast<const bf16x8*>(&H[li * LDH + 32 * s4 + 8 * lq]);
+        af[1] = *reinterpret_cast<const bf16x8*>(&H[(16 + li) * LDH + 32 * s4 + 8 * lq]);
+        const bf16* wb = &Wl[L][wlane + 32 * (s4 ^ sx)];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const bf16x8 bf = *reinterpret_cast<const bf16x8*>(wb + 2048 * c);
+          acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf, acc[0][c], 0, 0, 0);
+          acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf, acc[1][c], 0, 0, 0);
+        }
+      }
+      if (L == 1) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const float b = g.b2[16 * c + li];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              H[(16 * t + 4 * lq + r) * LDH + 16 * c + li] = (bf16)fmaxf(acc[t][c][r] + b, 0.0f);
+          }
+      }
+    }
+    // + bias + residual, in place (residuals of a row half loaded in a batch)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      float rs[8][4];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          rs[c][r] = g.E[(size_t)min(m0 + 16 * t + 4 * lq + r, g.ne - 1) * 128 + 16 * c + li];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int n = 16 * c + li;
+        const float b = g.b3[n];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + 16 * t + 4 * lq + r;
+          if (m < g.ne) g.E[(size_t)m * 128 + n] = acc[t][c][r] + b + rs[c][r];
+        }
+      }
+    }
+  }
 }
 
 __global__ void transpose_kernel(AOperand A, int R, int C, bf16* __restrict__ out, int ldo) {
@@ -602,6 +784,22 @@ hipError_t mlp3(const Mlp3Args& g, hipStream_t s) {
     case 4: hipLaunchKernelGGL(mlp3_kernel<4>, grid, dim3(256), 0, s, g); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t edge_mlp(const EdgeMlpArgs& g, hipStream_t s) {
+  if (g.ne <= 0) return hipSuccess;
+  if (!g.E || !g.P || !g.recv || !g.send || !g.W1 || !g.W2 || !g.W3 || g.ldw1 < 128) return hipErrorInvalidValue;
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  const int tiles = (g.ne + 31) / 32;
+  const int blocks = std::min(cus, (tiles + EW - 1) / EW);
+  hipLaunchKernelGGL(edge_mlp_kernel, dim3(blocks), dim3(64 * EW), 0, s, g);
   return hipGetLastError();
 }
 
