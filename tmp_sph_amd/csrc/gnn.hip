@@ -76,6 +76,7 @@ struct sph_gnn {
   float *part = nullptr, *colpart = nullptr, *scal = nullptr;
   double* lpart = nullptr;  // per-block loss partials
   float* Pn = nullptr;      // inference: node projections of the edge MLPs' first layer (nodes x 256)
+  bf16* Wp = nullptr;       // inference: their weights in lane order, per block (pack_projection)
   int split_cap = 512;  // K chunks of the weight-gradient GEMMs (enough blocks to fill the GPU)
   // forward MLP calls: fused kernel with this load depth (1, 2, 4), three GEMMs (0); 8: edge MLPs by the
   // persistent kernel with node projections (inference contexts), other calls fused with depth 2
@@ -356,6 +357,9 @@ Mlp make_mlp(sph_gnn* g, int kin) {
 
 hipError_t pack_all(sph_gnn* g) {
   for (const Lin& l : g->lins) HIPCHK(pack_weights(g->P + l.w, l.N, l.K, g->Wb + l.wb, g->Wt + l.wb, g->stream));
+  if (g->Wp)
+    for (int l = 0; l < g->L; ++l)
+      HIPCHK(pack_projection(g->Wb + g->pe[l].l[0].wb, g->Wp + (size_t)l * 2 * W * W, g->stream));
   g->packed = true;
   return hipSuccess;
 }
@@ -482,12 +486,13 @@ hipError_t mlp_backward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, con
 // Edge update of an inference context in place: E += MLP_e([E, V[recv], V[send]]),
 // the first layer's receiver and sender blocks applied to the nodes first
 // (P = V W1r^T | V W1s^T), then one persistent kernel per edge set.
-hipError_t edge_forward_split(sph_gnn* g, const Mlp& m, const float* V, float* E) {
+hipError_t edge_forward_split(sph_gnn* g, int l, const float* V, float* E) {
+  const Mlp& m = g->pe[l];
   for (int half = 0; half < 2; ++half) {
     GemmArgs a{};
     a.A = op1(segf(V, W, W));
-    a.B = g->Wb + m.l[0].wb + (size_t)W * (1 + half);  // columns [128, 256) / [256, 384) of W1
-    a.ldb = 3 * W;
+    a.B = g->Wp + ((size_t)2 * l + half) * W * W;  // columns [128, 256) / [256, 384) of W1, rows in lane order
+    a.ldb = W;
     a.out_f32 = g->Pn + (size_t)W * half;
     a.M = g->n;
     a.N = W;
@@ -539,7 +544,7 @@ hipError_t forward(sph_gnn* g) {
     float* Vl1 = V_of(g, tr ? l + 1 : 0);
     float* Al = A_of(g, l);
     if (g->fused == 8 && !tr) {
-      HIPCHK(edge_forward_split(g, g->pe[l], Vl, El));
+      HIPCHK(edge_forward_split(g, l, Vl, El));
     } else {
       AOperand ein = op3(segf(El, W, W), segf(Vl, W, W, g->recv), segf(Vl, W, W, g->send));
       HIPCHK(mlp_forward(g, g->pe[l], ein, ne, h1_of(g, call), h2_of(g, call), El1, El));
@@ -669,6 +674,7 @@ extern "C" int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out) {
   rc = rc ? rc : galloc(g, &g->E, (tr ? (size_t)g->L + 1 : 1) * Ne * W);
   rc = rc ? rc : galloc(g, &g->Agg, (tr ? (size_t)g->L : 1) * Nn * W);
   if (!tr) rc = rc ? rc : galloc(g, &g->Pn, Nn * 2 * W);
+  if (!tr) rc = rc ? rc : galloc(g, &g->Wp, (size_t)g->L * 2 * W * W);
   rc = rc ? rc : galloc(g, &g->Y, Nn * W);
   rc = rc ? rc : galloc(g, &g->H, (size_t)calls * h_slot(g));
   rc = rc ? rc : galloc(g, &g->scal, 64);

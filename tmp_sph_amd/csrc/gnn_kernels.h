@@ -53,8 +53,9 @@ hipError_t gemm(const GemmArgs& g, hipStream_t s);
 // staging them): E[e] += L3(relu(L2(relu(E[e] W1e^T + P[recv[e]][0:128] +
 // P[send[e]][128:256] + b1)))), where P (nodes x 256) holds the node
 // projections V W1r^T | V W1s^T of the first layer's receiver and sender
-// blocks (W1 = [W1e | W1r | W1s], 128 x 384, row stride ldw1).  The same
-// products as the concatenated first layer, summed in a different order.
+// blocks (W1 = [W1e | W1r | W1s], 128 x 384, row stride ldw1) with columns in
+// the lane order of pack_projection.  The same products as the concatenated
+// first layer, summed in a different order.
 struct EdgeMlpArgs {
   float* E;
   int ne;
@@ -120,6 +121,12 @@ hipError_t fill_f32(float* p, float v, long long count, hipStream_t s);
 hipError_t f32_to_bf16(const float* x, bf16* y, long long count, hipStream_t s);
 // W[N][K] (FP32) -> Wb[N][K] and Wt[K][N] (BF16)
 hipError_t pack_weights(const float* W, int N, int K, bf16* Wb, bf16* Wt, hipStream_t s);
+// Receiver and sender blocks of an edge MLP's first layer (W1b: BF16, 128 x
+// 384) for the node projections, rows in lane order: out[h][j][k] =
+// W1b[16 (j % 8) + j / 8][128 (1 + h) + k], so that projection column
+// 8 i + c holds output feature 16 c + i and lane i of edge_mlp_kernel reads
+// its eight features as two 16-byte words.
+hipError_t pack_projection(const bf16* W1b, bf16* out, hipStream_t s);
 // Adam step on count parameters (bias-corrected, step t >= 1)
 hipError_t adam(float* w, const float* g, float* m, float* v, long long count, float lr, float b1, float b2,
                 float eps, int t, float clip_scale, hipStream_t s);

@@ -416,6 +416,8 @@ __global__ __launch_bounds__(256) void mlp3_kernel(Mlp3Args g) {
 // padded 32 x 128 hidden-activation tile per wave (row stride LDH); the LDS
 // holds 7 such tiles.  Waves walk their tiles independently.
 constexpr int EW = 7;
+constexpr int LDF = 128 + 4;  // FP32 row stride of the epilogue's staging rows (16 x LDF x 4 B <= 32 x LDH x 2 B)
+static_assert(16 * LDF * 4 <= 32 * LDH * 2, "staging rows must fit in the hidden tile");
 
 __device__ __forceinline__ int swz(int row, int col) { return row * 128 + ((((col >> 3) ^ (row & 15))) << 3) + (col & 7); }
 
@@ -494,13 +496,21 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
       }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
+      // projection columns 8 li .. 8 li + 7 hold this lane's features 16 c + li
       float pv[4][8];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float* pr = g.P + (size_t)ri[t][r] * 256 + li;
-        const float* ps = g.P + (size_t)si[t][r] * 256 + 128 + li;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) pv[r][c] = pr[16 * c] + ps[16 * c];
+        const float4* pr = reinterpret_cast<const float4*>(g.P + (size_t)ri[t][r] * 256 + 8 * li);
+        const float4* ps = reinterpret_cast<const float4*>(g.P + (size_t)si[t][r] * 256 + 128 + 8 * li);
+        const float4 r0 = pr[0], r1 = pr[1], s0 = ps[0], s1 = ps[1];
+        pv[r][0] = r0.x + s0.x;
+        pv[r][1] = r0.y + s0.y;
+        pv[r][2] = r0.z + s0.z;
+        pv[r][3] = r0.w + s0.w;
+        pv[r][4] = r1.x + s1.x;
+        pv[r][5] = r1.y + s1.y;
+        pv[r][6] = r1.z + s1.z;
+        pv[r][7] = r1.w + s1.w;
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r)
@@ -541,24 +551,32 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
           }
       }
     }
-    // + bias + residual, in place (residuals of a row half loaded in a batch)
+    // + bias + residual, in place: each 16-row half goes through the wave's
+    // (now free) hidden-tile LDS as FP32 rows, so the residual is read and
+    // the result written with row-contiguous 16-byte accesses
+    float* F = reinterpret_cast<float*>(H);  // 16 rows x LDF floats fit in the tile
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      float rs[8][4];
-#pragma unroll
-      for (int c = 0; c < 8; ++c)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          rs[c][r] = g.E[(size_t)min(m0 + 16 * t + 4 * lq + r, g.ne - 1) * 128 + 16 * c + li];
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        const int n = 16 * c + li;
-        const float b = g.b3[n];
+        const float b = g.b3[16 * c + li];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int m = m0 + 16 * t + 4 * lq + r;
-          if (m < g.ne) g.E[(size_t)m * 128 + n] = acc[t][c][r] + b + rs[c][r];
-        }
+        for (int r = 0; r < 4; ++r) F[(4 * lq + r) * LDF + 16 * c + li] = acc[t][c][r] + b;
+      }
+      float4 v[8], rr[8];
+      const int c4 = lane & 31;  // 32 lanes per row, rows 2 j + lane / 32
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int row = 2 * j + (lane >> 5);
+        v[j] = *reinterpret_cast<const float4*>(&F[row * LDF + 4 * c4]);
+        rr[j] = *reinterpret_cast<const float4*>(g.E + (size_t)min(m0 + 16 * t + row, g.ne - 1) * 128 + 4 * c4);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int m = m0 + 16 * t + 2 * j + (lane >> 5);
+        if (m < g.ne)
+          *reinterpret_cast<float4*>(g.E + (size_t)m * 128 + 4 * c4) =
+              make_float4(v[j].x + rr[j].x, v[j].y + rr[j].y, v[j].z + rr[j].z, v[j].w + rr[j].w);
       }
     }
   }
@@ -858,6 +876,17 @@ hipError_t fill_f32(float* p, float v, long long count, hipStream_t s) {
 hipError_t f32_to_bf16(const float* x, bf16* y, long long count, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_for(count)), dim3(256), 0, s, x, y, count);
+  return hipGetLastError();
+}
+
+__global__ void pack_projection_kernel(const bf16* __restrict__ W1b, bf16* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // < 2 * 128 * 128
+  const int h = i >> 14, j = (i >> 7) & 127, k = i & 127;
+  out[i] = W1b[(size_t)(16 * (j & 7) + (j >> 3)) * 384 + 128 * (1 + h) + k];
+}
+
+hipError_t pack_projection(const bf16* W1b, bf16* out, hipStream_t s) {
+  hipLaunchKernelGGL(pack_projection_kernel, dim3(2 * 128 * 128 / 256), dim3(256), 0, s, W1b, out);
   return hipGetLastError();
 }
 

@@ -249,18 +249,25 @@ def normalised_target(s: Snapshot, norm: dict) -> np.ndarray:
 
 
 def train(g: "Gnn", snaps: list, norm: dict, iters: int, lr: float = 3e-4, lr_final: float = 3e-5,
-          clip: float = 1.0, seed: int = 0, seconds: float | None = None, progress=None) -> list:
+          clip: float = 1.0, seed: int = 0, seconds: float | None = None, progress=None,
+          target_clip: float | None = 10.0) -> list:
     """Adam on single-snapshot batches; cosine learning-rate decay over the
     iterations or, with a time budget, over the elapsed fraction of it.  Stops
     after `iters` steps or `seconds` of wall-clock, whichever comes first;
-    progress(iterations, recent mean loss) is called every 30 s."""
+    progress(iterations, recent mean loss) is called every 30 s.  Normalised
+    training targets are clipped to +-target_clip: the few splash particles
+    whose rates lie tens of standard deviations out would otherwise dominate
+    single-snapshot batches (evaluation uses the unclipped targets)."""
     import time
     rng = np.random.default_rng(seed)
     losses = []
     t0 = t_last = time.time()
     for it in range(iters):
         s = snaps[rng.integers(len(snaps))]
-        g.load(s.params, s.pos, s.hist, s.rho, s.type, normalised_target(s, norm))
+        t = normalised_target(s, norm)
+        if target_clip is not None:
+            t = np.clip(t, -target_clip, target_clip)
+        g.load(s.params, s.pos, s.hist, s.rho, s.type, t)
         g.zero_grad()
         _, loss = g.forward(want_out=False)
         g.backward()
