@@ -181,3 +181,29 @@ def test_gauges_and_pressure_probes_match_numpy(gpu, dim):
         assert got[g] == np.float32(want), (g, got[g], want)
     assert got[0] > 0 and got[3] == 0  # a band through the column, one beyond the water
     np.testing.assert_array_equal(p, st["p"][ids])
+
+
+@pytest.mark.parametrize("dim,steps,every", [(2, 2400, 100), (3, 300, 50)])
+def test_front_and_height_histories_match_fp64_host_run(gpu, orc, dim, steps, every):
+    """Surge-front and water-column-height histories of the GPU run against
+    the oracle's all-FP64 run of the same case (2-D: t = 0.4 s, the front
+    travels from 1.0 to 2.0 m).  The host's own FP32 and FP64 runs agree to
+    1e-6 m here, so 1e-4 m (a fifth of a percent of the spacing) leaves room
+    only for rounding-level differences, not for a wrong term."""
+    case = case2d(0.02) if dim == 2 else case3d(0.04)
+    band = 2.0 * case.params.particleR
+    fr, ht = [], []
+    with gpu.Simulation(case) as sim:
+        sim.capture()
+        for _ in range(steps // every):
+            sim.replay(every)
+            f, h = sim.probe(band)
+            fr.append(f)
+            ht.append(h)
+    _, f64, h64, _ = orc.run(case.params, case, steps, mode=2, band=band, probe_every=every)
+    fr, ht = np.array(fr), np.array(ht)
+    print(f"dim {dim}: front {fr[0]:.4f} -> {fr[-1]:.4f} m, max |GPU - FP64| front {np.abs(fr - f64).max():.2e}, "
+          f"height {np.abs(ht - h64).max():.2e}")
+    assert fr[-1] > fr[0] + (0.5 if dim == 2 else 0.05)  # the surge has moved
+    np.testing.assert_allclose(fr, f64, rtol=0, atol=1e-4)
+    np.testing.assert_allclose(ht, h64, rtol=0, atol=1e-4)

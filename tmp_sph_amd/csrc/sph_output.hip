@@ -249,7 +249,7 @@ extern "C" int sph_sim_restore(const char* path, sph_sim** out) {
   a.each([&](const void* p, size_t len) { sum.add(p, len); });
   if (sum.h != want) return set_error(SPH_E_INVAL, (std::string("sph_sim_restore: ") + path + ": checksum mismatch").c_str());
   for (size_t i = 0; i < n; ++i)
-    if (a.ID[i] < 0 || a.ID[i] >= h.n || (a.TY[i] >> 8) >= h.K)
+    if (a.ID[i] < 0 || a.ID[i] >= h.n || a.TY[i] < 0 || (a.TY[i] >> 8) >= h.K)
       return set_error(SPH_E_INVAL, "sph_sim_restore: corrupt particle arrays");
   // Build a context of the right shape (its upload is overwritten below),
   // then install the saved sorted state and step count.
