@@ -450,18 +450,29 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
   const int sx = li >> 2;
   const int ntiles = (g.ne + 31) / 32;
   const int stride = gridDim.x * EW;
-  for (int tile = blockIdx.x * EW + w; tile < ntiles; tile += stride) {  // wave-uniform: every wave exits
-    const int m0 = tile * 32;
-    f32x4 acc[2][8];
+  // receiver / sender ids of this lane's rows 16 t + 4 lq + r of a tile
+  auto load_ids = [&](int tl, int (&ri)[2][4], int (&si)[2][4]) {
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int c = 0; c < 8; ++c) acc[t][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // layer 1 (E block of W1); A = E rows in the FP32 k order
+      for (int r = 0; r < 4; ++r) {
+        const int m = min(tl * 32 + 16 * t + 4 * lq + r, g.ne - 1);
+        ri[t][r] = g.recv[m];
+        si[t][r] = g.send[m];
+      }
+  };
+  int ri[2][4], si[2][4];
+  load_ids(min(blockIdx.x * EW + w, ntiles - 1), ri, si);
+  for (int tile = blockIdx.x * EW + w; tile < ntiles; tile += stride) {  // wave-uniform: every wave exits
+    const int m0 = tile * 32;
+    // one memory round trip per tile: the edge latents (layer-1 operand) and
+    // the node projections, whose sum P[recv] + P[send] seeds the
+    // accumulators (ids loaded during the previous tile); projection columns
+    // 8 li .. 8 li + 7 hold this lane's features 16 c + li
+    uint4 ea[4][2][2];
     {
       const float* pa = g.E + (size_t)min(m0 + li, g.ne - 1) * 128 + 4 * lq;
       const float* pb = g.E + (size_t)min(m0 + 16 + li, g.ne - 1) * 128 + 4 * lq;
-      uint4 ea[4][2][2];
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) {
         ea[s4][0][0] = *reinterpret_cast<const uint4*>(pa + 32 * s4);
@@ -469,56 +480,59 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
         ea[s4][1][0] = *reinterpret_cast<const uint4*>(pb + 32 * s4);
         ea[s4][1][1] = *reinterpret_cast<const uint4*>(pb + 32 * s4 + 16);
       }
-#pragma unroll
-      for (int s4 = 0; s4 < 4; ++s4) {
-        bf16x8 af[2];
-        af[0] = cvt_raw(ea[s4][0][0], ea[s4][0][1]);
-        af[1] = cvt_raw(ea[s4][1][0], ea[s4][1][1]);
-        const bf16* wb = &Wl[0][wlane + 32 * (s4 ^ sx)];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const bf16x8 bf = *reinterpret_cast<const bf16x8*>(wb + 2048 * c);
-          acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf, acc[0][c], 0, 0, 0);
-          acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf, acc[1][c], 0, 0, 0);
-        }
-      }
     }
-    // + bias + receiver and sender projections, ReLU, into H
-    // (all gathers of a row half are issued before any is used)
-    int ri[2][4], si[2][4];
+    f32x4 acc[2][8];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = min(m0 + 16 * t + 4 * lq + r, g.ne - 1);
-        ri[t][r] = g.recv[m];
-        si[t][r] = g.send[m];
-      }
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      // projection columns 8 li .. 8 li + 7 hold this lane's features 16 c + li
-      float pv[4][8];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const float4* pr = reinterpret_cast<const float4*>(g.P + (size_t)ri[t][r] * 256 + 8 * li);
         const float4* ps = reinterpret_cast<const float4*>(g.P + (size_t)si[t][r] * 256 + 128 + 8 * li);
         const float4 r0 = pr[0], r1 = pr[1], s0 = ps[0], s1 = ps[1];
-        pv[r][0] = r0.x + s0.x;
-        pv[r][1] = r0.y + s0.y;
-        pv[r][2] = r0.z + s0.z;
-        pv[r][3] = r0.w + s0.w;
-        pv[r][4] = r1.x + s1.x;
-        pv[r][5] = r1.y + s1.y;
-        pv[r][6] = r1.z + s1.z;
-        pv[r][7] = r1.w + s1.w;
+        acc[t][0][r] = r0.x + s0.x;
+        acc[t][1][r] = r0.y + s0.y;
+        acc[t][2][r] = r0.z + s0.z;
+        acc[t][3][r] = r0.w + s0.w;
+        acc[t][4][r] = r1.x + s1.x;
+        acc[t][5][r] = r1.y + s1.y;
+        acc[t][6][r] = r1.z + s1.z;
+        acc[t][7][r] = r1.w + s1.w;
       }
+    // layer 1 (E block of W1); A = E rows in the FP32 k order
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+    for (int s4 = 0; s4 < 4; ++s4) {
+      bf16x8 af[2];
+      af[0] = cvt_raw(ea[s4][0][0], ea[s4][0][1]);
+      af[1] = cvt_raw(ea[s4][1][0], ea[s4][1][1]);
+      const bf16* wb = &Wl[0][wlane + 32 * (s4 ^ sx)];
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-          H[(16 * t + 4 * lq + r) * LDH + 16 * c + li] =
-              (bf16)fmaxf(acc[t][c][r] + g.b1[16 * c + li] + pv[r][c], 0.0f);
+      for (int c = 0; c < 8; ++c) {
+        const bf16x8 bf = *reinterpret_cast<const bf16x8*>(wb + 2048 * c);
+        acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[0], bf, acc[0][c], 0, 0, 0);
+        acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[1], bf, acc[1][c], 0, 0, 0);
+      }
     }
+    // + bias, ReLU, into H
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float b = g.b1[16 * c + li];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          H[(16 * t + 4 * lq + r) * LDH + 16 * c + li] = (bf16)fmaxf(acc[t][c][r] + b, 0.0f);
+      }
+    // the residual rows (row-contiguous, 32 lanes per row) and the next
+    // tile's ids load during layers 2 and 3
+    const int c4 = lane & 31;
+    float4 rr[2][8];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        rr[t][j] = *reinterpret_cast<const float4*>(
+            g.E + (size_t)min(m0 + 16 * t + 2 * j + (lane >> 5), g.ne - 1) * 128 + 4 * c4);
+    load_ids(min(tile + stride, ntiles - 1), ri, si);
     // layers 2 and 3 (A = H, rows 16 t + li at k 32 s + 8 lq)
 #pragma unroll
     for (int L = 1; L < 3; ++L) {
@@ -563,20 +577,14 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) F[(4 * lq + r) * LDF + 16 * c + li] = acc[t][c][r] + b;
       }
-      float4 v[8], rr[8];
-      const int c4 = lane & 31;  // 32 lanes per row, rows 2 j + lane / 32
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int row = 2 * j + (lane >> 5);
-        v[j] = *reinterpret_cast<const float4*>(&F[row * LDF + 4 * c4]);
-        rr[j] = *reinterpret_cast<const float4*>(g.E + (size_t)min(m0 + 16 * t + row, g.ne - 1) * 128 + 4 * c4);
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int m = m0 + 16 * t + 2 * j + (lane >> 5);
+        const int row = 2 * j + (lane >> 5), m = m0 + 16 * t + row;
+        const float4 v = *reinterpret_cast<const float4*>(&F[row * LDF + 4 * c4]);
+        const float4 e = rr[t][j];
         if (m < g.ne)
           *reinterpret_cast<float4*>(g.E + (size_t)m * 128 + 4 * c4) =
-              make_float4(v[j].x + rr[j].x, v[j].y + rr[j].y, v[j].z + rr[j].z, v[j].w + rr[j].w);
+              make_float4(v.x + e.x, v.y + e.y, v.z + e.z, v.w + e.w);
       }
     }
   }
