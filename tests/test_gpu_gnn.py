@@ -163,9 +163,14 @@ def test_split_edge_kernel_matches_the_concatenated_first_layer(gpu):
         g.set_fused(mode)
         out[mode], _ = g.forward(want_out=True)
     rel = np.linalg.norm(out[8] - out[2]) / np.linalg.norm(out[2])
-    print(f"split vs exact first layer: output rel diff {rel:.2e}")
+    # per node, against the output scale: a lost or doubled message (the
+    # persistent kernel also aggregates the messages, with a fix-up pass for
+    # receivers cut between two waves' edge chunks) would be an O(1) error
+    worst = float(np.abs(out[8] - out[2]).max() / np.abs(out[2]).max())
+    print(f"split vs exact first layer: output rel diff {rel:.2e}, worst node {worst:.2e}")
     assert np.all(np.isfinite(out[8]))
     assert rel < 2e-2, rel
+    assert worst < 5e-2, worst
     g.close()
 
 

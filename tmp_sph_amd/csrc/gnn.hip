@@ -483,10 +483,11 @@ hipError_t mlp_backward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, con
   return gemm(b, g->stream);
 }
 
-// Edge update of an inference context in place: E += MLP_e([E, V[recv], V[send]]),
-// the first layer's receiver and sender blocks applied to the nodes first
-// (P = V W1r^T | V W1s^T), then one persistent kernel per edge set.
-hipError_t edge_forward_split(sph_gnn* g, int l, const float* V, float* E) {
+// Edge update of an inference context in place, E += MLP_e([E, V[recv], V[send]]),
+// and the aggregation agg = segment sums of the new E by receiver: the first
+// layer's receiver and sender blocks applied to the nodes first (P = V W1r^T |
+// V W1s^T), then one persistent kernel per edge set.
+hipError_t edge_forward_split(sph_gnn* g, int l, const float* V, float* E, float* agg) {
   const Mlp& m = g->pe[l];
   for (int half = 0; half < 2; ++half) {
     GemmArgs a{};
@@ -514,6 +515,9 @@ hipError_t edge_forward_split(sph_gnn* g, int l, const float* V, float* E) {
   e.b1 = g->P + m.l[0].b;
   e.b2 = g->P + m.l[1].b;
   e.b3 = g->P + m.l[2].b;
+  e.agg = agg;
+  e.off = g->off;
+  e.nodes = g->n;
   return edge_mlp(e, g->stream);
 }
 
@@ -544,13 +548,13 @@ hipError_t forward(sph_gnn* g) {
     float* Vl1 = V_of(g, tr ? l + 1 : 0);
     float* Al = A_of(g, l);
     if (g->fused == 8 && !tr) {
-      HIPCHK(edge_forward_split(g, l, Vl, El));
+      HIPCHK(edge_forward_split(g, l, Vl, El, Al));  // also aggregates into Al
     } else {
       AOperand ein = op3(segf(El, W, W), segf(Vl, W, W, g->recv), segf(Vl, W, W, g->send));
       HIPCHK(mlp_forward(g, g->pe[l], ein, ne, h1_of(g, call), h2_of(g, call), El1, El));
+      HIPCHK(segment_sum(El1, W, g->off, nullptr, n, Al, W, 0, g->stream));
     }
     ++call;
-    HIPCHK(segment_sum(El1, W, g->off, nullptr, n, Al, W, 0, g->stream));
     AOperand nin = op2(segf(Vl, W, W), segf(Al, W, W));
     HIPCHK(mlp_forward(g, g->pn[l], nin, n, h1_of(g, call), h2_of(g, call), Vl1, Vl));
     ++call;

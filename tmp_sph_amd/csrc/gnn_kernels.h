@@ -48,9 +48,9 @@ struct GemmArgs {
 
 hipError_t gemm(const GemmArgs& g, hipStream_t s);
 
-// Inference form of an edge-update MLP call, persistent (one block of 8
-// waves per CU, the three weight matrices resident in LDS, no barriers after
-// staging them): E[e] += L3(relu(L2(relu(E[e] W1e^T + P[recv[e]][0:128] +
+// Inference form of an edge-update MLP call and the aggregation that follows
+// it, persistent (one block of 7 waves per CU, the three weight matrices
+// resident in LDS, no barriers after staging them): E[e] += L3(relu(L2(relu(E[e] W1e^T + P[recv[e]][0:128] +
 // P[send[e]][128:256] + b1)))), where P (nodes x 256) holds the node
 // projections V W1r^T | V W1s^T of the first layer's receiver and sender
 // blocks (W1 = [W1e | W1r | W1s], 128 x 384, row stride ldw1) with columns in
@@ -69,6 +69,13 @@ struct EdgeMlpArgs {
   const float* b1;
   const float* b2;
   const float* b3;
+  // the message aggregation agg[i] = sum over edges [off[i], off[i+1]) of
+  // the new E rows (edges ordered by receiver), in edge order, bit-identical
+  // to segment_sum
+  float* agg;
+  const int* off;
+  int nodes;
+  int chunk;  // tiles per wave (set by edge_mlp)
 };
 
 hipError_t edge_mlp(const EdgeMlpArgs& g, hipStream_t s);

@@ -449,7 +449,17 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
   const int wlane = li * 128 + 8 * (lq ^ (li & 3));
   const int sx = li >> 2;
   const int ntiles = (g.ne + 31) / 32;
-  const int stride = gridDim.x * EW;
+  // each wave walks a contiguous chunk of tiles (g.chunk of them), so the
+  // aggregation below can carry per-receiver sums from tile to tile
+  const int t_begin = (blockIdx.x * EW + w) * g.chunk;
+  const int t_end = min(t_begin + g.chunk, ntiles);
+  if (t_begin >= t_end) return;  // wave-uniform; no barrier follows
+  const int row0 = t_begin * 32, row1 = min(t_end * 32, g.ne);
+  // receivers cut by this chunk's ends are summed by agg_fixup_kernel
+  const int cut_first = (row0 > 0 && g.recv[row0 - 1] == g.recv[row0]) ? g.recv[row0] : -1;
+  const int cut_last = (row1 < g.ne && g.recv[row1] == g.recv[row1 - 1]) ? g.recv[row1 - 1] : -1;
+  int cur = g.recv[row0];  // receiver of the running sum
+  float2 run = make_float2(0.0f, 0.0f);
   // receiver / sender ids of this lane's rows 16 t + 4 lq + r of a tile
   auto load_ids = [&](int tl, int (&ri)[2][4], int (&si)[2][4]) {
 #pragma unroll
@@ -462,9 +472,10 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
       }
   };
   int ri[2][4], si[2][4];
-  load_ids(min(blockIdx.x * EW + w, ntiles - 1), ri, si);
-  for (int tile = blockIdx.x * EW + w; tile < ntiles; tile += stride) {  // wave-uniform: every wave exits
+  load_ids(t_begin, ri, si);
+  for (int tile = t_begin; tile < t_end; ++tile) {  // wave-uniform: every wave exits
     const int m0 = tile * 32;
+    const int rvec = g.recv[min(m0 + (lane & 31), g.ne - 1)];  // lane k < 32: receiver of row m0 + k
     // one memory round trip per tile: the edge latents (layer-1 operand) and
     // the node projections, whose sum P[recv] + P[send] seeds the
     // accumulators (ids loaded during the previous tile); projection columns
@@ -532,7 +543,7 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
       for (int j = 0; j < 8; ++j)
         rr[t][j] = *reinterpret_cast<const float4*>(
             g.E + (size_t)min(m0 + 16 * t + 2 * j + (lane >> 5), g.ne - 1) * 128 + 4 * c4);
-    load_ids(min(tile + stride, ntiles - 1), ri, si);
+    load_ids(min(tile + 1, ntiles - 1), ri, si);
     // layers 2 and 3 (A = H, rows 16 t + li at k 32 s + 8 lq)
 #pragma unroll
     for (int L = 1; L < 3; ++L) {
@@ -582,12 +593,50 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
         const int row = 2 * j + (lane >> 5), m = m0 + 16 * t + row;
         const float4 v = *reinterpret_cast<const float4*>(&F[row * LDF + 4 * c4]);
         const float4 e = rr[t][j];
-        if (m < g.ne)
-          *reinterpret_cast<float4*>(g.E + (size_t)m * 128 + 4 * c4) =
-              make_float4(v.x + e.x, v.y + e.y, v.z + e.z, v.w + e.w);
+        const float4 o = make_float4(v.x + e.x, v.y + e.y, v.z + e.z, v.w + e.w);
+        *reinterpret_cast<float4*>(&F[row * LDF + 4 * c4]) = o;  // kept for the aggregation
+        if (m < g.ne) *reinterpret_cast<float4*>(g.E + (size_t)m * 128 + 4 * c4) = o;
+      }
+      // aggregation A[recv] = sum of the new latents of the receiver's edges,
+      // in edge order as segment_sum_kernel adds them: lane l owns columns
+      // 2 l, 2 l + 1; a receiver's sum is written when its last edge is seen
+#pragma unroll
+      for (int row = 0; row < 16; ++row) {
+        const int m = m0 + 16 * t + row;
+        if (m >= row1) break;  // wave-uniform
+        const int rid = __builtin_amdgcn_readlane(rvec, 16 * t + row);
+        if (rid != cur) {
+          if (cur != cut_first && cur != cut_last)
+            *reinterpret_cast<float2*>(g.agg + (size_t)cur * 128 + 2 * lane) = run;
+          cur = rid;
+          run = make_float2(0.0f, 0.0f);
+        }
+        const float2 v = *reinterpret_cast<const float2*>(&F[row * LDF + 2 * lane]);
+        run.x += v.x;
+        run.y += v.y;
       }
     }
   }
+  if (cur != cut_first && cur != cut_last) *reinterpret_cast<float2*>(g.agg + (size_t)cur * 128 + 2 * lane) = run;
+}
+
+// Completes the aggregation of edge_mlp_kernel: receivers without edges get
+// zero, receivers whose edges straddle two waves' chunks (chunk_rows edges
+// each) are summed here in edge order; the others were written by the kernel.
+__global__ void agg_fixup_kernel(const float* __restrict__ E, const int* __restrict__ off, int nodes, int chunk_rows,
+                                 float* __restrict__ agg) {
+  const int node = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (node >= nodes) return;
+  const int b = off[node], e = off[node + 1];
+  if (e > b && b / chunk_rows == (e - 1) / chunk_rows) return;
+  float s0 = 0.0f, s1 = 0.0f;
+  for (int k = b; k < e; ++k) {
+    const float2 v = *reinterpret_cast<const float2*>(E + (size_t)k * 128 + 2 * lane);
+    s0 += v.x;
+    s1 += v.y;
+  }
+  *reinterpret_cast<float2*>(agg + (size_t)node * 128 + 2 * lane) = make_float2(s0, s1);
 }
 
 __global__ void transpose_kernel(AOperand A, int R, int C, bf16* __restrict__ out, int ldo) {
@@ -814,8 +863,14 @@ hipError_t mlp3(const Mlp3Args& g, hipStream_t s) {
 }
 
 hipError_t edge_mlp(const EdgeMlpArgs& g, hipStream_t s) {
-  if (g.ne <= 0) return hipSuccess;
-  if (!g.E || !g.P || !g.recv || !g.send || !g.W1 || !g.W2 || !g.W3 || g.ldw1 < 128) return hipErrorInvalidValue;
+  if (g.ne <= 0) {  // no edges: every aggregate is zero
+    if (g.nodes > 0 && g.agg && g.off)
+      hipLaunchKernelGGL(agg_fixup_kernel, dim3(cdiv(g.nodes, 4)), dim3(256), 0, s, g.E, g.off, g.nodes, 1, g.agg);
+    return hipGetLastError();
+  }
+  if (!g.E || !g.P || !g.recv || !g.send || !g.W1 || !g.W2 || !g.W3 || g.ldw1 < 128 || !g.agg || !g.off ||
+      g.nodes <= 0)
+    return hipErrorInvalidValue;
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
@@ -825,7 +880,10 @@ hipError_t edge_mlp(const EdgeMlpArgs& g, hipStream_t s) {
   }
   const int tiles = (g.ne + 31) / 32;
   const int blocks = std::min(cus, (tiles + EW - 1) / EW);
-  hipLaunchKernelGGL(edge_mlp_kernel, dim3(blocks), dim3(64 * EW), 0, s, g);
+  EdgeMlpArgs a = g;
+  a.chunk = (tiles + blocks * EW - 1) / (blocks * EW);
+  hipLaunchKernelGGL(edge_mlp_kernel, dim3(blocks), dim3(64 * EW), 0, s, a);
+  hipLaunchKernelGGL(agg_fixup_kernel, dim3(cdiv(g.nodes, 4)), dim3(256), 0, s, g.E, g.off, g.nodes, 32 * a.chunk, g.agg);
   return hipGetLastError();
 }
 
