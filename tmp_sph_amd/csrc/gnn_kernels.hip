@@ -871,13 +871,10 @@ hipError_t edge_mlp(const EdgeMlpArgs& g, hipStream_t s) {
   if (!g.E || !g.P || !g.recv || !g.send || !g.W1 || !g.W2 || !g.W3 || g.ldw1 < 128 || !g.agg || !g.off ||
       g.nodes <= 0)
     return hipErrorInvalidValue;
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-      cus = 256;
-  }
+  int dev = 0, cus = 0;  // one persistent block per CU of the current device
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
   const int tiles = (g.ne + 31) / 32;
   const int blocks = std::min(cus, (tiles + EW - 1) / EW);
   EdgeMlpArgs a = g;
