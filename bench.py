@@ -312,8 +312,8 @@ def main() -> int:
     if args.config == "surrogate":
         return surrogate_bench(args)
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}): this build has no multi-GPU path; "
-              "the slab decomposition it would use is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): there is no multi-GPU path; "
+              "the slab decomposition it would use is described in the design notes", file=sys.stderr)
         return 2
 
     from tmp_sph_amd import sph

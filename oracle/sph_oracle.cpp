@@ -19,7 +19,8 @@
 //     (the reference reads past cellStart there, src/sph/sph_run.cu:92);
 //   * cell coordinates are clamped into the grid;
 //   * 2-D mode (plane x-z, 3 stencil rows), Tait EOS, FP64 positions and the
-//     FP64 mode have no reference counterpart (BASELINE configurations).
+//     FP64 mode have no reference counterpart (they serve the 2-D and Tait
+//     dam-break cases).
 //
 // Precision modes of orc_run / orc_eval:
 //   0  FP32 everywhere, float positions, reference increments  (reference step)
