@@ -266,7 +266,8 @@ def learned_bench(args) -> int:
         "data": "synthetic: trained on plain-solver trajectories of 12 sweep members (dx = 0.004 m)",
         "config": {"workload": "held-out 2-D dam break, 0.8x0.5 m column in 4.0x1.0 m tank, dx=0.002 m",
                    "particles_total": int(case.n), "particles_fluid": int(case.n_fluid), "k_steps_per_net_step": k,
-                   "layers": layers, "latent": 128, "fallback_density_deviation": 0.01},
+                   "layers": layers, "latent": 128, "fallback_density_deviation": 0.01,
+                   "prediction_clamp_sigma": gi.cfg.out_clip},
         "front_rel_error_max": res["front_rel_error_max"],
         "front_within_2pct": bool(res["front_rel_error_max"] <= 0.02),
         "evaluation": res,

@@ -361,6 +361,8 @@ typedef struct sph_gnn_cfg {
     float out_mean[3];   /* rate = normalised prediction * scale + out_mean (dvx/dt, dvz/dt, drho/dt);
                             the change over k steps is rate * k * dt, so one network serves
                             cases of different resolution (dt) */
+    float out_clip;      /* rollouts clamp normalised predictions to [-out_clip, out_clip]
+                            (the range the training targets were clipped to); <= 0: no clamp */
 } sph_gnn_cfg;
 
 typedef struct sph_rollout_stats {

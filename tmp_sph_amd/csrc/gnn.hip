@@ -864,7 +864,10 @@ __global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* 
   const float vx = vel[3 * i], vz = vel[3 * i + 2];
   float nx = x, nz = z, nvx = vx, nvz = vz, nr = rho[i];
   if (type[i] == 1) {
-    const float* y = Y + (size_t)i * W;
+    const float* yp = Y + (size_t)i * W;
+    float y[3] = {yp[0], yp[1], yp[2]};
+    if (cfg.out_clip > 0.0f)
+      for (int c = 0; c < 3; ++c) y[c] = fminf(fmaxf(y[c], -cfg.out_clip), cfg.out_clip);
     const float T = (float)k * P.timeStep;
     nvx = vx + (y[0] * cfg.dv_scale + cfg.out_mean[0]) * T;
     nvz = vz + (y[1] * cfg.dv_scale + cfg.out_mean[1]) * T;

@@ -26,7 +26,8 @@ W = 128
 class GnnCfg(C.Structure):
     _fields_ = [("layers", C.c_int), ("history", C.c_int), ("max_nodes", C.c_int), ("max_edges", C.c_int),
                 ("train", C.c_int), ("seed", C.c_uint), ("v_scale", C.c_float), ("dv_scale", C.c_float),
-                ("drho_scale", C.c_float), ("rho_scale", C.c_float), ("out_mean", C.c_float * 3)]
+                ("drho_scale", C.c_float), ("rho_scale", C.c_float), ("out_mean", C.c_float * 3),
+                ("out_clip", C.c_float)]
 
 
 class RolloutStats(C.Structure):
@@ -56,10 +57,11 @@ class Gnn:
     """The surrogate network on the current device."""
 
     def __init__(self, layers=10, history=5, max_nodes=60_000, max_edges=1_500_000, train=True, seed=1,
-                 v_scale=1.0, dv_scale=0.1, drho_scale=1.0, rho_scale=100.0, out_mean=(0.0, 0.0, 0.0)):
+                 v_scale=1.0, dv_scale=0.1, drho_scale=1.0, rho_scale=100.0, out_mean=(0.0, 0.0, 0.0),
+                 out_clip=10.0):
         load()
         self.cfg = GnnCfg(layers, history, max_nodes, max_edges, 1 if train else 0, seed, v_scale, dv_scale,
-                          drho_scale, rho_scale, (C.c_float * 3)(*out_mean))
+                          drho_scale, rho_scale, (C.c_float * 3)(*out_mean), out_clip)
         h = C.c_void_p()
         call("sph_gnn_create", C.byref(self.cfg), C.byref(h))
         self._h = h
