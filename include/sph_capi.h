@@ -372,6 +372,9 @@ typedef struct sph_rollout_stats {
     double max_dev;      /* largest density deviation the judge measured */
     double mean_dev;     /* mean over judged intervals */
     double seconds;      /* wall-clock of the rollout */
+    int rejected_penetration; /* network steps rejected for wall penetration (a fluid particle
+                                 predicted outside the escape box, where the solver would park it) */
+    double max_penetration;   /* deepest predicted position beyond the escape box seen by the judge (m) */
 } sph_rollout_stats;
 
 /* Advances the single-instance 2-D context `sim` by `intervals` intervals of
@@ -379,11 +382,12 @@ typedef struct sph_rollout_stats {
    method: the SPH summation density (the kernel sum of kernel A's Wendland
    weights) of the predicted positions must change like the predicted density
    -- mean |d rho_sum - d rho_pred| / rho0 over fluid particles with a full
-   support <= dev_tol -- and no fluid particle may pass a tank face by more
-   than half a particle spacing (into the wall layer); otherwise the interval
-   is redone with k plain steps.  The first `history` intervals
-   are plain (they fill the velocity history).  front/height (may be null)
-   receive the probes after every interval. */
+   support <= dev_tol -- and no fluid particle may be predicted outside the
+   escape box (the tank faces; the solver would park it); otherwise the
+   interval is redone with k plain steps.  Particles the solver has already
+   parked are left where they are and out of the judgement.  The first
+   `history` intervals are plain (they fill the velocity history).
+   front/height (may be null) receive the probes after every interval. */
 int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, float dev_tol, double band,
                     double* front, double* height, sph_rollout_stats* stats);
 

@@ -854,6 +854,13 @@ __device__ __forceinline__ float wendland2d(const sph_params& P, float rr) {
 // Applies the prediction (fluid only): with T = k dt and the predicted rates,
 // v' = v + T dv/dt, x' = x + T (v + v')/2, rho' = max(rho0, rho + T drho/dt),
 // p' = EOS(rho'); walls keep their state.
+// Outside the escape box in the x-z plane: the solver parks such a fluid
+// particle (below the floor, inside the wall layers) and leaves it there.
+__device__ __forceinline__ bool outside_xz(const sph_params& P, float x, float z) {
+  return (double)x > P.domain_max[0] || (double)x < P.domain_min[0] || (double)z > P.domain_max[2] ||
+         (double)z < P.domain_min[2];
+}
+
 __global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* __restrict__ Y,
                              const float* __restrict__ pos, const float* __restrict__ vel,
                              const float* __restrict__ rho, const int* __restrict__ type, float* __restrict__ pos2,
@@ -863,7 +870,7 @@ __global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* 
   const float x = pos[3 * i], z = pos[3 * i + 2];
   const float vx = vel[3 * i], vz = vel[3 * i + 2];
   float nx = x, nz = z, nvx = vx, nvz = vz, nr = rho[i];
-  if (type[i] == 1) {
+  if (type[i] == 1 && !outside_xz(P, x, z)) {  // parked particles stay where they are
     const float* yp = Y + (size_t)i * W;
     float y[3] = {yp[0], yp[1], yp[2]};
     if (cfg.out_clip > 0.0f)
@@ -888,19 +895,23 @@ __global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* 
 // Judge: for fluid particles with a full support (degree >= full_deg),
 // |(rho_sum(x') - rho_sum(x)) - (rho' - rho)| / rho0, where rho_sum is the
 // kernel sum m sum_j W_ij over the graph (self included).  Also flags fluid
-// that left the tank.  out: [0] sum of deviations, [1] count, [2] max, [3] escaped
+// predicted outside the escape box (particles already parked there are left
+// out).  out: [0] sum of deviations, [1] count, [2] max, [3] escaped,
+// [4] deepest position beyond the box
 __global__ void judge_kernel(sph_params P, const int* __restrict__ off, const int* __restrict__ send,
                              const float* __restrict__ pos, const float* __restrict__ pos2,
                              const float* __restrict__ rho, const float* __restrict__ rho2,
                              const int* __restrict__ type, int full_deg, float* __restrict__ out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  float dev = 0.0f, cnt = 0.0f, esc = 0.0f;
-  if (i < n && type[i] == 1) {
+  float dev = 0.0f, cnt = 0.0f, esc = 0.0f, depth = 0.0f;
+  if (i < n && type[i] == 1 && !outside_xz(P, pos[3 * i], pos[3 * i + 2])) {
     const float x2 = pos2[3 * i], z2 = pos2[3 * i + 2];
-    // penetration deeper than half a particle spacing into a wall layer
-    const double tol = 0.5 * (double)P.particleR;
-    if ((double)x2 < P.domain_min[0] - tol || (double)x2 > P.domain_max[0] + tol || (double)z2 < P.domain_min[2] - tol)
-      esc = 1.0f;
+    // wall penetration: a prediction that leaves the escape box (the solver
+    // would park the particle), recorded with its depth beyond the box
+    const double d = fmax(fmax(P.domain_min[0] - (double)x2, (double)x2 - P.domain_max[0]),
+                          fmax(fmax(P.domain_min[2] - (double)z2, (double)z2 - P.domain_max[2]), 0.0));
+    depth = (float)d;
+    if (outside_xz(P, x2, z2) || d != d) esc = 1.0f;
     const int b = off[i], e = off[i + 1];
     if (e - b >= full_deg) {
       const float W0 = wendland2d(P, 0.0f);
@@ -923,12 +934,14 @@ __global__ void judge_kernel(sph_params P, const int* __restrict__ off, const in
     cnt += __shfl_xor(cnt, o);
     esc += __shfl_xor(esc, o);
     mx = fmaxf(mx, __shfl_xor(mx, o));
+    depth = fmaxf(depth, __shfl_xor(depth, o));
   }
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(&out[0], dev);
     atomicAdd(&out[1], cnt);
     atomicMax(reinterpret_cast<int*>(&out[2]), __float_as_int(mx));  // non-negative floats order as ints
     atomicAdd(&out[3], esc);
+    atomicMax(reinterpret_cast<int*>(&out[4]), __float_as_int(depth));
   }
 }
 
@@ -959,7 +972,7 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, f
   if (he == hipSuccess) he = hipMalloc(&vel2, 3 * N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&rho2, N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&p2, N * sizeof(float));
-  if (he == hipSuccess) he = hipMalloc(&judge, 4 * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&judge, 5 * sizeof(float));
   if (he != hipSuccess) {
     hipFree(vel); hipFree(prs); hipFree(pos2); hipFree(vel2); hipFree(rho2); hipFree(p2); hipFree(judge);
     return hip_status(he, "sph_gnn_rollout: scratch");
@@ -993,10 +1006,10 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, f
       if ((he = forward(g)) != hipSuccess) { rc = hip_status(he, "rollout forward"); break; }
       hipLaunchKernelGGL(apply_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->cfg, k, g->Y, g->pos, vel, g->rho,
                          g->type, pos2, vel2, rho2, p2, n);
-      hipMemsetAsync(judge, 0, 4 * sizeof(float), s);
+      hipMemsetAsync(judge, 0, 5 * sizeof(float), s);
       hipLaunchKernelGGL(judge_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->off, g->send, g->pos, pos2, g->rho, rho2,
                          g->type, full_deg, judge, n);
-      float jh[4];
+      float jh[5];
       he = hipMemcpyAsync(jh, judge, sizeof(jh), hipMemcpyDeviceToHost, s);
       if (he == hipSuccess) he = hipStreamSynchronize(s);
       if (he != hipSuccess) { rc = hip_status(he, "rollout judge"); break; }
@@ -1004,6 +1017,8 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, f
       dev_sum += dev;
       ++judged;
       st.max_dev = std::max(st.max_dev, dev);
+      st.max_penetration = std::max(st.max_penetration, (double)jh[4]);
+      if (jh[3] != 0.0f) ++st.rejected_penetration;
       if (dev <= dev_tol && jh[3] == 0.0f) {
         rc = sph_sim_set_state_device(sim, pos2, vel2, rho2, p2, g->type);
         if (rc) break;
