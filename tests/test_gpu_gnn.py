@@ -197,6 +197,25 @@ def test_rollout_fallback_reproduces_plain_run(gpu):
     g.close()
 
 
+def test_parked_particle_does_not_block_the_surrogate(gpu):
+    """A fluid particle the solver has parked (outside the escape box, below
+    the floor) stays parked through surrogate intervals and does not count as
+    a wall penetration, so the network steps are still accepted."""
+    from tmp_sph_amd import sph
+    from tmp_sph_amd.surrogate import Gnn
+    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.02)).copy()
+    i = int(np.flatnonzero(case.type == 1)[0])
+    park = np.array([case.params.park_pos[0], 0.0, case.params.park_pos[2]], np.float32)
+    case.pos[i] = park
+    k, intervals = 5, 9
+    g = Gnn(layers=2, max_nodes=4096, max_edges=120_000, train=False, seed=2)
+    with sph.Simulation(case) as s:
+        st, _, _ = g.rollout(s, k, intervals, dev_tol=1e30)
+        assert st["rejected_penetration"] == 0 and st["accepted"] == intervals - 4, st
+        np.testing.assert_array_equal(s.state()["pos"][i], park)
+    g.close()
+
+
 def test_training_reduces_loss_and_beats_no_network_baseline(gpu):
     """Trained on plain-solver snapshots, the network predicts the next-k-step
     velocity change on held-out snapshots better than the best constant
