@@ -152,10 +152,11 @@ def surrogate_bench(args) -> int:
     ne = int(send.size)
     flops = sg.forward_flops(layers, case.n, ne)
     res = {}
-    for depth in (8, 2, 0):
+    for depth in (9, 8, 2, 0):
         g.set_fused(depth)
         dt = timed(lambda: g.forward(want_out=False), reps)
-        name = {8: "persistent_split", 0: "three_gemms"}.get(depth, f"fused_depth{depth}")
+        name = {9: "persistent_split_bf16_latents", 8: "persistent_split", 0: "three_gemms"}.get(
+            depth, f"fused_depth{depth}")
         res[name] = {"ms": 1e3 * dt, "tflops": flops / dt / 1e12}
         print(f"# forward ({name}): {1e3 * dt:.2f} ms, {flops / dt / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
     g.close()
@@ -177,7 +178,7 @@ def surrogate_bench(args) -> int:
     print(f"# training step: {1e3 * dtt:.2f} ms (~{tflops_train:.1f} TFLOP/s counting 3x forward flops)",
           file=sys.stderr, flush=True)
     gt.close()
-    best = res["persistent_split"]
+    best = res["persistent_split_bf16_latents"]
     out = {
         "metric": "surrogate forward throughput (edges x layers / s)",
         "value": ne * layers / (best["ms"] * 1e-3),

@@ -434,12 +434,15 @@ int sph_gnn_adam(sph_gnn* g, float lr, float clip);
 /* Forward MLP calls as one fused kernel each (fused = 1, 2 or 4: hidden
    activations stay in LDS, operand loads run `fused` k-steps ahead of the
    MFMAs; the default with train = 1) or as three GEMM launches (fused = 0);
-   these settings give bit-identical results.  fused = 8 (train = 0 only, the
-   default there): the edge MLPs apply their first layer's receiver and
-   sender blocks to the nodes first and run as one persistent kernel with the
-   weights resident in LDS -- the same products summed in another order, so
-   outputs agree with the other settings to FP32 rounding (and the BF16
-   rounding of hidden activations that it can flip). */
+   these settings give bit-identical results.  fused = 8 (train = 0 only):
+   the edge MLPs apply their first layer's receiver and sender blocks to the
+   nodes first and run as one persistent kernel with the weights resident in
+   LDS, which also aggregates the messages -- the same products summed in
+   another order, so outputs agree with the other settings to FP32 rounding
+   (and the BF16 rounding of hidden activations that it can flip).  fused = 9
+   (train = 0 only, the default there): as 8 with the edge latents stored as
+   BF16 between blocks (halving their HBM traffic; outputs agree with 8 to
+   about 1e-2). */
 int sph_gnn_set_fused(sph_gnn* g, int fused);
 /* One MFMA tile D(16x16) = A(16x32) B(32x16) in BF16 (host arrays as raw
    uint16 BF16 bits), for checking the operand layout. */

@@ -174,6 +174,29 @@ def test_split_edge_kernel_matches_the_concatenated_first_layer(gpu):
     g.close()
 
 
+def test_bf16_edge_latents_stay_close_to_fp32_latents(gpu):
+    """Mode 9 stores the edge latents as BF16 between blocks (the residual
+    sum is formed in FP32 and rounded once per block): outputs agree with the
+    FP32-latent persistent kernel to BF16 rounding, far inside the network's
+    own error."""
+    from tmp_sph_amd import sph
+    from tmp_sph_amd.surrogate import Gnn
+    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.02))
+    hist, _ = _state(case)
+    g = Gnn(layers=10, max_nodes=case.n, max_edges=40 * case.n, seed=9, train=False)
+    g.load(case.params, case.pos, hist, case.rho, case.type)
+    out = {}
+    for mode in (8, 9):
+        g.set_fused(mode)
+        out[mode], _ = g.forward(want_out=True)
+    rel = np.linalg.norm(out[9] - out[8]) / np.linalg.norm(out[8])
+    worst = float(np.abs(out[9] - out[8]).max() / np.abs(out[8]).max())
+    print(f"BF16 vs FP32 edge latents: output rel diff {rel:.2e}, worst node {worst:.2e}")
+    assert np.all(np.isfinite(out[9]))
+    assert rel < 2e-2 and worst < 5e-2, (rel, worst)
+    g.close()
+
+
 def test_rollout_fallback_reproduces_plain_run(gpu):
     """With a judge that rejects every surrogate step the rollout is exactly the
     plain solver; with one that accepts everything the surrogate advances."""

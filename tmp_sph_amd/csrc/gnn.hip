@@ -77,9 +77,11 @@ struct sph_gnn {
   double* lpart = nullptr;  // per-block loss partials
   float* Pn = nullptr;      // inference: node projections of the edge MLPs' first layer (nodes x 256)
   bf16* Wp = nullptr;       // inference: their weights in lane order, per block (pack_projection)
+  bf16* Eb = nullptr;       // inference with BF16 edge latents (fused = 9)
   int split_cap = 512;  // K chunks of the weight-gradient GEMMs (enough blocks to fill the GPU)
   // forward MLP calls: fused kernel with this load depth (1, 2, 4), three GEMMs (0); 8: edge MLPs by the
-  // persistent kernel with node projections (inference contexts), other calls fused with depth 2
+  // persistent kernel with node projections (inference contexts), other calls fused with depth 2;
+  // 9: as 8 with the edge latents stored as BF16
   int fused = 2;
 };
 
@@ -366,8 +368,8 @@ hipError_t pack_all(sph_gnn* g) {
 
 // out (f32, M x 128) = MLP(in) (+ resid); h1, h2 receive the hidden activations
 hipError_t mlp_forward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, bf16* h1, bf16* h2, float* out,
-                       const float* resid) {
-  if (g->fused) {
+                       const float* resid, bf16* out_b = nullptr) {
+  if (g->fused || out_b) {
     Mlp3Args f{};
     f.A = in;
     f.K1 = m.kin;
@@ -384,7 +386,11 @@ hipError_t mlp_forward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, bf16
     f.h1 = g->cfg.train ? h1 : nullptr;
     f.h2 = g->cfg.train ? h2 : nullptr;
     f.M = M;
-    f.depth = g->fused == 8 ? 2 : g->fused;
+    f.depth = (g->fused == 0 || g->fused >= 8) ? 2 : g->fused;
+    if (out_b) {  // BF16 output (edge encoder of an inference context with BF16 latents)
+      f.out = nullptr;
+      f.out_bf16 = out_b;
+    }
     return mlp3(f, g->stream);
   }
   GemmArgs a{};
@@ -487,7 +493,7 @@ hipError_t mlp_backward(sph_gnn* g, const Mlp& m, const AOperand& in, int M, con
 // and the aggregation agg = segment sums of the new E by receiver: the first
 // layer's receiver and sender blocks applied to the nodes first (P = V W1r^T |
 // V W1s^T), then one persistent kernel per edge set.
-hipError_t edge_forward_split(sph_gnn* g, int l, const float* V, float* E, float* agg) {
+hipError_t edge_forward_split(sph_gnn* g, int l, const float* V, float* E, bf16* Eb, float* agg) {
   const Mlp& m = g->pe[l];
   for (int half = 0; half < 2; ++half) {
     GemmArgs a{};
@@ -504,6 +510,7 @@ hipError_t edge_forward_split(sph_gnn* g, int l, const float* V, float* E, float
   }
   EdgeMlpArgs e{};
   e.E = E;
+  e.Eb = Eb;
   e.ne = g->ne;
   e.P = g->Pn;
   e.recv = g->recv;
@@ -539,7 +546,9 @@ hipError_t forward(sph_gnn* g) {
   int call = 0;
   HIPCHK(mlp_forward(g, g->enc_n, op1(segf(g->Xn, W, W)), n, h1_of(g, call), h2_of(g, call), V_of(g, 0), nullptr));
   ++call;
-  HIPCHK(mlp_forward(g, g->enc_e, op1(segf(g->Xe, W, W)), ne, h1_of(g, call), h2_of(g, call), E_of(g, 0), nullptr));
+  const bool be = g->fused == 9 && !tr;  // BF16 edge latents
+  HIPCHK(mlp_forward(g, g->enc_e, op1(segf(g->Xe, W, W)), ne, h1_of(g, call), h2_of(g, call), E_of(g, 0), nullptr,
+                     be ? g->Eb : nullptr));
   ++call;
   for (int l = 0; l < L; ++l) {
     float* Vl = V_of(g, l);
@@ -547,8 +556,8 @@ hipError_t forward(sph_gnn* g) {
     float* El1 = E_of(g, tr ? l + 1 : 0);
     float* Vl1 = V_of(g, tr ? l + 1 : 0);
     float* Al = A_of(g, l);
-    if (g->fused == 8 && !tr) {
-      HIPCHK(edge_forward_split(g, l, Vl, El, Al));  // also aggregates into Al
+    if (g->fused >= 8 && !tr) {
+      HIPCHK(edge_forward_split(g, l, Vl, be ? nullptr : El, be ? g->Eb : nullptr, Al));  // also aggregates
     } else {
       AOperand ein = op3(segf(El, W, W), segf(Vl, W, W, g->recv), segf(Vl, W, W, g->send));
       HIPCHK(mlp_forward(g, g->pe[l], ein, ne, h1_of(g, call), h2_of(g, call), El1, El));
@@ -628,7 +637,7 @@ extern "C" int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out) {
   sph_gnn* g = new sph_gnn();
   g->cfg = *cfg;
   g->L = cfg->layers;
-  g->fused = cfg->train ? 2 : 8;
+  g->fused = cfg->train ? 2 : 9;
   g->enc_n = make_mlp(g, W);
   g->enc_e = make_mlp(g, W);
   for (int l = 0; l < g->L; ++l) {
@@ -679,6 +688,7 @@ extern "C" int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out) {
   rc = rc ? rc : galloc(g, &g->Agg, (tr ? (size_t)g->L : 1) * Nn * W);
   if (!tr) rc = rc ? rc : galloc(g, &g->Pn, Nn * 2 * W);
   if (!tr) rc = rc ? rc : galloc(g, &g->Wp, (size_t)g->L * 2 * W * W);
+  if (!tr) rc = rc ? rc : galloc(g, &g->Eb, Ne * W);
   rc = rc ? rc : galloc(g, &g->Y, Nn * W);
   rc = rc ? rc : galloc(g, &g->H, (size_t)calls * h_slot(g));
   rc = rc ? rc : galloc(g, &g->scal, 64);
@@ -1204,9 +1214,9 @@ extern "C" int sph_gnn_adam(sph_gnn* g, float lr, float clip) {
 }
 
 extern "C" int sph_gnn_set_fused(sph_gnn* g, int fused) {
-  if (!g || !(fused == 0 || fused == 1 || fused == 2 || fused == 4 || fused == 8))
+  if (!g || !(fused == 0 || fused == 1 || fused == 2 || fused == 4 || fused == 8 || fused == 9))
     return set_error(SPH_E_INVAL, "sph_gnn_set_fused: bad argument");
-  if (fused == 8 && g->cfg.train) return set_error(SPH_E_STATE, "sph_gnn_set_fused: mode 8 needs train = 0");
+  if (fused >= 8 && g->cfg.train) return set_error(SPH_E_STATE, "sph_gnn_set_fused: modes 8 and 9 need train = 0");
   g->fused = fused;
   return SPH_OK;
 }

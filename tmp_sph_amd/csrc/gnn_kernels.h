@@ -57,7 +57,8 @@ hipError_t gemm(const GemmArgs& g, hipStream_t s);
 // the lane order of pack_projection.  The same products as the concatenated
 // first layer, summed in a different order.
 struct EdgeMlpArgs {
-  float* E;
+  float* E;   // FP32 edge latents, or null when Eb holds them
+  bf16* Eb;   // BF16 edge latents (inference with BF16 latents)
   int ne;
   const float* P;
   const int* recv;
@@ -103,6 +104,7 @@ struct Mlp3Args {
   bf16* h2;
   int M;
   int depth;  // k-steps of operand loads in flight (1, 2 or 4; divides K1 / 32)
+  bf16* out_bf16;  // when set, the output is stored here as BF16 (row stride ldo) instead of to `out`
 };
 
 hipError_t mlp3(const Mlp3Args& g, hipStream_t s);

@@ -403,7 +403,12 @@ __global__ __launch_bounds__(256) void mlp3_kernel(Mlp3Args g) {
         const int m = m0 + 16 * t + 4 * lq + r;
         float v = acc[t][c][r] + b;
         if (g.resid) v += rs[c][r];
-        if (m < g.M) g.out[(size_t)m * g.ldo + n] = v;
+        if (m < g.M) {
+          if (g.out_bf16)
+            g.out_bf16[(size_t)m * g.ldo + n] = (bf16)v;
+          else
+            g.out[(size_t)m * g.ldo + n] = v;
+        }
       }
     }
   }
@@ -421,6 +426,7 @@ static_assert(16 * LDF * 4 <= 32 * LDH * 2, "staging rows must fit in the hidden
 
 __device__ __forceinline__ int swz(int row, int col) { return row * 128 + ((((col >> 3) ^ (row & 15))) << 3) + (col & 7); }
 
+template <bool BE>  // BE: edge latents stored as BF16 (g.Eb), else FP32 (g.E)
 __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
   __shared__ __attribute__((aligned(16))) bf16 Wl[3][128 * 128];
   __shared__ __attribute__((aligned(16))) bf16 Hs[EW][32 * LDH];
@@ -431,11 +437,13 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
   for (int c = tid; c < 3 * 2048; c += 64 * EW) {
     const int mtx = c >> 11, n = (c >> 4) & 127, ch = c & 15;
     uint4 v;
-    if (mtx == 0) {
+    if (mtx == 0 && !BE) {  // FP32 A operand: the FP32 k order
       const bf16* src = g.W1 + (size_t)n * g.ldw1 + 32 * (ch >> 2) + 4 * (ch & 3);
       const uint2 lo = *reinterpret_cast<const uint2*>(src);
       const uint2 hi = *reinterpret_cast<const uint2*>(src + 16);
       v = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    } else if (mtx == 0) {  // BF16 A operand: natural k order
+      v = *reinterpret_cast<const uint4*>(g.W1 + (size_t)n * g.ldw1 + 8 * ch);
     } else {
       v = *reinterpret_cast<const uint4*>((mtx == 1 ? g.W2 : g.W3) + (size_t)n * 128 + 8 * ch);
     }
@@ -480,8 +488,17 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
     // the node projections, whose sum P[recv] + P[send] seeds the
     // accumulators (ids loaded during the previous tile); projection columns
     // 8 li .. 8 li + 7 hold this lane's features 16 c + li
-    uint4 ea[4][2][2];
-    {
+    uint4 ea[4][2][2];  // FP32: two words per fragment (FP32 k order); BF16: word 0 (natural order)
+    if (BE) {
+      const bf16* pa = g.Eb + (size_t)min(m0 + li, g.ne - 1) * 128 + 8 * lq;
+      const bf16* pb = g.Eb + (size_t)min(m0 + 16 + li, g.ne - 1) * 128 + 8 * lq;
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        ea[s4][0][0] = *reinterpret_cast<const uint4*>(pa + 32 * s4);
+        ea[s4][1][0] = *reinterpret_cast<const uint4*>(pb + 32 * s4);
+        ea[s4][0][1] = ea[s4][1][1] = make_uint4(0u, 0u, 0u, 0u);
+      }
+    } else {
       const float* pa = g.E + (size_t)min(m0 + li, g.ne - 1) * 128 + 4 * lq;
       const float* pb = g.E + (size_t)min(m0 + 16 + li, g.ne - 1) * 128 + 4 * lq;
 #pragma unroll
@@ -513,8 +530,13 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
 #pragma unroll
     for (int s4 = 0; s4 < 4; ++s4) {
       bf16x8 af[2];
-      af[0] = cvt_raw(ea[s4][0][0], ea[s4][0][1]);
-      af[1] = cvt_raw(ea[s4][1][0], ea[s4][1][1]);
+      if (BE) {
+        __builtin_memcpy(&af[0], &ea[s4][0][0], 16);
+        __builtin_memcpy(&af[1], &ea[s4][1][0], 16);
+      } else {
+        af[0] = cvt_raw(ea[s4][0][0], ea[s4][0][1]);
+        af[1] = cvt_raw(ea[s4][1][0], ea[s4][1][1]);
+      }
       const bf16* wb = &Wl[0][wlane + 32 * (s4 ^ sx)];
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
@@ -540,9 +562,16 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        rr[t][j] = *reinterpret_cast<const float4*>(
-            g.E + (size_t)min(m0 + 16 * t + 2 * j + (lane >> 5), g.ne - 1) * 128 + 4 * c4);
+      for (int j = 0; j < 8; ++j) {
+        const size_t row = (size_t)min(m0 + 16 * t + 2 * j + (lane >> 5), g.ne - 1);
+        if (BE) {
+          const uint2 q = *reinterpret_cast<const uint2*>(g.Eb + row * 128 + 4 * c4);
+          rr[t][j] = make_float4(__uint_as_float(q.x << 16), __uint_as_float(q.x & 0xffff0000u),
+                                 __uint_as_float(q.y << 16), __uint_as_float(q.y & 0xffff0000u));
+        } else {
+          rr[t][j] = *reinterpret_cast<const float4*>(g.E + row * 128 + 4 * c4);
+        }
+      }
     load_ids(min(tile + 1, ntiles - 1), ri, si);
     // layers 2 and 3 (A = H, rows 16 t + li at k 32 s + 8 lq)
 #pragma unroll
@@ -593,9 +622,22 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
         const int row = 2 * j + (lane >> 5), m = m0 + 16 * t + row;
         const float4 v = *reinterpret_cast<const float4*>(&F[row * LDF + 4 * c4]);
         const float4 e = rr[t][j];
-        const float4 o = make_float4(v.x + e.x, v.y + e.y, v.z + e.z, v.w + e.w);
+        float4 o = make_float4(v.x + e.x, v.y + e.y, v.z + e.z, v.w + e.w);
+        if (BE) {  // stored as BF16; the aggregation sums the stored values
+          const bf16 b0 = (bf16)o.x, b1 = (bf16)o.y, b2 = (bf16)o.z, b3 = (bf16)o.w;
+          o = make_float4((float)b0, (float)b1, (float)b2, (float)b3);
+          unsigned short u[4];
+          __builtin_memcpy(&u[0], &b0, 2);
+          __builtin_memcpy(&u[1], &b1, 2);
+          __builtin_memcpy(&u[2], &b2, 2);
+          __builtin_memcpy(&u[3], &b3, 2);
+          if (m < g.ne)
+            *reinterpret_cast<uint2*>(g.Eb + (size_t)m * 128 + 4 * c4) =
+                make_uint2(u[0] | ((unsigned)u[1] << 16), u[2] | ((unsigned)u[3] << 16));
+        } else if (m < g.ne) {
+          *reinterpret_cast<float4*>(g.E + (size_t)m * 128 + 4 * c4) = o;
+        }
         *reinterpret_cast<float4*>(&F[row * LDF + 4 * c4]) = o;  // kept for the aggregation
-        if (m < g.ne) *reinterpret_cast<float4*>(g.E + (size_t)m * 128 + 4 * c4) = o;
       }
       // aggregation A[recv] = sum of the new latents of the receiver's edges,
       // in edge order as segment_sum_kernel adds them: lane l owns columns
@@ -623,8 +665,8 @@ __global__ __launch_bounds__(64 * EW) void edge_mlp_kernel(EdgeMlpArgs g) {
 // Completes the aggregation of edge_mlp_kernel: receivers without edges get
 // zero, receivers whose edges straddle two waves' chunks (chunk_rows edges
 // each) are summed here in edge order; the others were written by the kernel.
-__global__ void agg_fixup_kernel(const float* __restrict__ E, const int* __restrict__ off, int nodes, int chunk_rows,
-                                 float* __restrict__ agg) {
+__global__ void agg_fixup_kernel(const float* __restrict__ E, const bf16* __restrict__ Eb, const int* __restrict__ off,
+                                 int nodes, int chunk_rows, float* __restrict__ agg) {
   const int node = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (node >= nodes) return;
@@ -632,7 +674,13 @@ __global__ void agg_fixup_kernel(const float* __restrict__ E, const int* __restr
   if (e > b && b / chunk_rows == (e - 1) / chunk_rows) return;
   float s0 = 0.0f, s1 = 0.0f;
   for (int k = b; k < e; ++k) {
-    const float2 v = *reinterpret_cast<const float2*>(E + (size_t)k * 128 + 2 * lane);
+    float2 v;
+    if (Eb) {
+      const unsigned q = *reinterpret_cast<const unsigned*>(Eb + (size_t)k * 128 + 2 * lane);
+      v = make_float2(__uint_as_float(q << 16), __uint_as_float(q & 0xffff0000u));
+    } else {
+      v = *reinterpret_cast<const float2*>(E + (size_t)k * 128 + 2 * lane);
+    }
     s0 += v.x;
     s1 += v.y;
   }
@@ -844,7 +892,8 @@ hipError_t gemm(const GemmArgs& g, hipStream_t s) {
 
 hipError_t mlp3(const Mlp3Args& g, hipStream_t s) {
   if (g.M <= 0) return hipSuccess;
-  if (g.K1 % BK != 0 || g.K1 <= 0 || !g.W1 || !g.W2 || !g.W3 || !g.out) return hipErrorInvalidValue;
+  if (g.K1 % BK != 0 || g.K1 <= 0 || !g.W1 || !g.W2 || !g.W3 || (!g.out && !g.out_bf16)) return hipErrorInvalidValue;
+  if (g.out_bf16 && g.resid) return hipErrorInvalidValue;  // BF16 output only for the encoders
   int width = 0;
   for (int i = 0; i < g.A.nseg; ++i) {
     if (!g.A.seg[i].is_f32 || g.A.seg[i].width % BK != 0) return hipErrorInvalidValue;  // FP32 operands only
@@ -865,10 +914,11 @@ hipError_t mlp3(const Mlp3Args& g, hipStream_t s) {
 hipError_t edge_mlp(const EdgeMlpArgs& g, hipStream_t s) {
   if (g.ne <= 0) {  // no edges: every aggregate is zero
     if (g.nodes > 0 && g.agg && g.off)
-      hipLaunchKernelGGL(agg_fixup_kernel, dim3(cdiv(g.nodes, 4)), dim3(256), 0, s, g.E, g.off, g.nodes, 1, g.agg);
+      hipLaunchKernelGGL(agg_fixup_kernel, dim3(cdiv(g.nodes, 4)), dim3(256), 0, s, g.E, g.Eb, g.off, g.nodes, 1,
+                         g.agg);
     return hipGetLastError();
   }
-  if (!g.E || !g.P || !g.recv || !g.send || !g.W1 || !g.W2 || !g.W3 || g.ldw1 < 128 || !g.agg || !g.off ||
+  if ((!g.E && !g.Eb) || !g.P || !g.recv || !g.send || !g.W1 || !g.W2 || !g.W3 || g.ldw1 < 128 || !g.agg || !g.off ||
       g.nodes <= 0)
     return hipErrorInvalidValue;
   int dev = 0, cus = 0;  // one persistent block per CU of the current device
@@ -879,8 +929,12 @@ hipError_t edge_mlp(const EdgeMlpArgs& g, hipStream_t s) {
   const int blocks = std::min(cus, (tiles + EW - 1) / EW);
   EdgeMlpArgs a = g;
   a.chunk = (tiles + blocks * EW - 1) / (blocks * EW);
-  hipLaunchKernelGGL(edge_mlp_kernel, dim3(blocks), dim3(64 * EW), 0, s, a);
-  hipLaunchKernelGGL(agg_fixup_kernel, dim3(cdiv(g.nodes, 4)), dim3(256), 0, s, g.E, g.off, g.nodes, 32 * a.chunk, g.agg);
+  if (g.Eb)
+    hipLaunchKernelGGL(edge_mlp_kernel<true>, dim3(blocks), dim3(64 * EW), 0, s, a);
+  else
+    hipLaunchKernelGGL(edge_mlp_kernel<false>, dim3(blocks), dim3(64 * EW), 0, s, a);
+  hipLaunchKernelGGL(agg_fixup_kernel, dim3(cdiv(g.nodes, 4)), dim3(256), 0, s, g.E, g.Eb, g.off, g.nodes,
+                     32 * a.chunk, g.agg);
   return hipGetLastError();
 }
 

@@ -127,8 +127,9 @@ class Gnn:
     def set_fused(self, depth: int) -> None:
         """One fused kernel per MLP call with `depth` k-steps of loads in
         flight (1, 2 or 4; default for training contexts), three GEMM
-        launches (0), or 8 (inference contexts, their default): persistent
-        edge kernel with the first layer split into node projections."""
+        launches (0), 8 (inference contexts): persistent edge kernel with the
+        first layer split into node projections, or 9 (the inference
+        default): as 8 with BF16 edge latents."""
         call("sph_gnn_set_fused", self._h, int(depth))
 
     def tensor(self, kind: int, index: int = 0) -> np.ndarray:
