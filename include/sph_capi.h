@@ -369,20 +369,23 @@ typedef struct sph_rollout_stats {
     int accepted;        /* intervals advanced by the surrogate */
     int fallbacks;       /* surrogate intervals rejected and redone with k plain steps */
     long long plain_steps; /* plain SPH steps taken (history warm-up and fallbacks) */
-    double max_dev;      /* largest density deviation the judge measured */
-    double mean_dev;     /* mean over judged intervals */
+    double max_dev;      /* largest per-particle density deviation the judge measured */
+    double mean_dev;     /* mean over judged intervals of the mean per-particle deviation */
     double seconds;      /* wall-clock of the rollout */
     int rejected_penetration; /* network steps rejected for wall penetration (a fluid particle
                                  predicted outside the escape box, where the solver would park it) */
     double max_penetration;   /* deepest predicted position beyond the escape box seen by the judge (m) */
+    int rejected_density;     /* network steps rejected for a density deviation above dev_tol */
+    int rejected_graph;       /* intervals left to the plain step because the state's graph had more
+                                 than max_edges edges */
 } sph_rollout_stats;
 
 /* Advances the single-instance 2-D context `sim` by `intervals` intervals of
    k plain-step lengths, each by one surrogate step judged by the plain
    method: the SPH summation density (the kernel sum of kernel A's Wendland
    weights) of the predicted positions must change like the predicted density
-   -- mean |d rho_sum - d rho_pred| / rho0 over fluid particles with a full
-   support <= dev_tol -- and no fluid particle may be predicted outside the
+   -- |d rho_sum - d rho_pred| / rho0 <= dev_tol for every fluid particle with
+   a full support -- and no fluid particle may be predicted outside the
    escape box (the tank faces; the solver would park it); otherwise the
    interval is redone with k plain steps.  Particles the solver has already
    parked are left where they are and out of the judgement.  The first

@@ -796,7 +796,9 @@ extern "C" int sph_gnn_get_grads(sph_gnn* g, float* host) {
 // Radius graph and input features of the state held in g->pos / vhist / rho /
 // type (device, original order): hash, radix sort, cell ranges, degrees,
 // scan, edges, reverse edges.  One host synchronisation (the edge count).
-static int build_graph_features(sph_gnn* g, const sph_params& p, int n) {
+// Graph and features of the loaded state.  *overflow (when given) reports a
+// graph with more edges than max_edges (nothing built) instead of an error.
+static int build_graph_features(sph_gnn* g, const sph_params& p, int n, bool* overflow = nullptr) {
   hipStream_t s = g->stream;
   g->prm = p;
   g->n = n;
@@ -834,7 +836,9 @@ static int build_graph_features(sph_gnn* g, const sph_params& p, int n) {
   int ne = 0;
   SPH_HIP(hipMemcpyAsync(&ne, g->off + n, sizeof(int), hipMemcpyDeviceToHost, s), "graph edges");
   SPH_HIP(hipStreamSynchronize(s), "graph sync");
-  if (ne > g->cfg.max_edges) return set_error(SPH_E_INVAL, "surrogate graph: more edges than max_edges");
+  if (overflow) *overflow = ne > g->cfg.max_edges;
+  if (ne > g->cfg.max_edges)
+    return overflow ? SPH_OK : set_error(SPH_E_INVAL, "surrogate graph: more edges than max_edges");
   g->ne = ne;
   hipLaunchKernelGGL(fill_edges_kernel, dim3(nblk(n)), dim3(256), 0, s, p, g->pos, keys, idx, g->cellStart,
                      g->cellEnd, g->off, g->send, g->recv, n);
@@ -1011,8 +1015,16 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, f
   for (int it = 0; it < intervals && !rc; ++it) {
     bool plain = filled < H;
     if (!plain) {
-      rc = build_graph_features(g, P, n);
+      bool overflow = false;
+      rc = build_graph_features(g, P, n, &overflow);
       if (rc) break;
+      if (overflow) {  // a state whose graph does not fit: the plain step takes the interval
+        plain = true;
+        ++st.fallbacks;
+        ++st.rejected_graph;
+      }
+    }
+    if (!plain) {
       if ((he = forward(g)) != hipSuccess) { rc = hip_status(he, "rollout forward"); break; }
       hipLaunchKernelGGL(apply_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->cfg, k, g->Y, g->pos, vel, g->rho,
                          g->type, pos2, vel2, rho2, p2, n);
@@ -1023,12 +1035,16 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, f
       he = hipMemcpyAsync(jh, judge, sizeof(jh), hipMemcpyDeviceToHost, s);
       if (he == hipSuccess) he = hipStreamSynchronize(s);
       if (he != hipSuccess) { rc = hip_status(he, "rollout judge"); break; }
-      const double dev = jh[1] > 0 ? (double)jh[0] / (double)jh[1] : 0.0;
-      dev_sum += dev;
+      // the largest per-particle deviation decides (a mean would hide a local
+      // compression); the mean is kept for the statistics
+      const double dev_mean = jh[1] > 0 ? (double)jh[0] / (double)jh[1] : 0.0;
+      const double dev = jh[1] > 0 ? (double)jh[2] : 0.0;
+      dev_sum += dev_mean;
       ++judged;
       st.max_dev = std::max(st.max_dev, dev);
       st.max_penetration = std::max(st.max_penetration, (double)jh[4]);
       if (jh[3] != 0.0f) ++st.rejected_penetration;
+      if (!(dev <= dev_tol)) ++st.rejected_density;
       if (dev <= dev_tol && jh[3] == 0.0f) {
         rc = sph_sim_set_state_device(sim, pos2, vel2, rho2, p2, g->type);
         if (rc) break;

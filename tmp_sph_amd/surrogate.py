@@ -33,7 +33,8 @@ class GnnCfg(C.Structure):
 class RolloutStats(C.Structure):
     _fields_ = [("accepted", C.c_int), ("fallbacks", C.c_int), ("plain_steps", C.c_longlong),
                 ("max_dev", C.c_double), ("mean_dev", C.c_double), ("seconds", C.c_double),
-                ("rejected_penetration", C.c_int), ("max_penetration", C.c_double)]
+                ("rejected_penetration", C.c_int), ("max_penetration", C.c_double),
+                ("rejected_density", C.c_int), ("rejected_graph", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
