@@ -23,9 +23,29 @@ constexpr int RS_MIN_TILES = 512;  // small sorts use smaller tiles to fill the 
 
 static_assert(RS_THREADS == 256, "one thread per digit in the per-digit phases");
 
-// Lanes of the wave whose digit equals this lane's digit (among `active`).
+// Lanes of the wave whose digit equals this lane's digit (among `active`,
+// a prefix of the wave).
+//
+// The cell keys arrive nearly sorted (the previous step's order), and every
+// LSD pass keeps each bucket of the previous digit in key order, so in almost
+// every wave the digits do not decrease from lane to lane.  Then equal digits
+// form one run of consecutive lanes, found with one shuffle and one ballot.
+// Otherwise (a bucket boundary, or unsorted keys) the digit is matched bit by
+// bit with one ballot per bit.
 __device__ __forceinline__ unsigned long long match_digit(unsigned d, int nbits,
                                                           unsigned long long active) {
+  const int lane = __lane_id();
+  const bool on = (active >> lane) & 1ull;
+  const unsigned dp = __shfl_up(d, 1);
+  const bool start = lane == 0 || d != dp;
+  if (__ballot(on && lane > 0 && d < dp) == 0ull) {  // wave-uniform: non-decreasing digits
+    const unsigned long long starts = __ballot(on && start);
+    const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int first = below ? 63 - __builtin_clzll(below) : 0;  // start of this lane's run
+    const unsigned long long above = starts & ~(lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const unsigned long long upto = above ? ((1ull << __builtin_ctzll(above)) - 1ull) : ~0ull;
+    return active & upto & ~((1ull << first) - 1ull);
+  }
   unsigned long long m = active;
   for (int b = 0; b < nbits; ++b) {
     const int bit = (d >> b) & 1;
