@@ -191,8 +191,10 @@ typedef struct sph_tuning {
     int nbr_unroll;      /* candidate loads issued together: 1, 2, 4 or 8 */
     int xcd_remap;       /* 1: each XCD gets a contiguous run of target tiles */
     int flags;           /* bit 0: per-kernel HIP event timing in sph_sim_step */
-    int nbr_lds;         /* 1: one-wave blocks stage the wave's candidate rows in LDS
-                            (nbr_block is then ignored) */
+    int nbr_lds;         /* 1: one-wave blocks stage the wave's candidate rows in LDS;
+                            2: the same one z-plane (three rows) at a time, plain
+                            loop only (nbr_compact ignored); nbr_block is then
+                            ignored */
     int nbr_compact;     /* 1: scan candidates into per-lane LDS lists, then run the
                             pair body densely over the accepted ones */
     int nbr_prefetch;    /* 1: load a candidate's velocity/dofv together with its

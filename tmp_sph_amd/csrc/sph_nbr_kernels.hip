@@ -12,6 +12,9 @@
 //            row of one instance, the union of their stencil rows is copied
 //            once into LDS and candidates are read from there; other waves
 //            read HBM/L2 ("global").
+//   planes   the same with one z-plane of three stencil rows in LDS at a
+//            time (a quarter of the LDS, so more waves per CU); the wave
+//            restages at the first row of each plane.
 //   compact  only ~15% of the candidates of a 3-D cell stencil are within the
 //            cutoff, and a wave runs the pair body whenever any lane accepts,
 //            so the plain loop pays the full pair cost on almost every
@@ -31,6 +34,7 @@ inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) /
 
 constexpr int STAGE_CAP = 1024;   // float4 entries per staged wave (16 KB of LDS)
 constexpr int STAGE_PAD = 8;      // spare tile entries read by unrolled loads past a row's end
+constexpr int PLANE_CAP = 448;    // float4 entries of one staged plane (7 KB of LDS)
 constexpr int COMPACT_CAP = 16;   // accepted candidates per lane per dense phase
 
 template <int DIM>
@@ -70,6 +74,44 @@ struct StagedSrc {
 #pragma unroll
     for (int k = 0; k < NR; ++k) v |= shift[k] & -(int)(r == k);
     cur = v;
+  }
+  __device__ __forceinline__ float4 at(int j0, int u) const { return tile[j0 + cur + u]; }
+};
+
+// Candidates of the current plane (rows 3p..3p+2) at tile[j + sh[r - 3p]]:
+// uni[r] is the wave's union of row r (wave-uniform).  row() is called by
+// all 64 lanes of the (single-wave) block at every row; at the first row of
+// a plane it copies the plane's three unions into the tile.
+template <int NR>
+struct PlaneSrc {
+  const float4* PR;
+  float4* tile;
+  RowRange uni[NR];
+  int sh[3];
+  int cur;
+  __device__ __forceinline__ void row(int r) {
+    const int k = r % 3;
+    if (k == 0) {
+      __syncthreads();  // every lane is done with the previous plane
+      const int lane = threadIdx.x;
+      int base = 0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const RowRange u = pick_row(uni, r + q);
+        const int len = u.e - u.s;
+        for (int t = lane; t < len; t += 128) {
+          const float4 a = PR[u.s + t];
+          const bool two = t + 64 < len;
+          const float4 b = PR[u.s + (two ? t + 64 : t)];
+          tile[base + t] = a;
+          if (two) tile[base + t + 64] = b;
+        }
+        sh[q] = base - u.s;
+        base += len;
+      }
+      __syncthreads();
+    }
+    cur = k == 0 ? sh[0] : (k == 1 ? sh[1] : sh[2]);
   }
   __device__ __forceinline__ float4 at(int j0, int u) const { return tile[j0 + cur + u]; }
 };
@@ -235,7 +277,8 @@ __device__ __forceinline__ float geom(const float4& pi, const float4& pj, float&
 // Kernel A for target i (fluid neighbours only).
 template <int DIM, class Visit>
 __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const float4& pi, const float4& vi,
-                                           Visit visit, float4* PR, float4* VP, float* __restrict__ DV) {
+                                           Visit visit, float4* PR, float4* VP, float* __restrict__ DV,
+                                           bool store = true) {
 #pragma clang fp contract(off)
   const bool fi = pi.w > 0.0f;
   const float inv_h = 1.0f / P.h;
@@ -267,6 +310,7 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
           dv = fmaf(-(cfr * t3 * m), dvdx * fast_rcp(pj.w), dv);
         }
       });
+  if (!store) return;
   DV[i] = dv;
   if (!fi) {
     float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
@@ -286,7 +330,8 @@ struct NbrB {
 template <int DIM, class Visit>
 __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const float4& pi, const float4& vi, float dvi,
                                            Visit visit, const float4* __restrict__ VP,
-                                           const float* __restrict__ DV, float4* __restrict__ RATE) {
+                                           const float* __restrict__ DV, float4* __restrict__ RATE,
+                                           bool store = true) {
 #pragma clang fp contract(off)
   const float rho_i = pi.w;
   const float inv_h = 1.0f / P.h;
@@ -331,7 +376,7 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         if (DIM == 3) ay = fmaf(vt, dy, ay);
         az = fmaf(vt, dz, az);
       });
-  RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
+  if (store) RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
 }
 
 // ------------------------------------------------------------ staging
@@ -381,20 +426,53 @@ __device__ __forceinline__ bool stage_rows(const sph_params& P, int coff, int ke
   return true;
 }
 
+// Plane staging: the wave stages when its active targets lie in one cell row
+// of one instance and every plane's three row unions fit PLANE_CAP.  Inactive
+// lanes come with empty rows and only help with the copies.
+template <int DIM>
+__device__ __forceinline__ bool plan_planes(const sph_params& P, int coff, int key, bool active, bool same_inst,
+                                            const RowRange (&rows)[DIM == 3 ? 9 : 3], const float4* PR,
+                                            float4* tile, PlaneSrc<nrows<DIM>()>& src) {
+  constexpr int NR = nrows<DIM>();
+  const unsigned long long act = __ballot(active);
+  const int row_id = (key - coff) / P.gridxdim;
+  const int u_row = __builtin_amdgcn_readlane(row_id, __builtin_ctzll(act));
+  if (!same_inst || __ballot(active && row_id != u_row) != 0ull) return false;
+  bool fits = true;
+#pragma unroll
+  for (int p0 = 0; p0 < NR; p0 += 3) {
+    int total = 0;
+#pragma unroll
+    for (int r = p0; r < p0 + 3; ++r) {
+      const bool ne = rows[r].e > rows[r].s;
+      const int l = wave_min(ne ? rows[r].s : 0x7fffffff);
+      const int h = wave_max(ne ? rows[r].e : 0);
+      src.uni[r] = l == 0x7fffffff ? RowRange{0, 0} : RowRange{l, h};
+      total += src.uni[r].e - src.uni[r].s;
+    }
+    fits = fits && total <= PLANE_CAP;
+  }
+  src.PR = PR;
+  src.tile = tile;
+  src.cur = 0;
+  return fits;
+}
+
 // ------------------------------------------------------------ kernels
-// STAGED: 64-thread blocks with LDS staging; else up to 256 threads.
+// STAGED: 64-thread blocks with LDS staging (1: whole stencil, 2: plane by
+// plane, plain loop only); else up to 256 threads.
 // COMPACT: scan/dense loop; else the plain loop.
-template <int DIM, int U, bool STAGED, bool COMPACT>
+template <int DIM, int U, int STAGED, bool COMPACT>
 struct Cfg {
   static constexpr int NR = nrows<DIM>();
   static constexpr int LIST = COMPACT ? COMPACT_CAP * (STAGED ? 64 : 256) : 1;
-  static constexpr int TILE = STAGED ? STAGE_CAP + STAGE_PAD : 1;
+  static constexpr int TILE = STAGED == 2 ? PLANE_CAP + STAGE_PAD : STAGED ? STAGE_CAP + STAGE_PAD : 1;
   static constexpr int SLOTS = (!STAGED && !COMPACT) ? NR * 256 : 1;
 };
 
 // Runs body(P, visit) for this lane with a visit loop over its stencil.  The
 // global path serves lanes instance by instance (wave-uniform parameters).
-template <int DIM, int U, bool STAGED, bool COMPACT, bool PF, class Body>
+template <int DIM, int U, int STAGED, bool COMPACT, bool PF, class Body>
 __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key, const float4& p, bool active,
                                            const float4* PR,
                                            const int* __restrict__ cs, const int* __restrict__ ce,
@@ -411,17 +489,39 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
       const int t = threadIdx.x;
 #pragma unroll
       for (int r = 0; r < NR; ++r) slots[r * 256 + t] = rows[r];
-      body(P, PlainVisit<NR, U, PF, LdsRows, GlobalSrc>{LdsRows{slots + t, 256}, GlobalSrc{PR}});
+      body(P, PlainVisit<NR, U, PF, LdsRows, GlobalSrc>{LdsRows{slots + t, 256}, GlobalSrc{PR}}, true);
     } else {
       RegRows<NR> rows;
       row_ranges<DIM>(P, coff, key, p, cs, ce, fc, rows.v);
       if constexpr (COMPACT)
-        body(P, CompactVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}, list, PR});
+        body(P, CompactVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}, list, PR}, true);
       else
-        body(P, PlainVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}});
+        body(P, PlainVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}}, true);
     }
   };
-  if constexpr (STAGED) {
+  if constexpr (STAGED == 2) {
+    // the whole wave stays in the loop (inactive lanes with empty rows) for
+    // the per-plane copies
+    if (__ballot(active) == 0ull) return;
+    const unsigned long long act = __ballot(active);
+    const int u = __builtin_amdgcn_readlane(inst, __builtin_ctzll(act));
+    const bool same_inst = __ballot(active && inst != u) == 0ull;
+    const sph_params& P = T.P[u];
+    const int coff = T.cell_off[u];
+    RegRows<NR> rows;
+    row_ranges<DIM>(P, coff, key, p, cs, ce, fc, rows.v);
+    if (!active) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) rows.v[r] = RowRange{0, 0};
+    }
+    PlaneSrc<NR> src;
+    if (plan_planes<DIM>(P, coff, key, active, same_inst, rows.v, PR, tile, src)) {
+      body(P, PlainVisit<NR, U, PF, RegRows<NR>, PlaneSrc<NR>>{rows, src}, active);
+    } else {
+      if (!active) return;
+      with_instance(T, inst, global_path);
+    }
+  } else if constexpr (STAGED) {
     // the whole wave takes part in the staging decision and copy
     const int u = __builtin_amdgcn_readfirstlane(inst);
     const bool same_inst = __ballot(inst != u) == 0ull;
@@ -434,9 +534,9 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
     if (!active) return;
     if (staged) {
       if constexpr (COMPACT)
-        body(P, CompactVisit<NR, U, PF, RegRows<NR>, StagedSrc<NR>>{rows, src, list, PR});
+        body(P, CompactVisit<NR, U, PF, RegRows<NR>, StagedSrc<NR>>{rows, src, list, PR}, true);
       else
-        body(P, PlainVisit<NR, U, PF, RegRows<NR>, StagedSrc<NR>>{rows, src});
+        body(P, PlainVisit<NR, U, PF, RegRows<NR>, StagedSrc<NR>>{rows, src}, true);
     } else {
       with_instance(T, inst, global_path);
     }
@@ -446,7 +546,7 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
   }
 }
 
-template <int DIM, int U, bool STAGED, bool COMPACT, bool PF>
+template <int DIM, int U, int STAGED, bool COMPACT, bool PF>
 __global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, float4* PR, float4* VP,
                                                                 const int* __restrict__ TY, float* __restrict__ DV,
                                                                 const int* __restrict__ keys,
@@ -465,12 +565,12 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, flo
   const float4 vi = VP[ic];
   const int key = keys[ic];
   with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid, PR, cs, ce, fc, tile, lists, slots,
-                                      [&](const sph_params& P, auto visit) {
-                                        nsA_target<DIM>(P, i, pi, vi, visit, PR, VP, DV);
+                                      [&](const sph_params& P, auto visit, bool store) {
+                                        nsA_target<DIM>(P, i, pi, vi, visit, PR, VP, DV, store);
                                       });
 }
 
-template <int DIM, int U, bool STAGED, bool COMPACT, bool PF>
+template <int DIM, int U, int STAGED, bool COMPACT, bool PF>
 __global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, const float4* __restrict__ PR,
                                                                 const float4* __restrict__ VP,
                                                                 const int* __restrict__ TY,
@@ -496,8 +596,8 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, con
   const int key = keys[ic];
   with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid && fluid, PR, cs, ce, nullptr, tile,
                                           lists, slots,
-                                      [&](const sph_params& P, auto visit) {
-                                        nsB_target<DIM>(P, i, pi, vi, dvi, visit, VP, DV, RATE);
+                                      [&](const sph_params& P, auto visit, bool store) {
+                                        nsB_target<DIM>(P, i, pi, vi, dvi, visit, VP, DV, RATE, store);
                                       });
 }
 
@@ -597,7 +697,7 @@ __global__ __launch_bounds__(256) void nsB_list_kernel(InstTable T, const float4
 }
 
 // ------------------------------------------------------------ launchers
-template <int U, bool STAGED, bool COMPACT, bool PF>
+template <int U, int STAGED, bool COMPACT, bool PF>
 hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
                  hipStream_t s) {
   const int b = STAGED ? 64 : block;
@@ -613,7 +713,7 @@ hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n
   return hipGetLastError();
 }
 
-template <int U, bool STAGED, bool COMPACT, bool PF>
+template <int U, int STAGED, bool COMPACT, bool PF>
 hipError_t nsB_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
                  hipStream_t s) {
   const int b = STAGED ? 64 : block;
@@ -640,6 +740,8 @@ hipError_t nsA_u(const InstTable& T, const FastSet& st, const FastWork& w, int n
     case 5: return nsA_v<U, true, false, true>(T, st, w, n, block, remap, s);
     case 6: return nsA_v<U, false, true, true>(T, st, w, n, block, remap, s);
     case 7: return nsA_v<U, true, true, true>(T, st, w, n, block, remap, s);
+    case 8: case 10: return nsA_v<U, 2, false, false>(T, st, w, n, block, remap, s);
+    case 12: case 14: return nsA_v<U, 2, false, true>(T, st, w, n, block, remap, s);
     default: return nsA_v<U, false, false, false>(T, st, w, n, block, remap, s);
   }
 }
@@ -655,13 +757,15 @@ hipError_t nsB_u(const InstTable& T, const FastSet& st, const FastWork& w, int n
     case 5: return nsB_v<U, true, false, true>(T, st, w, n, block, remap, s);
     case 6: return nsB_v<U, false, true, true>(T, st, w, n, block, remap, s);
     case 7: return nsB_v<U, true, true, true>(T, st, w, n, block, remap, s);
+    case 8: case 10: return nsB_v<U, 2, false, false>(T, st, w, n, block, remap, s);
+    case 12: case 14: return nsB_v<U, 2, false, true>(T, st, w, n, block, remap, s);
     default: return nsB_v<U, false, false, false>(T, st, w, n, block, remap, s);
   }
 }
 
 int norm_block(int b) { return (b >= 64 && b <= 256 && b % 64 == 0) ? b : 256; }
 int mode_of(const sph_tuning& tu) {
-  return (tu.nbr_lds ? 1 : 0) | (tu.nbr_compact ? 2 : 0) | (tu.nbr_prefetch ? 4 : 0);
+  return (tu.nbr_lds == 2 ? 8 : tu.nbr_lds ? 1 : 0) | (tu.nbr_compact ? 2 : 0) | (tu.nbr_prefetch ? 4 : 0);
 }
 
 template <int U>
