@@ -192,11 +192,17 @@ typedef struct sph_tuning {
     int xcd_remap;       /* 1: each XCD gets a contiguous run of target tiles */
     int flags;           /* bit 0: per-kernel HIP event timing in sph_sim_step */
     int nbr_lds;         /* 1: one-wave blocks stage the wave's candidate rows in LDS;
-                            2: the same one z-plane (three rows) at a time, plain
-                            loop only (nbr_compact ignored); nbr_block is then
+                            2: the same one z-plane (three rows) at a time;
+                            3: one row at a time together with the neighbour
+                            data the pair bodies read (no global loads in the
+                            loop); 2 and 3 run the plain loop or mask groups
+                            (nbr_compact 1 is ignored); nbr_block is then
                             ignored */
     int nbr_compact;     /* 1: scan candidates into per-lane LDS lists, then run the
-                            pair body densely over the accepted ones */
+                            pair body densely over the accepted ones;
+                            2: test groups of 4 x nbr_unroll candidates into a bit
+                            mask, then run the pair body over the set bits (with
+                            nbr_lds 0, 2 or 3; nbr_lds 1 and nbr_prefetch ignored) */
     int nbr_prefetch;    /* 1: load a candidate's velocity/dofv together with its
                             position instead of after the cut-off test */
     int nbr_list;        /* 1: one scan records each target's neighbours, kernels A

@@ -74,13 +74,16 @@ def test_autotune_keeps_results(gpu):
 
 _VARIANTS = ([(lds, compact, pf, 0, un) for lds in (0, 1) for compact in (0, 1) for pf in (0, 1) for un in (1, 4, 8)]
              + [(2, 0, pf, 0, un) for pf in (0, 1) for un in (1, 4, 8)]
+             + [(lds, 2, 0, 0, un) for lds in (0, 2, 3) for un in (1, 2, 8)]
+             + [(3, 0, pf, 0, un) for pf in (0, 1) for un in (1, 4, 8)]
              + [(0, 0, 0, 1, un) for un in (1, 4, 8)])
 
 
 @pytest.mark.parametrize("dim", [2, 3])
 def test_neighbour_loop_variants_bit_identical(gpu, dim):
-    """LDS staging (whole stencil or plane by plane), the scan/dense (compact)
-    loop, prefetched neighbour loads and
+    """LDS staging (whole stencil, plane by plane, row by row with the
+    neighbour data), the scan/dense (compact)
+    loop, mask groups, prefetched neighbour loads and
     neighbour lists visit the same pairs in the same order as the plain global
     loop: every rate is bit-identical."""
     from tmp_sph_amd.sph import dambreak_cfg, make_case

@@ -41,7 +41,7 @@ hipError_t launch_pc_aos(const sph_params& P, bool predictor, float* sPos, float
 // VP = (vx, vy, vz, p); X = FP64 positions (SoA); ID = original id; TY = type.
 // Spare entries after PR of a fast set: the neighbour loops issue their
 // unrolled candidate loads without clamping to the row end.
-constexpr int CAND_PAD = 8;
+constexpr int CAND_PAD = 32;
 
 struct FastSet {
   float4* PR;

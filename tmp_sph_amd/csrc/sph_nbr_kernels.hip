@@ -22,6 +22,13 @@
 //            each lane's accepted ones to its own LDS list and, whenever a
 //            list is nearly full (and at the end), runs the pair body densely
 //            over all lists.
+//   rows     one stencil row at a time in LDS with the neighbour data the
+//            pair body reads (velocity/pressure, dofv), so the loop makes no
+//            global loads: the wave waits on memory once per row.
+//   masks    tests a group of 4 x UNROLL candidates into a per-lane bit mask,
+//            then runs the pair body over each lane's set bits in ascending
+//            order: a wave executes as many bodies per group as its busiest
+//            lane accepted, not one per candidate.
 #include "sph_fast_common.h"
 
 namespace sph {
@@ -35,6 +42,9 @@ inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) /
 constexpr int STAGE_CAP = 1024;   // float4 entries per staged wave (16 KB of LDS)
 constexpr int STAGE_PAD = 8;      // spare tile entries read by unrolled loads past a row's end
 constexpr int PLANE_CAP = 448;    // float4 entries of one staged plane (7 KB of LDS)
+constexpr int PLANE_PAD = 32;     // spare plane entries read by a mask group past a row's end
+constexpr int ROW_CAP = 192;      // entries of one staged row (with its neighbour data: 8 KB of LDS)
+constexpr int ROW_PAD = 32;
 constexpr int COMPACT_CAP = 16;   // accepted candidates per lane per dense phase
 
 template <int DIM>
@@ -55,17 +65,27 @@ struct LdsRows {  // in the thread's own LDS slots
 
 // ------------------------------------------------------- candidate sources
 // src.row(r) is called at the start of row r; src.at(j0, u) returns
-// PR[j0 + u] for u < UNROLL, reading up to UNROLL - 1 entries past a row's end
-// (PR is allocated with CAND_PAD spare entries; results there are discarded).
+// PR[j0 + u] for u < UNROLL (4 x UNROLL for the mask loop), reading up to that
+// many entries minus one past a row's end (PR is allocated with CAND_PAD spare
+// entries; results there are discarded).
+// src.ext(extra, j) returns candidate j's neighbour data (extra(j) reads it
+// from global memory).
+struct GlobalExtras {
+  template <class E>
+  __device__ __forceinline__ auto ext(const E& extra, int j) const { return extra(j); }
+};
+
 struct GlobalSrc {
   const float4* PR;
+  template <class E>
+  __device__ __forceinline__ auto ext(const E& extra, int j) const { return extra(j); }
   __device__ __forceinline__ void row(int) {}
   __device__ __forceinline__ float4 at(int j0, int u) const { return ld16(PR, (unsigned)j0 * 16u + 16u * u); }
 };
 
 // Candidates of row r live at tile[j + shift[r]]; the shifts are wave-uniform.
 template <int NR>
-struct StagedSrc {
+struct StagedSrc : GlobalExtras {
   const float4* tile;
   int shift[NR];
   int cur;
@@ -83,7 +103,7 @@ struct StagedSrc {
 // all 64 lanes of the (single-wave) block at every row; at the first row of
 // a plane it copies the plane's three unions into the tile.
 template <int NR>
-struct PlaneSrc {
+struct PlaneSrc : GlobalExtras {
   const float4* PR;
   float4* tile;
   RowRange uni[NR];
@@ -116,6 +136,51 @@ struct PlaneSrc {
   __device__ __forceinline__ float4 at(int j0, int u) const { return tile[j0 + cur + u]; }
 };
 
+// The current row's candidates at tp[j + cur] and their neighbour data at
+// tv/td[j + cur] (td only when DV is given): uni[r] is the wave's union of row
+// r.  row() is called by all 64 lanes of the (single-wave) block at every row.
+template <int NR>
+struct RowSrc {
+  const float4* PR;
+  const float4* VP;
+  const float* DV;
+  float4* tp;
+  float4* tv;
+  float* td;
+  RowRange uni[NR];
+  int cur;
+  __device__ __forceinline__ void row(int r) {
+    __syncthreads();  // every lane is done with the previous row
+    const RowRange u = pick_row(uni, r);
+    const int len = u.e - u.s;
+    const int lane = threadIdx.x;
+    for (int t = lane; t < len; t += 128) {  // two entries per lane per pass: loads first
+      const bool two = t + 64 < len;
+      const int t2 = two ? t + 64 : t;
+      const float4 a0 = PR[u.s + t], a1 = PR[u.s + t2];
+      const float4 b0 = VP[u.s + t], b1 = VP[u.s + t2];
+      float c0 = 0.0f, c1 = 0.0f;
+      if (DV) {
+        c0 = DV[u.s + t];
+        c1 = DV[u.s + t2];
+      }
+      tp[t] = a0;
+      tv[t] = b0;
+      if (DV) td[t] = c0;
+      if (two) {
+        tp[t2] = a1;
+        tv[t2] = b1;
+        if (DV) td[t2] = c1;
+      }
+    }
+    cur = -u.s;
+    __syncthreads();
+  }
+  __device__ __forceinline__ float4 at(int j0, int u) const { return tp[j0 + cur + u]; }
+  template <class E>
+  __device__ __forceinline__ auto ext(const E& extra, int j) const { return extra.from(tv, td, j + cur); }
+};
+
 // ------------------------------------------------------------ visit loops
 // visit(accept, extra, pair): accept(j, pj) is the candidate test, extra(j)
 // loads what the pair body needs of candidate j beyond PR (velocity etc.),
@@ -142,7 +207,7 @@ struct PlainVisit {
         for (int u = 0; u < U; ++u) pjs[u] = src.at(j0, u);  // issued together
         if constexpr (PF) {
 #pragma unroll
-          for (int u = 0; u < U; ++u) es[u] = extra(j0 + u);
+          for (int u = 0; u < U; ++u) es[u] = src.ext(extra, j0 + u);
         }
         bool acc[U];
 #pragma unroll
@@ -151,7 +216,7 @@ struct PlainVisit {
           // the accepted candidates' loads go out together: one wait per group
 #pragma unroll
           for (int u = 0; u < U; ++u)
-            if (acc[u]) es[u] = extra(j0 + u);
+            if (acc[u]) es[u] = src.ext(extra, j0 + u);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -187,7 +252,7 @@ struct CompactVisit {
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             pj[u] = ld16(PR, (unsigned)jj[u] * 16u);
-            e[u] = extra(jj[u]);
+            e[u] = src.ext(extra, jj[u]);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
@@ -197,7 +262,7 @@ struct CompactVisit {
 #pragma unroll 1
         for (int k = 0; k < cnt; ++k) {
           const int jj = list[k * 64 + lane];  // [slot][lane]: conflict-free for any per-lane count
-          pair(jj, ld16(PR, (unsigned)jj * 16u), extra(jj));
+          pair(jj, ld16(PR, (unsigned)jj * 16u), src.ext(extra, jj));
         }
       }
       cnt = 0;
@@ -227,6 +292,63 @@ struct CompactVisit {
     flush();
   }
 };
+
+template <int NR, int U, class Rows, class Src>
+struct MaskVisit {
+  Rows rows;
+  Src src;
+  template <class Accept, class Extra, class Pair>
+  __device__ __forceinline__ void operator()(Accept&& accept, Extra&& extra, Pair&& pair) {
+    constexpr int G = 4 * U;  // candidates per group (<= 32 mask bits)
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) {
+      const RowRange g = rows(r);
+      src.row(r);
+      for (int j0 = g.s; j0 < g.e; j0 += G) {
+        unsigned m = 0u;
+#pragma unroll
+        for (int b = 0; b < G; b += 4) {
+          float4 pjs[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) pjs[u] = src.at(j0, b + u);  // issued together
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (j0 + b + u < g.e && accept(j0 + b + u, pjs[u])) m |= 1u << (b + u);
+        }
+        if (m != 0u) {
+          // accepted candidates in ascending order; the loads of the next one
+          // go out before the body of the current one
+          int k = __builtin_ctz(m);
+          auto e = src.ext(extra, j0 + k);
+          for (;;) {
+            m &= m - 1u;
+            auto en = e;
+            int kn = k;
+            if (m != 0u) {
+              kn = __builtin_ctz(m);
+              en = src.ext(extra, j0 + kn);
+            }
+            pair(j0 + k, src.at(j0, k), e);
+            if (m == 0u) break;
+            k = kn;
+            e = en;
+          }
+        }
+      }
+    }
+  }
+};
+
+// The visit loop of a variant: COMPACT 1 = LDS lists, 2 = mask groups, else plain.
+template <int NR, int U, int COMPACT, bool PF, class Rows, class Src>
+__device__ __forceinline__ auto make_visit(Rows rows, Src src, int* list, const float4* PR) {
+  if constexpr (COMPACT == 1)
+    return CompactVisit<NR, U, PF, Rows, Src>{rows, src, list, PR};
+  else if constexpr (COMPACT == 2)
+    return MaskVisit<NR, U, Rows, Src>{rows, src};
+  else
+    return PlainVisit<NR, U, PF, Rows, Src>{rows, src};
+}
 
 // Iterates target i's recorded neighbour list (slot-major, NL[k * stride + i])
 // in recorded order; the loads of four entries go out before the first is
@@ -259,6 +381,18 @@ struct ListVisit {
 };
 
 struct NoExtra {};
+
+// What the pair bodies read of a neighbour beyond PR: from global memory by
+// index, or from a staged row (from(tv, td, t)).
+struct ExtraNone {
+  __device__ __forceinline__ NoExtra operator()(int) const { return NoExtra{}; }
+  __device__ __forceinline__ NoExtra from(const float4*, const float*, int) const { return NoExtra{}; }
+};
+struct ExtraA {  // kernel A: velocity, pressure
+  const float4* VP;
+  __device__ __forceinline__ float4 operator()(int j) const { return ld16(VP, (unsigned)j * 16u); }
+  __device__ __forceinline__ float4 from(const float4* tv, const float*, int t) const { return tv[t]; }
+};
 
 // ------------------------------------------------------------ per target
 // The pair arithmetic below is written with explicit fmaf and compiled with
@@ -293,7 +427,7 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
         return rr <= cut && j != i && pj.w > 0.0f;
       },
-      [&](int j) { return ld16(VP, (unsigned)j * 16u); },
+      ExtraA{VP},
       [&](int j, const float4& pj, const float4& vj) {
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
@@ -325,6 +459,14 @@ struct NbrB {
   float4 v;  // velocity, pressure
   float dv;  // dofv
 };
+struct ExtraB {
+  const float4* VP;
+  const float* DV;
+  __device__ __forceinline__ NbrB operator()(int j) const {
+    return NbrB{ld16(VP, (unsigned)j * 16u), ld4(DV, (unsigned)j * 4u)};
+  }
+  __device__ __forceinline__ NbrB from(const float4* tv, const float* td, int t) const { return NbrB{tv[t], td[t]}; }
+};
 
 // Kernel B for fluid target i (all neighbours).
 template <int DIM, class Visit>
@@ -353,7 +495,7 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
         return rr <= cut && j != i;
       },
-      [&](int j) { return NbrB{ld16(VP, (unsigned)j * 16u), ld4(DV, (unsigned)j * 4u)}; },
+      ExtraB{VP, DV},
       [&](int j, const float4& pj, const NbrB& nb) {
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
@@ -458,30 +600,56 @@ __device__ __forceinline__ bool plan_planes(const sph_params& P, int coff, int k
   return fits;
 }
 
+// Row staging: as plan_planes, with every row's union within ROW_CAP.
+template <int DIM>
+__device__ __forceinline__ bool plan_rows(const sph_params& P, int coff, int key, bool active, bool same_inst,
+                                          const RowRange (&rows)[DIM == 3 ? 9 : 3], RowSrc<nrows<DIM>()>& src) {
+  constexpr int NR = nrows<DIM>();
+  const unsigned long long act = __ballot(active);
+  const int row_id = (key - coff) / P.gridxdim;
+  const int u_row = __builtin_amdgcn_readlane(row_id, __builtin_ctzll(act));
+  if (!same_inst || __ballot(active && row_id != u_row) != 0ull) return false;
+  bool fits = true;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const bool ne = rows[r].e > rows[r].s;
+    const int l = wave_min(ne ? rows[r].s : 0x7fffffff);
+    const int h = wave_max(ne ? rows[r].e : 0);
+    src.uni[r] = l == 0x7fffffff ? RowRange{0, 0} : RowRange{l, h};
+    fits = fits && src.uni[r].e - src.uni[r].s <= ROW_CAP;
+  }
+  src.cur = 0;
+  return fits;
+}
+
 // ------------------------------------------------------------ kernels
 // STAGED: 64-thread blocks with LDS staging (1: whole stencil, 2: plane by
-// plane, plain loop only); else up to 256 threads.
-// COMPACT: scan/dense loop; else the plain loop.
-template <int DIM, int U, int STAGED, bool COMPACT>
+// plane, 3: row by row with the neighbour data); else up to 256 threads.
+// COMPACT: 1 scan/dense loop, 2 mask groups; else the plain loop.
+template <int DIM, int U, int STAGED, int COMPACT>
 struct Cfg {
   static constexpr int NR = nrows<DIM>();
-  static constexpr int LIST = COMPACT ? COMPACT_CAP * (STAGED ? 64 : 256) : 1;
-  static constexpr int TILE = STAGED == 2 ? PLANE_CAP + STAGE_PAD : STAGED ? STAGE_CAP + STAGE_PAD : 1;
-  static constexpr int SLOTS = (!STAGED && !COMPACT) ? NR * 256 : 1;
+  static constexpr int LIST = COMPACT == 1 ? COMPACT_CAP * (STAGED ? 64 : 256) : 1;
+  static constexpr int TILE = STAGED == 3   ? ROW_CAP + ROW_PAD
+                             : STAGED == 2 ? PLANE_CAP + PLANE_PAD
+                             : STAGED      ? STAGE_CAP + STAGE_PAD
+                                           : 1;
+  static constexpr int TILEX = STAGED == 3 ? ROW_CAP + ROW_PAD : 1;  // staged neighbour data
+  static constexpr int SLOTS = (!STAGED && COMPACT != 1) ? NR * 256 : 1;
 };
 
 // Runs body(P, visit) for this lane with a visit loop over its stencil.  The
 // global path serves lanes instance by instance (wave-uniform parameters).
-template <int DIM, int U, int STAGED, bool COMPACT, bool PF, class Body>
+template <int DIM, int U, int STAGED, int COMPACT, bool PF, class Body>
 __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key, const float4& p, bool active,
-                                           const float4* PR,
+                                           const float4* PR, const float4* VPx, const float* DVx,
                                            const int* __restrict__ cs, const int* __restrict__ ce,
-                                           const int* __restrict__ fc, float4* tile, int* lists, RowRange* slots,
-                                           Body&& body) {
+                                           const int* __restrict__ fc, float4* tile, float4* tilev, float* tiled,
+                                           int* lists, RowRange* slots, Body&& body) {
   constexpr int NR = nrows<DIM>();
   int* list = lists + (threadIdx.x >> 6) * COMPACT_CAP * 64;
   auto global_path = [&](const sph_params& P, int coff) {
-    if constexpr (!STAGED && !COMPACT) {
+    if constexpr (!STAGED && COMPACT != 1) {
       // row ranges parked in the thread's own LDS slots (no barrier needed):
       // keeps 18 VGPRs free in 3-D, which buys occupancy for the plain loop
       RowRange rows[NR];
@@ -489,17 +657,41 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
       const int t = threadIdx.x;
 #pragma unroll
       for (int r = 0; r < NR; ++r) slots[r * 256 + t] = rows[r];
-      body(P, PlainVisit<NR, U, PF, LdsRows, GlobalSrc>{LdsRows{slots + t, 256}, GlobalSrc{PR}}, true);
+      body(P, make_visit<NR, U, COMPACT, PF>(LdsRows{slots + t, 256}, GlobalSrc{PR}, list, PR), true);
     } else {
       RegRows<NR> rows;
       row_ranges<DIM>(P, coff, key, p, cs, ce, fc, rows.v);
-      if constexpr (COMPACT)
-        body(P, CompactVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}, list, PR}, true);
-      else
-        body(P, PlainVisit<NR, U, PF, RegRows<NR>, GlobalSrc>{rows, GlobalSrc{PR}}, true);
+      body(P, make_visit<NR, U, COMPACT, PF>(rows, GlobalSrc{PR}, list, PR), true);
     }
   };
-  if constexpr (STAGED == 2) {
+  if constexpr (STAGED == 3) {
+    // as the plane path below, restaging at every row
+    if (__ballot(active) == 0ull) return;
+    const unsigned long long act = __ballot(active);
+    const int u = __builtin_amdgcn_readlane(inst, __builtin_ctzll(act));
+    const bool same_inst = __ballot(active && inst != u) == 0ull;
+    const sph_params& P = T.P[u];
+    const int coff = T.cell_off[u];
+    RegRows<NR> rows;
+    row_ranges<DIM>(P, coff, key, p, cs, ce, fc, rows.v);
+    if (!active) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) rows.v[r] = RowRange{0, 0};
+    }
+    RowSrc<NR> src;
+    src.PR = PR;
+    src.VP = VPx;
+    src.DV = DVx;
+    src.tp = tile;
+    src.tv = tilev;
+    src.td = tiled;
+    if (plan_rows<DIM>(P, coff, key, active, same_inst, rows.v, src)) {
+      body(P, make_visit<NR, U, COMPACT, PF>(rows, src, list, PR), active);
+    } else {
+      if (!active) return;
+      with_instance(T, inst, global_path);
+    }
+  } else if constexpr (STAGED == 2) {
     // the whole wave stays in the loop (inactive lanes with empty rows) for
     // the per-plane copies
     if (__ballot(active) == 0ull) return;
@@ -516,7 +708,7 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
     }
     PlaneSrc<NR> src;
     if (plan_planes<DIM>(P, coff, key, active, same_inst, rows.v, PR, tile, src)) {
-      body(P, PlainVisit<NR, U, PF, RegRows<NR>, PlaneSrc<NR>>{rows, src}, active);
+      body(P, make_visit<NR, U, COMPACT, PF>(rows, src, list, PR), active);
     } else {
       if (!active) return;
       with_instance(T, inst, global_path);
@@ -533,10 +725,7 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
     const bool staged = stage_rows<DIM>(P, coff, key, same_inst, rows.v, PR, tile, src);
     if (!active) return;
     if (staged) {
-      if constexpr (COMPACT)
-        body(P, CompactVisit<NR, U, PF, RegRows<NR>, StagedSrc<NR>>{rows, src, list, PR}, true);
-      else
-        body(P, PlainVisit<NR, U, PF, RegRows<NR>, StagedSrc<NR>>{rows, src}, true);
+      body(P, make_visit<NR, U, COMPACT, PF>(rows, src, list, PR), true);
     } else {
       with_instance(T, inst, global_path);
     }
@@ -546,8 +735,8 @@ __device__ __forceinline__ void with_visit(const InstTable& T, int inst, int key
   }
 }
 
-template <int DIM, int U, int STAGED, bool COMPACT, bool PF>
-__global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, float4* PR, float4* VP,
+template <int DIM, int U, int STAGED, int COMPACT, bool PF>
+__global__ __launch_bounds__(STAGED ? 64 : 256, STAGED >= 2 ? 6 : 1) void nsA_kernel(InstTable T, float4* PR, float4* VP,
                                                                 const int* __restrict__ TY, float* __restrict__ DV,
                                                                 const int* __restrict__ keys,
                                                                 const int* __restrict__ cs,
@@ -555,6 +744,8 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, flo
                                                                 const int* __restrict__ fc, int n, int remap) {
   using C = Cfg<DIM, U, STAGED, COMPACT>;
   __shared__ float4 tile[C::TILE];
+  __shared__ float4 tilev[C::TILEX];
+  __shared__ float tiled[C::TILEX];
   __shared__ int lists[C::LIST];
   __shared__ RowRange slots[C::SLOTS];
   const int i = tile_id(remap) * blockDim.x + threadIdx.x;
@@ -564,14 +755,16 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsA_kernel(InstTable T, flo
   const float4 pi = PR[ic];
   const float4 vi = VP[ic];
   const int key = keys[ic];
-  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid, PR, cs, ce, fc, tile, lists, slots,
+  // kernel A reads no neighbour's dofv (it writes its own)
+  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid, PR, VP, nullptr, cs, ce, fc, tile,
+                                          tilev, tiled, lists, slots,
                                       [&](const sph_params& P, auto visit, bool store) {
                                         nsA_target<DIM>(P, i, pi, vi, visit, PR, VP, DV, store);
                                       });
 }
 
-template <int DIM, int U, int STAGED, bool COMPACT, bool PF>
-__global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, const float4* __restrict__ PR,
+template <int DIM, int U, int STAGED, int COMPACT, bool PF>
+__global__ __launch_bounds__(STAGED ? 64 : 256, STAGED >= 2 ? 6 : 1) void nsB_kernel(InstTable T, const float4* __restrict__ PR,
                                                                 const float4* __restrict__ VP,
                                                                 const int* __restrict__ TY,
                                                                 const float* __restrict__ DV,
@@ -581,6 +774,8 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, con
                                                                 float4* __restrict__ RATE, int n, int remap) {
   using C = Cfg<DIM, U, STAGED, COMPACT>;
   __shared__ float4 tile[C::TILE];
+  __shared__ float4 tilev[C::TILEX];
+  __shared__ float tiled[C::TILEX];
   __shared__ int lists[C::LIST];
   __shared__ RowRange slots[C::SLOTS];
   const int i = tile_id(remap) * blockDim.x + threadIdx.x;
@@ -594,8 +789,8 @@ __global__ __launch_bounds__(STAGED ? 64 : 256) void nsB_kernel(InstTable T, con
   const float4 vi = VP[ic];
   const float dvi = DV[ic];
   const int key = keys[ic];
-  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid && fluid, PR, cs, ce, nullptr, tile,
-                                          lists, slots,
+  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid && fluid, PR, VP, DV, cs, ce,
+                                          nullptr, tile, tilev, tiled, lists, slots,
                                       [&](const sph_params& P, auto visit, bool store) {
                                         nsB_target<DIM>(P, i, pi, vi, dvi, visit, VP, DV, RATE, store);
                                       });
@@ -629,7 +824,7 @@ __global__ __launch_bounds__(256) void nbrlist_kernel(InstTable T, const float4*
           float dx, dy, dz;
           return geom<DIM>(pi, pj, dx, dy, dz) <= cut && j != i;
         },
-        [&](int) { return NoExtra{}; },
+        ExtraNone{},
         [&](int j, const float4&, const NoExtra&) {
           if (cnt < cap) NL[(size_t)cnt * n + i] = j;
           ++cnt;
@@ -697,7 +892,7 @@ __global__ __launch_bounds__(256) void nsB_list_kernel(InstTable T, const float4
 }
 
 // ------------------------------------------------------------ launchers
-template <int U, int STAGED, bool COMPACT, bool PF>
+template <int U, int STAGED, int COMPACT, bool PF>
 hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
                  hipStream_t s) {
   const int b = STAGED ? 64 : block;
@@ -713,7 +908,7 @@ hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n
   return hipGetLastError();
 }
 
-template <int U, int STAGED, bool COMPACT, bool PF>
+template <int U, int STAGED, int COMPACT, bool PF>
 hipError_t nsB_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
                  hipStream_t s) {
   const int b = STAGED ? 64 : block;
@@ -742,6 +937,11 @@ hipError_t nsA_u(const InstTable& T, const FastSet& st, const FastWork& w, int n
     case 7: return nsA_v<U, true, true, true>(T, st, w, n, block, remap, s);
     case 8: case 10: return nsA_v<U, 2, false, false>(T, st, w, n, block, remap, s);
     case 12: case 14: return nsA_v<U, 2, false, true>(T, st, w, n, block, remap, s);
+    case 16: return nsA_v<U, 0, 2, false>(T, st, w, n, block, remap, s);
+    case 24: return nsA_v<U, 2, 2, false>(T, st, w, n, block, remap, s);
+    case 32: return nsA_v<U, 3, 0, false>(T, st, w, n, block, remap, s);
+    case 36: return nsA_v<U, 3, 0, true>(T, st, w, n, block, remap, s);
+    case 48: return nsA_v<U, 3, 2, false>(T, st, w, n, block, remap, s);
     default: return nsA_v<U, false, false, false>(T, st, w, n, block, remap, s);
   }
 }
@@ -759,12 +959,22 @@ hipError_t nsB_u(const InstTable& T, const FastSet& st, const FastWork& w, int n
     case 7: return nsB_v<U, true, true, true>(T, st, w, n, block, remap, s);
     case 8: case 10: return nsB_v<U, 2, false, false>(T, st, w, n, block, remap, s);
     case 12: case 14: return nsB_v<U, 2, false, true>(T, st, w, n, block, remap, s);
+    case 16: return nsB_v<U, 0, 2, false>(T, st, w, n, block, remap, s);
+    case 24: return nsB_v<U, 2, 2, false>(T, st, w, n, block, remap, s);
+    case 32: return nsB_v<U, 3, 0, false>(T, st, w, n, block, remap, s);
+    case 36: return nsB_v<U, 3, 0, true>(T, st, w, n, block, remap, s);
+    case 48: return nsB_v<U, 3, 2, false>(T, st, w, n, block, remap, s);
     default: return nsB_v<U, false, false, false>(T, st, w, n, block, remap, s);
   }
 }
 
 int norm_block(int b) { return (b >= 64 && b <= 256 && b % 64 == 0) ? b : 256; }
 int mode_of(const sph_tuning& tu) {
+  // row staging runs the plain loop (with or without prefetch) or mask groups;
+  // mask groups run unstaged, plane- or row-staged, with their own
+  // next-neighbour loads
+  if (tu.nbr_lds == 3) return 32 | (tu.nbr_compact == 2 ? 16 : (tu.nbr_prefetch ? 4 : 0));
+  if (tu.nbr_compact == 2) return 16 | (tu.nbr_lds == 2 ? 8 : 0);
   return (tu.nbr_lds == 2 ? 8 : tu.nbr_lds ? 1 : 0) | (tu.nbr_compact ? 2 : 0) | (tu.nbr_prefetch ? 4 : 0);
 }
 

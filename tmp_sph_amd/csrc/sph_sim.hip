@@ -313,16 +313,16 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
     return set_error(SPH_E_INVAL, "nbr_block must be 64, 128, 192 or 256");
   if (t->nbr_unroll != 1 && t->nbr_unroll != 2 && t->nbr_unroll != 4 && t->nbr_unroll != 8)
     return set_error(SPH_E_INVAL, "nbr_unroll must be 1, 2, 4 or 8");
-  if (t->nbr_lds < 0 || t->nbr_lds > 2) return set_error(SPH_E_INVAL, "nbr_lds must be 0, 1 or 2");
+  if (t->nbr_lds < 0 || t->nbr_lds > 3) return set_error(SPH_E_INVAL, "nbr_lds must be 0, 1, 2 or 3");
+  if (t->nbr_compact < 0 || t->nbr_compact > 2) return set_error(SPH_E_INVAL, "nbr_compact must be 0, 1 or 2");
   const bool shape_changed = t->nbr_block != s->tun.nbr_block || t->nbr_unroll != s->tun.nbr_unroll ||
                              (t->xcd_remap != 0) != (s->tun.xcd_remap != 0) ||
                              t->nbr_lds != s->tun.nbr_lds ||
-                             (t->nbr_compact != 0) != (s->tun.nbr_compact != 0) ||
+                             t->nbr_compact != s->tun.nbr_compact ||
                              (t->nbr_prefetch != 0) != (s->tun.nbr_prefetch != 0) ||
                              (t->nbr_list != 0) != (s->tun.nbr_list != 0);
   s->tun = *t;
   s->tun.xcd_remap = t->xcd_remap ? 1 : 0;
-  s->tun.nbr_compact = t->nbr_compact ? 1 : 0;
   s->tun.nbr_prefetch = t->nbr_prefetch ? 1 : 0;
   s->tun.nbr_list = t->nbr_list ? 1 : 0;
   if (shape_changed) drop_graph(s);  // a captured graph holds the old launch shapes
@@ -574,15 +574,15 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   const sph_tuning saved = s->tun;
   sph_tuning pick = saved;
   double pick_us = 1e300;
-  const int blocks[] = {-1, 0, 64, 128, 256};
+  const int blocks[] = {-2, -1, 0, 64, 128, 256};
   const int unrolls[] = {1, 2, 4, 8};
   hipError_t err = hipSuccess;
   // the LDS-staged variants always run one-wave blocks: each is tried once
-  // per unroll/remap pair (bl == 0: whole stencil, bl == -1: plane by plane,
-  // plain loop only)
+  // per unroll/remap pair (bl == 0: whole stencil, -1: plane by plane,
+  // -2: row by row with the neighbour data)
   for (int list = 0; list < 2 && err == hipSuccess; ++list)
   for (int prefetch = 0; prefetch < 2 - list && err == hipSuccess; ++prefetch)
-  for (int compact = 0; compact < 2 - list && err == hipSuccess; ++compact)
+  for (int compact = 0; compact < (list ? 1 : 3) && err == hipSuccess; ++compact)
   for (int remap = 0; remap < 2 && err == hipSuccess; ++remap)
     for (int bl : blocks)
       for (int un : unrolls) {
@@ -591,12 +591,13 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
         t.nbr_block = bl <= 0 ? 64 : bl;
         t.nbr_unroll = un;
         t.xcd_remap = remap;
-        t.nbr_lds = bl == 0 ? 1 : (bl < 0 ? 2 : 0);
+        t.nbr_lds = bl == 0 ? 1 : bl == -1 ? 2 : bl == -2 ? 3 : 0;
         t.nbr_compact = compact;
         t.nbr_prefetch = prefetch;
         t.nbr_list = list;
         if (list && bl <= 0) continue;     // the list kernels have no LDS-staged form
-        if (compact && bl < 0) continue;  // plane staging has no compact loop
+        if (compact == 1 && bl < 0) continue;  // plane and row staging have no list loop
+        if (compact == 2 && (bl == 0 || prefetch)) continue;  // mask groups: unstaged or planes, own loads
         s->tun = t;
         err = rates_phase(s, 1, w, false);  // warm-up
         if (err == hipSuccess) err = hipEventRecord(a, s->stream);
