@@ -43,8 +43,7 @@ constexpr int STAGE_CAP = 1024;   // float4 entries per staged wave (16 KB of LD
 constexpr int STAGE_PAD = 8;      // spare tile entries read by unrolled loads past a row's end
 constexpr int PLANE_CAP = 448;    // float4 entries of one staged plane (7 KB of LDS)
 constexpr int PLANE_PAD = 32;     // spare plane entries read by a mask group past a row's end
-constexpr int ROW_CAP = 192;      // entries of one staged row (with its neighbour data: 8 KB of LDS)
-constexpr int ROW_PAD = 32;
+constexpr int ROW_CAP = 192;      // entries of one staged row (with its neighbour data: 7 KB of LDS)
 constexpr int COMPACT_CAP = 16;   // accepted candidates per lane per dense phase
 
 template <int DIM>
@@ -630,6 +629,7 @@ template <int DIM, int U, int STAGED, int COMPACT>
 struct Cfg {
   static constexpr int NR = nrows<DIM>();
   static constexpr int LIST = COMPACT == 1 ? COMPACT_CAP * (STAGED ? 64 : 256) : 1;
+  static constexpr int ROW_PAD = COMPACT == 2 ? 4 * U : U;  // entries read past a row's end
   static constexpr int TILE = STAGED == 3   ? ROW_CAP + ROW_PAD
                              : STAGED == 2 ? PLANE_CAP + PLANE_PAD
                              : STAGED      ? STAGE_CAP + STAGE_PAD
@@ -745,7 +745,7 @@ __global__ __launch_bounds__(STAGED ? 64 : 256, STAGED >= 2 ? 6 : 1) void nsA_ke
   using C = Cfg<DIM, U, STAGED, COMPACT>;
   __shared__ float4 tile[C::TILE];
   __shared__ float4 tilev[C::TILEX];
-  __shared__ float tiled[C::TILEX];
+  __shared__ float tiled[1];  // kernel A stages no dofv
   __shared__ int lists[C::LIST];
   __shared__ RowRange slots[C::SLOTS];
   const int i = tile_id(remap) * blockDim.x + threadIdx.x;
