@@ -66,15 +66,21 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const int* __restrict__
   __syncthreads();
   const unsigned mask = (1u << nbits) - 1u;
   const long long base = (long long)blockIdx.x * (RS_THREADS * RS_IPT) + (long long)w * (64 * RS_IPT);
-  for (int r = 0; r < RS_IPT; ++r) {
+  unsigned dd[RS_IPT];
+#pragma unroll
+  for (int r = 0; r < RS_IPT; ++r) {  // every load of the tile goes out before the first count
     const long long i = base + r * 64 + lane;
-    const bool valid = i < n;
-    const unsigned d = valid ? (((unsigned)keys[i]) >> shift) & mask : 0u;
+    dd[r] = i < n ? (((unsigned)keys[i]) >> shift) & mask : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < RS_IPT; ++r) {
+    const bool valid = base + r * 64 + lane < n;
     const unsigned long long active = __ballot(valid);
-    if (active == 0ull) break;  // wave-uniform
-    const unsigned long long peers = match_digit(d, nbits, active);
-    const int leader = __ffsll((long long)peers) - 1;
-    if (valid && lane == leader) atomicAdd(&cnt[d], (unsigned)__popcll(peers));
+    if (active != 0ull) {  // wave-uniform
+      const unsigned long long peers = match_digit(dd[r], nbits, active);
+      const int leader = __ffsll((long long)peers) - 1;
+      if (valid && lane == leader) atomicAdd(&cnt[dd[r]], (unsigned)__popcll(peers));
+    }
   }
   __syncthreads();
   hist[(size_t)tid * numTiles + blockIdx.x] = cnt[tid];
