@@ -185,7 +185,8 @@ int sph_PC_correction(float* sortedPos, float* sortedVel, float* sortedDensity,
 typedef struct sph_sim sph_sim;
 
 /* Launch-shape knobs of the fast path, filled with defaults by sph_sim_create
-   and chosen per workload by sph_sim_autotune.  None changes the results. */
+   (row staging with neighbour data, unroll 2, prefetch) and chosen per
+   workload by sph_sim_autotune.  None changes the results. */
 typedef struct sph_tuning {
     int nbr_block;       /* threads per block of the neighbour-loop kernels: 64, 128, 192, 256 */
     int nbr_unroll;      /* candidate loads issued together: 1, 2, 4 or 8 */

@@ -180,13 +180,15 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
     c += params[k].numCells;
   }
   s->bits = key_bits((int)ctot);
-  s->tun.nbr_block = 256;
-  s->tun.nbr_unroll = 4;
+  // default launch shape: row staging with neighbour data, unroll 2, prefetch
+  // (the autotuner's pick for the 2-D and 3-D dam breaks from 0.5 M particles up)
+  s->tun.nbr_block = 64;
+  s->tun.nbr_unroll = 2;
   s->tun.xcd_remap = 0;
   s->tun.flags = 0;
-  s->tun.nbr_lds = 0;
+  s->tun.nbr_lds = 3;
   s->tun.nbr_compact = 0;
-  s->tun.nbr_prefetch = 0;
+  s->tun.nbr_prefetch = 1;
   s->tun.nbr_list = 0;
   int rc = SPH_OK;
   hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);

@@ -384,12 +384,14 @@ def learned_vs_plain(g: "Gnn", case: Case, k: int, t_learned: float, t_plain: fl
     rollout (k plain steps per network step, plain step as judge and
     fallback) also records it.  The learned time to t_plain is the measured
     rollout time scaled by t_plain / t_learned (the rollout cost per interval
-    does not depend on the time reached)."""
+    does not depend on the time reached).  Both runs use the launch shape the
+    autotuner picks for the case."""
     import time
     dt = float(case.params.timeStep)
     intervals = max(int(round(t_learned / (dt * k))), 1)
     steps_plain = int(round(t_plain / dt))
     with Simulation(case) as sim:
+        tuning, _ = sim.autotune()
         sim.capture()
         sim.replay(k)  # warm-up
         sim.synchronize()
@@ -399,18 +401,20 @@ def learned_vs_plain(g: "Gnn", case: Case, k: int, t_learned: float, t_plain: fl
         plain_s = time.perf_counter() - t0
     front_p = np.zeros(intervals)
     with Simulation(case) as sim:
+        sim.set_tuning(**tuning)
         sim.capture()
         for it in range(intervals):
             sim.replay(k)
             front_p[it], _ = sim.probe(band)
     with Simulation(case) as sim:
+        sim.set_tuning(**tuning)
         st, front_l, height_l = g.rollout(sim, k, intervals, dev_tol=dev_tol, band=band)
     learned_s = st["seconds"] * (steps_plain / (intervals * k))
     rel = np.abs(front_l - front_p) / np.maximum(front_p, 1e-9)
     return {"particles": int(case.n), "fluid": int(case.n_fluid), "dt": dt, "k": k,
             "t_plain": steps_plain * dt, "plain_seconds": plain_s,
             "t_learned": intervals * k * dt, "learned_seconds_measured": st["seconds"],
-            "learned_seconds_to_t_plain": learned_s, "speedup": plain_s / learned_s,
+            "learned_seconds_to_t_plain": learned_s, "speedup": plain_s / learned_s, "tuning": tuning,
             "front_rel_error_max": float(rel.max()), "front_rel_error_final": float(rel[-1]),
             "front_plain_final": float(front_p[-1]), "front_learned_final": float(front_l[-1]),
             "rollout": st}
