@@ -75,6 +75,31 @@ def test_sort_many_ties_and_full_range(gpu, orc):
         np.testing.assert_array_equal(dk.numpy(), keys[want_i])
 
 
+@pytest.mark.parametrize("num_cells", [37, 300, 2048, 2049, 65536, 1 << 20, 1_500_000, 1 << 22, (1 << 22) + 1])
+def test_sort_digit_plans(gpu, num_cells):
+    """Key ranges from 6 to 23 bits (one to three 8-bit passes, a partial last
+    digit): random keys (bucket boundaries everywhere), nearly sorted keys
+    (runs of equal digits, the per-run peer match) and one repeated key."""
+    sph = gpu
+    prm = sph.reference_params()
+    prm.gridxdim, prm.gridydim, prm.gridzdim = num_cells, 1, 1  # only the key range matters here
+    prm.numCells = num_cells
+    sph.set_params(prm)
+    rng = np.random.default_rng(num_cells)
+    n = 200_003
+    near = np.sort(rng.integers(0, num_cells, n))
+    swap = rng.integers(0, n - 1, n // 50)
+    near[swap], near[swap + 1] = near[swap + 1], near[swap].copy()
+    for keys in (rng.integers(0, num_cells, n), near, np.full(n, num_cells - 1)):
+        keys = keys.astype(np.int32)
+        dk = sph.DeviceArray.from_host(keys)
+        di = sph.DeviceArray.from_host(np.arange(n, dtype=np.int32))
+        sph.sortParticles(dk, di, n)
+        want_i = np.argsort(keys, kind="stable").astype(np.int32)
+        np.testing.assert_array_equal(di.numpy(), want_i)
+        np.testing.assert_array_equal(dk.numpy(), keys[want_i])
+
+
 def test_empty_input(gpu):
     sph = gpu
     prm = sph.reference_params()
