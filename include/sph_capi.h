@@ -156,6 +156,15 @@ int sph_reorder(const float* oldPos, const float* oldVel, const float* oldDensit
                 int* sortedType, int* sortedZone, const int* particleHash,
                 const int* particleIndex, int* cellStart, int* cellEnd, int numParticles,
                 int* table_l, const int* table_r);
+/* recover (src/header/mesh.cuh:33, src/lib/mesh.cu:181-194, recoverD :162-179; src
+   variant only): scatters the
+   sorted arrays back to original order, row i of the sorted arrays to row
+   particleIndex[i].  Same argument order as the reference; zone and sortedZone may
+   both be null.  Needs no parameters; runs on the sph_set_stream stream. */
+int sph_recover(float* pos, float* vel, float* density, float* pressure, int numParticles,
+                const int* particleIndex, int* particleType, int* particleZone,
+                const float* sortedPos, const float* sortedVel, const float* sortedDensity,
+                const float* sortedPressure, const int* sortedType, const int* sortedZone);
 /* SPH_NS_simpleversion (src/header/ns.cuh:4, opt_src_v1/equation/ns.cu:5-230): kernel A
    (wall pressure extrapolation, velocity divergence dofv) then kernel B (drho/dt with
    delta-SPH diffusion, dv/dt with pressure, artificial viscosity, acoustic damper and

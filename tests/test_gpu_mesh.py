@@ -149,3 +149,52 @@ def test_reorder_bit_exact(gpu, orc, dim):
     np.testing.assert_array_equal(cs_g, want["cellStart"])  # -1 marks empty cells
     nonempty = want["cellStart"] != -1
     np.testing.assert_array_equal(ce_g[nonempty], want["cellEnd"][nonempty])
+
+
+@pytest.mark.parametrize("with_zone", [True, False])
+def test_recover_inverts_reorder(gpu, orc, with_zone):
+    """recover (src/lib/mesh.cu:181-194) scatters the sorted rows back: after
+    hash, sort and reorder of a shuffled 3-D case, recovering into fresh
+    arrays gives the input arrays bit for bit (the sort's index array is a
+    permutation, so the scatter is its inverse)."""
+    sph = gpu
+    case = case3d(0.04)
+    prm = case.params
+    sph.set_params(prm)
+    n = case.n
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(n)
+    host = dict(pos=case.pos[perm], vel=rng.standard_normal((n, 3)).astype(np.float32),
+                rho=case.rho[perm], p=rng.standard_normal(n).astype(np.float32), typ=case.type[perm],
+                zone=rng.integers(0, 9, n).astype(np.int32))
+    d = {k: sph.DeviceArray.from_host(v) for k, v in host.items()}
+    h = sph.DeviceArray((n,), np.int32)
+    i = sph.DeviceArray((n,), np.int32)
+    sph.calcHash(d["pos"], h, i, n)
+    sph.sortParticles(h, i, n)
+    srt = {k: sph.DeviceArray(v.shape, v.dtype) for k, v in host.items()}
+    cs = sph.DeviceArray((prm.numCells,), np.int32)
+    ce = sph.DeviceArray((prm.numCells,), np.int32)
+    sph.reorder(d["pos"], d["vel"], d["rho"], d["p"], d["typ"], d["zone"], srt["pos"], srt["vel"], srt["rho"],
+                srt["p"], srt["typ"], srt["zone"], h, i, cs, ce, n)
+    back = {k: sph.DeviceArray(v.shape, v.dtype) for k, v in host.items()}
+    z, sz = (back["zone"], srt["zone"]) if with_zone else (None, None)
+    sph.recover(back["pos"], back["vel"], back["rho"], back["p"], n, i, back["typ"], z, srt["pos"], srt["vel"],
+                srt["rho"], srt["p"], srt["typ"], sz)
+    for k in ("pos", "vel", "rho", "p", "typ") + (("zone",) if with_zone else ()):
+        np.testing.assert_array_equal(back[k].numpy(), host[k], err_msg=k)
+    # and the same scatter restated on the host from the sorted rows
+    idx = i.numpy()
+    want = np.empty_like(host["p"])
+    want[idx] = srt["p"].numpy()
+    np.testing.assert_array_equal(back["p"].numpy(), want)
+
+
+def test_recover_rejects_half_a_zone_pair(gpu):
+    from tmp_sph_amd._lib import SphError
+    sph = gpu
+    a = sph.DeviceArray((4, 3), np.float32)
+    b = sph.DeviceArray((4,), np.float32)
+    k = sph.DeviceArray((4,), np.int32)
+    with pytest.raises(SphError, match="sph_recover: bad argument"):
+        sph.recover(a, a, b, b, 4, k, k, k, a, a, b, b, k, None)

@@ -139,6 +139,7 @@ _PROTOS = {
     "sph_calcHash": [P, P, P, I],
     "sph_sortParticles": [P, P, I],
     "sph_reorder": [P] * 16 + [I, P, P],
+    "sph_recover": [P] * 4 + [I] + [P] * 9,
     "sph_SPH_NS_simpleversion": [P] * 9 + [I, P, I, P, P, P],
     "sph_PC_prediction": [P] * 11 + [I, P],
     "sph_PC_correction": [P] * 11 + [I, P],

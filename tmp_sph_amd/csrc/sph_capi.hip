@@ -241,6 +241,23 @@ extern "C" int sph_reorder(const float* oPos, const float* oVel, const float* oR
   return SPH_OK;
 }
 
+extern "C" int sph_recover(float* pos, float* vel, float* rho, float* p, int n, const int* idx, int* type,
+                           int* zone, const float* sPos, const float* sVel, const float* sRho, const float* sP,
+                           const int* sType, const int* sZone) {
+  if (n < 0 || (n > 0 && (!pos || !vel || !rho || !p || !idx || !type || !sPos || !sVel || !sRho || !sP ||
+                          !sType)) ||
+      ((zone == nullptr) != (sZone == nullptr)))
+    return set_error(SPH_E_INVAL, "sph_recover: bad argument");
+  hipStream_t s;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    s = g_stream;
+  }
+  SPH_HIP(launch_recover_aos(pos, vel, rho, p, type, zone, idx, sPos, sVel, sRho, sP, sType, sZone, n, s),
+          "recover");
+  return SPH_OK;
+}
+
 extern "C" int sph_SPH_NS_simpleversion(const float* sPos, const float* sVel, float* sRho, float* sP,
                                         const int* sType, float* densitydt, float* Veldt,
                                         const int* cellStart, const int* cellEnd, int n,

@@ -27,6 +27,9 @@ hipError_t launch_reorder_aos(const sph_params& P, const float* oPos, const floa
                               int* sType, int* sZone, const int* hash, const int* idx,
                               int* cellStart, int* cellEnd, int n, int* table_l,
                               const int* table_r, hipStream_t s);
+hipError_t launch_recover_aos(float* pos, float* vel, float* rho, float* p, int* type, int* zone, const int* idx,
+                              const float* sPos, const float* sVel, const float* sRho, const float* sP,
+                              const int* sType, const int* sZone, int n, hipStream_t s);
 hipError_t launch_ns_aos(const sph_params& P, const float* sPos, const float* sVel, float* sRho,
                          float* sP, const int* sType, float* drho, float* acc,
                          const int* cellStart, const int* cellEnd, const int* hash, float* dofv,

@@ -56,6 +56,29 @@ __global__ void reorder_aos_kernel(const float* __restrict__ oPos, const float* 
   if (sZone && oZone) sZone[i] = oZone[src];
 }
 
+// Scatter from sorted back to original order (recoverD, src/lib/mesh.cu:162-179):
+// row i of the sorted arrays goes to row idx[i].
+__global__ void recover_aos_kernel(float* __restrict__ pos, float* __restrict__ vel, float* __restrict__ rho,
+                                   float* __restrict__ p, int* __restrict__ type, int* __restrict__ zone,
+                                   const int* __restrict__ idx, const float* __restrict__ sPos,
+                                   const float* __restrict__ sVel, const float* __restrict__ sRho,
+                                   const float* __restrict__ sP, const int* __restrict__ sType,
+                                   const int* __restrict__ sZone, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = idx[i];
+  pos[3 * o] = sPos[3 * i];
+  pos[3 * o + 1] = sPos[3 * i + 1];
+  pos[3 * o + 2] = sPos[3 * i + 2];
+  vel[3 * o] = sVel[3 * i];
+  vel[3 * o + 1] = sVel[3 * i + 1];
+  vel[3 * o + 2] = sVel[3 * i + 2];
+  rho[o] = sRho[i];
+  p[o] = sP[i];
+  type[o] = sType[i];
+  if (zone && sZone) zone[o] = sZone[i];
+}
+
 // Visits the stencil rows of the cell `hash` in the reference order (z outer,
 // y, then the x-adjacent cells, which are contiguous in sorted order).
 // f(start, end) receives the sorted index range of one row.  Per-axis bounds
@@ -277,6 +300,15 @@ hipError_t launch_reorder_aos(const sph_params& P, const float* oPos, const floa
   hipLaunchKernelGGL(reorder_aos_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, oPos, oVel,
                      oRho, oP, oType, oZone, sPos, sVel, sRho, sP, sType, sZone, hash, idx,
                      cellStart, cellEnd, n, table_l, table_r);
+  return hipGetLastError();
+}
+
+hipError_t launch_recover_aos(float* pos, float* vel, float* rho, float* p, int* type, int* zone, const int* idx,
+                              const float* sPos, const float* sVel, const float* sRho, const float* sP,
+                              const int* sType, const int* sZone, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(recover_aos_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, pos, vel, rho, p, type, zone,
+                     idx, sPos, sVel, sRho, sP, sType, sZone, n);
   return hipGetLastError();
 }
 

@@ -196,6 +196,14 @@ def reorder(oldPos, oldVel, olddensity, oldpressure, oldparticle_type, oldpartic
          _p(dCellEnd), int(numParticles), _p(table_l), _p(table_r))
 
 
+def recover(pos, vel, density, pressure, numParticles, particleIndex, particle_type, particle_zone, sortedpos,
+            sortedvel, sorteddensity, sortedpressure, sorted_particle_type, sorted_particle_zone) -> None:
+    """src/lib/mesh.cu:181-194 (scatter sorted rows back to original order; src variant)."""
+    call("sph_recover", _p(pos), _p(vel), _p(density), _p(pressure), int(numParticles), _p(particleIndex),
+         _p(particle_type), _p(particle_zone), _p(sortedpos), _p(sortedvel), _p(sorteddensity), _p(sortedpressure),
+         _p(sorted_particle_type), _p(sorted_particle_zone))
+
+
 def SPH_NS_simpleversion(sortedPos, sortedVel, sorteddensity, sortedpressure, sorted_particle_type, densitydt,
                          Veldt, cellStart, cellEnd, numParticles, particleHash, timestep, dofv, rhop_sum=None,
                          w_sum=None) -> None:
