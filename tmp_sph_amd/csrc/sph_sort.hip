@@ -55,33 +55,53 @@ __device__ __forceinline__ unsigned long long match_digit(unsigned d, int nbits,
   return m;
 }
 
+// Per-tile digit counts.  Only counts are needed, so each thread takes RS_IPT
+// consecutive keys (16-byte loads when the keys are 16-byte aligned) and adds
+// each run of equal digits with one LDS atomic: nearly sorted keys give one or
+// two runs per thread.
 template <int RS_IPT>
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const int* __restrict__ keys, int n,
                                                          int shift, int nbits,
                                                          unsigned* __restrict__ hist,
-                                                         int numTiles) {
+                                                         int numTiles, int vec) {
   __shared__ unsigned cnt[256];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x;
   cnt[tid] = 0;
   __syncthreads();
   const unsigned mask = (1u << nbits) - 1u;
-  const long long base = (long long)blockIdx.x * (RS_THREADS * RS_IPT) + (long long)w * (64 * RS_IPT);
-  unsigned dd[RS_IPT];
+  constexpr unsigned NONE = 0xffffffffu;  // past the end of the keys
+  const long long b0 = (long long)blockIdx.x * (RS_THREADS * RS_IPT) + (long long)tid * RS_IPT;
+  unsigned d[RS_IPT];
+  if constexpr (RS_IPT % 4 == 0) {
+    if (vec && b0 + RS_IPT <= n) {  // b0 is a multiple of 4
+      const int4* p = reinterpret_cast<const int4*>(keys + b0);
 #pragma unroll
-  for (int r = 0; r < RS_IPT; ++r) {  // every load of the tile goes out before the first count
-    const long long i = base + r * 64 + lane;
-    dd[r] = i < n ? (((unsigned)keys[i]) >> shift) & mask : 0u;
-  }
+      for (int q = 0; q < RS_IPT / 4; ++q) {
+        const int4 v = p[q];
+        d[4 * q] = ((unsigned)v.x >> shift) & mask;
+        d[4 * q + 1] = ((unsigned)v.y >> shift) & mask;
+        d[4 * q + 2] = ((unsigned)v.z >> shift) & mask;
+        d[4 * q + 3] = ((unsigned)v.w >> shift) & mask;
+      }
+    } else {
 #pragma unroll
-  for (int r = 0; r < RS_IPT; ++r) {
-    const bool valid = base + r * 64 + lane < n;
-    const unsigned long long active = __ballot(valid);
-    if (active != 0ull) {  // wave-uniform
-      const unsigned long long peers = match_digit(dd[r], nbits, active);
-      const int leader = __ffsll((long long)peers) - 1;
-      if (valid && lane == leader) atomicAdd(&cnt[dd[r]], (unsigned)__popcll(peers));
+      for (int k = 0; k < RS_IPT; ++k) d[k] = b0 + k < n ? (((unsigned)keys[b0 + k]) >> shift) & mask : NONE;
     }
+  } else {
+#pragma unroll
+    for (int k = 0; k < RS_IPT; ++k) d[k] = b0 + k < n ? (((unsigned)keys[b0 + k]) >> shift) & mask : NONE;
   }
+  unsigned cur = d[0], run = 0u;
+#pragma unroll
+  for (int k = 0; k < RS_IPT; ++k) {
+    if (d[k] != cur) {
+      if (cur != NONE) atomicAdd(&cnt[cur], run);
+      cur = d[k];
+      run = 0u;
+    }
+    ++run;
+  }
+  if (cur != NONE) atomicAdd(&cnt[cur], run);
   __syncthreads();
   hist[(size_t)tid * numTiles + blockIdx.x] = cnt[tid];
 }
@@ -226,7 +246,9 @@ hipError_t radix_passes(int* keys, int* vals, int* keys_alt, int* vals_alt, unsi
   bool in_alt = false;
   for (int shift = 0; shift < bits; shift += 8) {
     const int nb = bits - shift < 8 ? bits - shift : 8;
-    hipLaunchKernelGGL(rs_upsweep<IPT>, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, n, shift, nb, hist, numTiles);
+    const int vec = ((uintptr_t)kin & 15u) == 0u;
+    hipLaunchKernelGGL(rs_upsweep<IPT>, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, n, shift, nb, hist, numTiles,
+                       vec);
     hipLaunchKernelGGL(rs_scan_rows, dim3(256), dim3(256), 0, s, hist, numTiles, totals);
     hipLaunchKernelGGL(rs_downsweep<IPT>, dim3(numTiles), dim3(RS_THREADS), 0, s, kin, vin, kout, vout, n, shift, nb,
                        hist, totals, numTiles);
