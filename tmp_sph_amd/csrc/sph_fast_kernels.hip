@@ -44,6 +44,16 @@ __global__ void hash_fast_kernel(InstTable T, const float4* __restrict__ PR, con
   idx[i] = i;
 }
 
+// cellStart = -1 (empty) and cellFluid = 0 for every cell, grid-stride.
+__global__ void clear_cells_kernel(int* __restrict__ cellStart, int* __restrict__ cellFluid, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    cellStart[i] = -1;
+    cellFluid[i] = 0;
+  }
+}
+
 template <int DIM>
 __global__ void cells_gather_kernel(FastSet in, FastSet out, const int* __restrict__ keys,
                                     const int* __restrict__ idx, int* __restrict__ cellStart,
@@ -308,9 +318,14 @@ hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, in
 hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
                                     const int* idx, int* cellStart, int* cellEnd, int* cellFluid, int n,
                                     hipStream_t s) {
-  hipError_t e = fill_int(cellStart, -1, T.total_cells, s);
-  if (e == hipSuccess) e = fill_int(cellFluid, 0, T.total_cells, s);
-  if (e != hipSuccess || n <= 0) return e;
+  const long long nc = T.total_cells;
+  if (nc > 0) {  // both cell tables cleared by one launch
+    const long long blocks = std::min<long long>((nc + 255) / 256, 4096);
+    hipLaunchKernelGGL(clear_cells_kernel, dim3((unsigned)blocks), dim3(256), 0, s, cellStart, cellFluid, nc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (n <= 0) return hipSuccess;
   return launch_dim(T.dim, cells_gather_kernel<2>, cells_gather_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), s,
                     in, out, keys, idx, cellStart, cellEnd, cellFluid, n);
 }
