@@ -128,6 +128,13 @@ int sph_get_params(sph_params* p);
 int sph_params_reference(sph_params* p, float dx, int numParticles, int numFluid);
 /* Parameters of a dam-break case (no reference counterpart). */
 int sph_params_dambreak(const sph_dambreak_cfg* cfg, sph_params* p);
+/* The same parameters on a cell grid divided by div (1, 2 or 4) along every
+   axis of the case (no reference counterpart; the reference's
+   opt_src_v4/sph/sph_run.cu:73 also searches a grid finer than 2h): gridSize /
+   div (exact), grid dimensions times div (gridydim stays 1 in 2-D), same
+   origin.  The neighbour search of such a grid visits 2*div+1 cells per axis
+   (see stencil reach in sph_tuning.cell_div). */
+int sph_params_subdivide(const sph_params* in, int div, sph_params* out);
 /* Counts of a dam-break case without generating it. */
 int sph_case_dambreak_count(const sph_dambreak_cfg* cfg, int* n_total, int* n_fluid);
 /* Generates the lattice case into caller-owned host arrays of capacity cap:
@@ -207,7 +214,14 @@ typedef struct sph_tuning {
                             data the pair bodies read (no global loads in the
                             loop); 2 and 3 run the plain loop or mask groups
                             (nbr_compact 1 is ignored); nbr_block is then
-                            ignored */
+                            ignored;
+                            4: fine rows -- any stencil reach (cell_div 1 or 2):
+                            the wave's targets are taken one cell row at a time,
+                            the cell table of each stencil row is read once per
+                            wave and each lane's candidate range is trimmed to
+                            the cells its support reaches; candidates and their
+                            neighbour data staged in LDS (nbr_compact, nbr_block
+                            and nbr_list ignored) */
     int nbr_compact;     /* 1: scan candidates into per-lane LDS lists, then run the
                             pair body densely over the accepted ones;
                             2: test groups of 4 x nbr_unroll candidates into a bit
@@ -217,6 +231,14 @@ typedef struct sph_tuning {
                             position instead of after the cut-off test */
     int nbr_list;        /* 1: one scan records each target's neighbours, kernels A
                             and B iterate the lists (block/unroll/remap apply) */
+    int cell_div;        /* 1: the neighbour search uses the instances' own cell grid;
+                            2: a grid with half the cell edge (sph_params_subdivide),
+                            2 * 2 + 1 cells per axis, about half the candidates of a
+                            2h grid in 3-D.  Keys and sorted order are those of the
+                            grid in use; the pairs are the same, visited row by row
+                            (z, then y), ascending sorted index within a row, as on
+                            any grid, so results match the oracle run on that grid.
+                            Needs nbr_lds 4 unless 1. */
 } sph_tuning;
 
 /* Creates a context on the current device from host arrays in the layout of
@@ -266,7 +288,8 @@ int sph_sim_download(sph_sim* s, float* pos, float* vel, float* rho, float* p, i
 /* FP64 positions in original order, AoS double[3N]. */
 int sph_sim_download_pos64(sph_sim* s, double* pos);
 /* Surge front (max x over in-domain fluid) and column height (max z over fluid
-   with x <= band) computed on the device; values are written to host doubles. */
+   with x <= band) computed on the device; values are written to host doubles
+   (any sign; 0 when no particle qualifies). */
 int sph_sim_probe(sph_sim* s, double band, double* front, double* height);
 /* Measuring probes of the reference's output step (src/io/io.cu:236-279),
    on the device.  Height gauges: for gauge g the largest z of a fluid

@@ -86,6 +86,17 @@ inline R eos_density(const sph_params& P, R p) {
 }
 
 // ------------------------------------------------------ neighbour search
+// Stencil reach of a grid: the smallest K >= 1 with K * gridSize >= 2h.  The
+// reference's grid (gridSize = 2h) has K = 1, its 27-cell search
+// (src/equation/ns.cu:37-55); a finer grid searches K cells on each side, as
+// the reference's last version does with cells of one spacing
+// (opt_src_v4/equation/ns.cu:62-93).
+inline int stencil_reach(const sph_params& P) {
+  int k = 1;
+  while (k < 8 && (float)k * P.gridSize < P.kh * 0.999999f) ++k;
+  return k;
+}
+
 // Calls f(j) for every candidate j of target cell `hash` in the reference
 // order: z outer, y, x inner over cells; ascending sorted index in a cell.
 template <int DIM, class F>
@@ -93,9 +104,10 @@ inline void for_candidates(const sph_params& P, int hash, const int* cellStart,
                            const int* cellEnd, F&& f) {
   const int X = P.gridxdim, Y = P.gridydim, Z = P.gridzdim, XY = X * Y;
   const int cz = hash / XY, cy = (hash % XY) / X, cx = hash % X;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = (DIM == 3 ? -1 : 0); dy <= (DIM == 3 ? 1 : 0); ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
+  const int K = stencil_reach(P);
+  for (int dz = -K; dz <= K; ++dz)
+    for (int dy = (DIM == 3 ? -K : 0); dy <= (DIM == 3 ? K : 0); ++dy)
+      for (int dx = -K; dx <= K; ++dx) {
         const int x = cx + dx, y = cy + dy, z = cz + dz;
         if (x < 0 || x >= X || y < 0 || y >= Y || z < 0 || z >= Z) continue;
         const int c = z * XY + y * X + x;

@@ -153,3 +153,22 @@ def test_case_loaded_from_dat_files_runs_like_the_generated_one(gpu, tmp_path):
         a.step(20)
         b.step(20)
         np.testing.assert_array_equal(a.pos64(), b.pos64())
+
+
+def test_contexts_leave_the_reference_surface_parameters_alone(gpu, tmp_path):
+    """Creating or restoring a context validates its instances' parameters
+    without replacing the process-wide block of the reference call surface
+    (sph_set_params / sph_get_params)."""
+    from tmp_sph_amd import sph
+    a = sph.reference_params()
+    sph.set_params(a)
+    case = _case(2)
+    assert bytes(case.params) != bytes(a)
+    ck = tmp_path / "p.ckpt"
+    with sph.Simulation(case) as s:
+        s.step(2)
+        s.checkpoint(ck)
+    assert bytes(sph.get_params()) == bytes(a)
+    with sph.Simulation.restore(ck):
+        pass
+    assert bytes(sph.get_params()) == bytes(a)

@@ -115,6 +115,31 @@ def test_probe_matches_numpy(gpu, orc):
     assert h == pytest.approx(p64[sel, 2].max(), abs=0)
 
 
+def test_probe_with_a_domain_at_negative_coordinates(gpu):
+    """The probe's maxima hold for any sign: the whole case (tank, grid,
+    escape box) shifted to negative x and z."""
+    case = case2d(0.02)
+    shift = np.array([-7.0, 0.0, -3.0], np.float32)
+    case.pos = (case.pos + shift).astype(np.float32)
+    prm = case.params
+    prm.gridxmin += float(shift[0])
+    prm.gridzmin += float(shift[2])
+    for k in (0, 2):
+        prm.domain_min[k] += float(shift[k])
+        prm.domain_max[k] += float(shift[k])
+    prm.park_pos[0] += float(shift[0])
+    prm.park_pos[2] += float(shift[2])
+    band = float(shift[0]) + 0.05
+    with gpu.Simulation(case) as sim:
+        sim.step(3)
+        f, h = sim.probe(band=band)
+        p64 = sim.pos64()
+    fl = case.type == 1
+    assert f == p64[fl, 0].max() and f < 0
+    sel = fl & (p64[:, 0] <= band)
+    assert h == p64[sel, 2].max() and h < 0
+
+
 def test_escaped_particle_is_parked(gpu):
     case = case2d(0.02)
     fl = np.flatnonzero(case.type == 1)
@@ -126,33 +151,55 @@ def test_escaped_particle_is_parked(gpu):
     assert np.all(st["vel"][fl[-1]] == 0) and st["p"][fl[-1]] == 0
 
 
-def test_full_size_hash_sort_c2(gpu, orc):
-    """2-D dam break at dx = 0.001 (500k fluid + 18k wall): hash, sorted order
-    bit-exact and rates within 1e-5 at the full size."""
+def _gpu_flow_state(gpu, case, steps):
+    """The case advanced `steps` steps by the GPU (graph replay), returned as a
+    new case in original particle order: a developed flow state at full size
+    that the oracle can evaluate once."""
+    with gpu.Simulation(case) as sim:
+        sim.capture()
+        sim.replay(steps)
+        sim.synchronize()
+        st = sim.state()
+    out = case.copy()
+    out.pos, out.vel, out.rho, out.p, out.type = st["pos"], st["vel"], st["rho"], st["p"], st["type"]
+    return out
+
+
+@pytest.mark.parametrize("steps", [0, 2000])
+def test_full_size_c2(gpu, orc, steps):
+    """2-D dam break at dx = 0.001 (500k fluid + 18k wall), at rest and after
+    2000 steps: hash, sorted order bit-exact and rates within 1e-5 at the
+    full size."""
     case = case2d(0.001)
+    if steps:
+        case = _gpu_flow_state(gpu, case, steps)
+        assert np.abs(case.vel[case.type == 1]).max() > 0.01  # the column has started to fall
     with gpu.Simulation(case) as sim:
         got = sim.eval_rates()
     want = orc.eval_rates(case.params, case, mode=1)
     _check_rates(case, got, want)
 
 
-def test_full_size_properties_c3(gpu, orc):
-    """3-D dam break at dx = 0.005 (2.0M fluid + 1.41M wall): hash and sorted
-    order bit-exact against the oracle; cell ranges consistent."""
+@pytest.mark.parametrize("steps", [0, 1000])
+def test_full_size_c3(gpu, orc, steps):
+    """3-D dam break at dx = 0.005 (2.0M fluid + 1.41M wall), at rest and after
+    1000 steps: hash and sorted order bit-exact against the oracle, rates
+    within 1e-5, and the default launch shape (row staging with its fallback
+    waves, 23-bit keys) bit-identical to the plain global loop."""
     from tmp_sph_amd.sph import dambreak_cfg, make_case
     case = make_case(dambreak_cfg(dim=3, dx=0.005, tank=(3.2, 1.0, 1.0), column=(1.0, 0.5, 0.5)))
     assert case.n_fluid == 2_000_000
+    if steps:
+        case = _gpu_flow_state(gpu, case, steps)
+        assert np.abs(case.vel[case.type == 1]).max() > 0.05
     with gpu.Simulation(case) as sim:
         got = sim.eval_rates()
-    h, i = orc.calcHash(case.params, case.pos)
-    hs, is_ = orc.sortParticles(case.params, h, i)
-    np.testing.assert_array_equal(got["keys"], hs)
-    np.testing.assert_array_equal(got["perm"], is_)
-    assert np.all(np.diff(got["keys"]) >= 0)
-    fl = case.type == 1
-    # at rest the interior is hydrostatic: |a_z| small against g away from walls and surface
-    assert np.isfinite(got["acc"]).all() and np.isfinite(got["drho"]).all()
-    assert np.all(got["rho"][~fl] >= case.params.restDensity)
+        sim.set_tuning(nbr_lds=0, nbr_compact=0, nbr_list=0, nbr_prefetch=0, nbr_block=256)
+        plain = sim.eval_rates()
+    for k in ("drho", "acc", "dofv", "p", "rho", "keys", "perm"):
+        np.testing.assert_array_equal(plain[k], got[k], err_msg=k)
+    want = orc.eval_rates(case.params, case, mode=1)
+    _check_rates(case, got, want)
 
 
 @pytest.mark.parametrize("dim", [2, 3])

@@ -105,6 +105,7 @@ class SphTuning(C.Structure):
         ("nbr_compact", C.c_int),
         ("nbr_prefetch", C.c_int),
         ("nbr_list", C.c_int),
+        ("cell_div", C.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -134,6 +135,7 @@ _PROTOS = {
     "sph_get_params": [C.POINTER(SphParams)],
     "sph_params_reference": [C.POINTER(SphParams), C.c_float, I, I],
     "sph_params_dambreak": [C.POINTER(DamBreakCfg), C.POINTER(SphParams)],
+    "sph_params_subdivide": [C.POINTER(SphParams), I, C.POINTER(SphParams)],
     "sph_case_dambreak_count": [C.POINTER(DamBreakCfg), IP, IP],
     "sph_case_dambreak": [C.POINTER(DamBreakCfg), I, P, P, P, P, P, C.POINTER(SphParams), IP, IP],
     "sph_calcHash": [P, P, P, I],

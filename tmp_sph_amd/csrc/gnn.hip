@@ -1028,7 +1028,10 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, f
       if ((he = forward(g)) != hipSuccess) { rc = hip_status(he, "rollout forward"); break; }
       hipLaunchKernelGGL(apply_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->cfg, k, g->Y, g->pos, vel, g->rho,
                          g->type, pos2, vel2, rho2, p2, n);
-      hipMemsetAsync(judge, 0, 5 * sizeof(float), s);
+      if ((he = hipMemsetAsync(judge, 0, 5 * sizeof(float), s)) != hipSuccess) {
+        rc = hip_status(he, "rollout judge clear");
+        break;
+      }
       hipLaunchKernelGGL(judge_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->off, g->send, g->pos, pos2, g->rho, rho2,
                          g->type, full_deg, judge, n);
       float jh[5];

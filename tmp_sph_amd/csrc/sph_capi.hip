@@ -64,6 +64,10 @@ int current_params(sph_params* out, hipStream_t* s) {
   return SPH_OK;
 }
 
+}  // namespace
+
+namespace sph {
+
 int check_params(const sph_params* p) {
   if (!p) return set_error(SPH_E_INVAL, "null params");
   if (p->dim != 2 && p->dim != 3) return set_error(SPH_E_INVAL, "params.dim must be 2 or 3");
@@ -81,6 +85,10 @@ int check_params(const sph_params* p) {
     return set_error(SPH_E_INVAL, "params: Tait EOS needs tait_B > 0 and gamma > 0");
   return SPH_OK;
 }
+
+}  // namespace sph
+
+namespace {
 
 int ensure_pool(int n) {
   if (g_pool.cap >= n && g_pool.keys_alt) return SPH_OK;

@@ -261,9 +261,19 @@ __global__ void download_rates_kernel(FastSet st, FastWork w, float* __restrict_
 // Surge front (max x of in-domain fluid) and column height (max z of fluid
 // with x <= band) per instance.  Both are >= 0 inside the domain, so the IEEE
 // bit patterns order like the values and an integer atomicMax reduces them.
+// Order-preserving map of a double onto an unsigned 64-bit key (negative
+// values: all bits flipped; others: sign bit set), so an unsigned atomicMax on
+// keys is a max on the doubles for any sign.  Key 0 (the cleared word) lies
+// below every finite value and means "no particle".
+__device__ __forceinline__ unsigned long long ordered_key(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
 __global__ void probe_kernel(InstTable T, FastSet st, double band, unsigned long long* __restrict__ out, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double front = 0.0, height = 0.0;
+  const double none = -__builtin_huge_val();
+  double front = none, height = none;
   int k = 0;
   if (i < n) {
     const int ty = st.TY[i];
@@ -285,14 +295,14 @@ __global__ void probe_kernel(InstTable T, FastSet st, double band, unsigned long
     const int first = __ffsll((long long)pending) - 1;
     const int u = __shfl(k, first);
     const bool mine = todo && k == u;
-    double f = mine ? front : 0.0, h = mine ? height : 0.0;
+    double f = mine ? front : none, h = mine ? height : none;
     for (int off = 32; off > 0; off >>= 1) {
       f = fmax(f, __shfl_xor(f, off));
       h = fmax(h, __shfl_xor(h, off));
     }
     if ((int)__lane_id() == first) {
-      atomicMax(&out[2 * u], (unsigned long long)__double_as_longlong(f));
-      atomicMax(&out[2 * u + 1], (unsigned long long)__double_as_longlong(h));
+      if (f != none) atomicMax(&out[2 * u], ordered_key(f));
+      if (h != none) atomicMax(&out[2 * u + 1], ordered_key(h));
     }
     if (mine) todo = false;
   }

@@ -291,3 +291,25 @@ extern "C" int sph_case_dambreak(const sph_dambreak_cfg* c, int cap, float* pos,
   if (n_fluid) *n_fluid = nf;
   return SPH_OK;
 }
+
+// Finer cell grid over the same box (no reference counterpart; the
+// reference's last version also searches a grid finer than 2h,
+// opt_src_v4/sph/sph_run.cu:73): the cell edge is divided by div (a power of
+// two, so gridSize / div is exact and every fine cell lies inside one cell of
+// the input grid), the cell counts are multiplied by div (y stays 1 in 2-D).
+extern "C" int sph_params_subdivide(const sph_params* in, int div, sph_params* out) {
+  if (!in || !out) return set_error(SPH_E_INVAL, "sph_params_subdivide: null argument");
+  if (div != 1 && div != 2 && div != 4) return set_error(SPH_E_INVAL, "sph_params_subdivide: div must be 1, 2 or 4");
+  sph_params t = *in;
+  t.gridSize = in->gridSize / (float)div;
+  t.gridxdim = in->gridxdim * div;
+  t.gridydim = in->dim == 3 ? in->gridydim * div : in->gridydim;
+  t.gridzdim = in->gridzdim * div;
+  const long long cells = (long long)t.gridxdim * t.gridydim * t.gridzdim;
+  if (cells > (1LL << 30)) return set_error(SPH_E_INVAL, "sph_params_subdivide: cell grid too large");
+  t.numCells = (int)cells;
+  const long long hm = cells + (long long)t.gridxdim * t.gridydim + t.gridxdim;
+  t.hash_max = hm > 0x7fffffffLL ? 0x7fffffff : (int)hm;
+  *out = t;
+  return SPH_OK;
+}

@@ -32,7 +32,7 @@
 // A context starts with rows, unroll 2 and prefetch (neighbour data read with
 // every candidate), the fastest shape measured for the 2-D and 3-D dam breaks
 // from 0.5 M particles up.
-#include "sph_fast_common.h"
+#include "sph_pair.h"
 
 namespace sph {
 
@@ -381,157 +381,6 @@ struct ListVisit {
     }
   }
 };
-
-struct NoExtra {};
-
-// What the pair bodies read of a neighbour beyond PR: from global memory by
-// index, or from a staged row (from(tv, td, t)).
-struct ExtraNone {
-  __device__ __forceinline__ NoExtra operator()(int) const { return NoExtra{}; }
-  __device__ __forceinline__ NoExtra from(const float4*, const float*, int) const { return NoExtra{}; }
-};
-struct ExtraA {  // kernel A: velocity, pressure
-  const float4* VP;
-  __device__ __forceinline__ float4 operator()(int j) const { return ld16(VP, (unsigned)j * 16u); }
-  __device__ __forceinline__ float4 from(const float4* tv, const float*, int t) const { return tv[t]; }
-};
-
-// ------------------------------------------------------------ per target
-// The pair arithmetic below is written with explicit fmaf and compiled with
-// contraction off, so that every loop variant (whose code the compiler
-// schedules differently) rounds identically.  The squared distance is the
-// reference's unfused sum, so the pair set is exactly the reference's.
-template <int DIM>
-__device__ __forceinline__ float geom(const float4& pi, const float4& pj, float& dx, float& dy, float& dz) {
-#pragma clang fp contract(off)
-  dx = pi.x - pj.x;
-  dy = DIM == 3 ? pi.y - pj.y : 0.0f;
-  dz = pi.z - pj.z;
-  return DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
-}
-
-// Kernel A for target i (fluid neighbours only).
-template <int DIM, class Visit>
-__device__ __forceinline__ void nsA_target(const sph_params& P, int i, const float4& pi, const float4& vi,
-                                           Visit visit, float4* PR, float4* VP, float* __restrict__ DV,
-                                           bool store = true) {
-#pragma clang fp contract(off)
-  const bool fi = pi.w > 0.0f;
-  const float inv_h = 1.0f / P.h;
-  const float cfr = -5.0f * P.adh / (P.h * P.h);
-  const float mg = 0.0f - P.gravity;
-  const float m = P.particleMass;
-  const float cut = P.rr_cut;
-  float sp = 0.0f, sw = 0.0f, dv = 0.0f;
-  visit(
-      [&](int j, const float4& pj) {
-        float dx, dy, dz;
-        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        return rr <= cut && j != i && pj.w > 0.0f;
-      },
-      ExtraA{VP},
-      [&](int j, const float4& pj, const float4& vj) {
-        float dx, dy, dz;
-        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        const float q = fast_sqrt(rr) * inv_h;
-        const float t = fmaf(-0.5f, q, 1.0f);
-        const float t3 = t * t * t;
-        if (!fi) {
-          const float w = (t3 * t) * (fmaf(2.0f, q, 1.0f) * P.adh);
-          sp = fmaf(fmaf(-pj.w, mg * dz, vj.w), w, sp);
-          sw += w;
-        } else {
-          const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
-                                      : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
-          dv = fmaf(-(cfr * t3 * m), dvdx * fast_rcp(pj.w), dv);
-        }
-      });
-  if (!store) return;
-  DV[i] = dv;
-  if (!fi) {
-    float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
-    if (pw < 0.0f) pw = 0.0f;
-    reinterpret_cast<float*>(VP)[4 * i + 3] = pw;
-    reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
-  }
-}
-
-// What kernel B reads of a neighbour beyond PR.
-struct NbrB {
-  float4 v;  // velocity, pressure
-  float dv;  // dofv
-};
-struct ExtraB {
-  const float4* VP;
-  const float* DV;
-  __device__ __forceinline__ NbrB operator()(int j) const {
-    return NbrB{ld16(VP, (unsigned)j * 16u), ld4(DV, (unsigned)j * 4u)};
-  }
-  __device__ __forceinline__ NbrB from(const float4* tv, const float* td, int t) const { return NbrB{tv[t], td[t]}; }
-};
-
-// Kernel B for fluid target i (all neighbours).
-template <int DIM, class Visit>
-__device__ __forceinline__ void nsB_target(const sph_params& P, int i, const float4& pi, const float4& vi, float dvi,
-                                           Visit visit, const float4* __restrict__ VP,
-                                           const float* __restrict__ DV, float4* __restrict__ RATE,
-                                           bool store = true) {
-#pragma clang fp contract(off)
-  const float rho_i = pi.w;
-  const float inv_h = 1.0f / P.h;
-  const float cfr = -5.0f * P.adh / (P.h * P.h);
-  const float m = P.particleMass;
-  const float dhc = P.delta * P.h * P.cs;
-  const float eta2 = P.eta * P.eta;
-  const float cut = P.rr_cut;
-  const float damper = P.damper;
-  const float afa = P.afa;
-  const float dofv_i = damper * dvi;
-  const float inv_rho_i = 1.0f / rho_i;
-  const float avc = P.h * P.cs * P.restDensity * m * inv_rho_i;
-  const float pm_i = m * inv_rho_i;
-  float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
-  visit(
-      [&](int j, const float4& pj) {
-        float dx, dy, dz;
-        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        return rr <= cut && j != i;
-      },
-      ExtraB{VP, DV},
-      [&](int j, const float4& pj, const NbrB& nb) {
-        float dx, dy, dz;
-        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        const float4 vj = nb.v;
-        const bool fj = pj.w > 0.0f;
-        const float rho_j = fabsf(pj.w);
-        const float q = fast_sqrt(rr) * inv_h;
-        const float t = fmaf(-0.5f, q, 1.0f);
-        const float fr = cfr * (t * t * t);
-        const float inv_rho_j = fast_rcp(rho_j);
-        const float inv_re = fast_rcp(rr + eta2);
-        const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
-                                    : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
-        const float frj = fr * inv_rho_j;
-        drho = fmaf(frj * m, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
-        float vt = -(vi.w + vj.w) * pm_i;
-        if (fj) vt = fmaf(avc, fmaf(damper, nb.dv, dofv_i) + afa * dvdx * inv_re, vt);
-        vt *= frj;
-        ax = fmaf(vt, dx, ax);
-        if (DIM == 3) ay = fmaf(vt, dy, ay);
-        az = fmaf(vt, dz, az);
-      });
-  if (store) RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
-}
-
-// ------------------------------------------------------------ staging
-__device__ __forceinline__ int wave_min(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-  return __builtin_amdgcn_readfirstlane(v);
-}
-__device__ __forceinline__ int wave_max(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-  return __builtin_amdgcn_readfirstlane(v);
-}
 
 // Decides whether the wave can stage its candidates (all 64 targets in one
 // cell row of one instance, union of the row ranges within STAGE_CAP) and, if

@@ -150,8 +150,7 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
     return set_error(SPH_E_INVAL, "sph_sim_create: bad argument");
   long long ntot = 0, ctot = 0;
   for (int k = 0; k < K; ++k) {
-    sph_params tmp = params[k];
-    int rc = sph_set_params(&tmp);  // validates each instance
+    int rc = check_params(&params[k]);  // the process-wide block of the reference surface stays as it is
     if (rc) return rc;
     rc = check_instance(params[k], params[0].dim);
     if (rc) return rc;
@@ -439,9 +438,17 @@ extern "C" int sph_sim_probe_batch(sph_sim* s, double band, double* front, doubl
   SPH_HIP(hipMemcpyAsync(hv.data(), s->probe, words * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream),
           "probe copy");
   SPH_HIP(hipStreamSynchronize(s->stream), "probe sync");
+  // inverse of the kernel's order-preserving key; key 0: no fluid particle (0.0)
+  auto decode = [](unsigned long long key) {
+    if (key == 0ull) return 0.0;
+    const unsigned long long b = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+    double v;
+    std::memcpy(&v, &b, 8);
+    return v;
+  };
   for (int k = 0; k < s->K; ++k) {
-    std::memcpy(&front[k], &hv[2 * k], 8);
-    std::memcpy(&height[k], &hv[2 * k + 1], 8);
+    front[k] = decode(hv[2 * k]);
+    height[k] = decode(hv[2 * k + 1]);
   }
   return SPH_OK;
 }
