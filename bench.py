@@ -301,6 +301,8 @@ def main() -> int:
     ap.add_argument("--nbr-compact", type=int, default=-1)
     ap.add_argument("--nbr-prefetch", type=int, default=-1)
     ap.add_argument("--nbr-list", type=int, default=-1)
+    ap.add_argument("--cell-div", type=int, default=0, help="1: the case's 2h grid, 2: half the cell edge "
+                    "(with --nbr-lds 4)")
     ap.add_argument("--output-every", type=int, default=0, metavar="M",
                     help="also time VTK frame output every M steps: asynchronous writer vs download-and-write")
     ap.add_argument("--output-frames", type=int, default=4)
@@ -334,6 +336,8 @@ def main() -> int:
                                      ("nbr_compact", args.nbr_compact), ("nbr_prefetch", args.nbr_prefetch),
                                      ("nbr_list", args.nbr_list))
                   if v >= 0})
+    if args.cell_div > 0:
+        fixed["cell_div"] = args.cell_div
     if fixed:
         sim.set_tuning(**fixed)
         tuned = sim.tuning.as_dict()

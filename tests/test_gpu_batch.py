@@ -76,7 +76,8 @@ _VARIANTS = ([(lds, compact, pf, 0, un) for lds in (0, 1) for compact in (0, 1) 
              + [(2, 0, pf, 0, un) for pf in (0, 1) for un in (1, 4, 8)]
              + [(lds, 2, 0, 0, un) for lds in (0, 2, 3) for un in (1, 2, 8)]
              + [(3, 0, pf, 0, un) for pf in (0, 1) for un in (1, 4, 8)]
-             + [(0, 0, 0, 1, un) for un in (1, 4, 8)])
+             + [(0, 0, 0, 1, un) for un in (1, 4, 8)]
+             + [(4, 0, pf, 0, un) for pf in (0, 1) for un in (1, 2, 4)])
 
 
 @pytest.mark.parametrize("dim", [2, 3])
@@ -123,6 +124,42 @@ def test_neighbour_loop_variants_batched(gpu):
             continue
         np.testing.assert_array_equal(r, ref, err_msg=f"lds={lds} compact={compact} prefetch={pf} list={lst} "
                                                      f"unroll={un}")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fine_rows_on_the_halved_grid_bit_identical_across_shapes(gpu, dim):
+    """On the halved grid every fine-row launch shape (unroll, prefetch,
+    XCD remap) gives bit-identical rates and steps; a batch gives every
+    instance the results of running it alone."""
+    from tmp_sph_amd.sph import dambreak_cfg, make_case
+    if dim == 3:
+        cases = [make_case(dambreak_cfg(dim=3, dx=0.03, tank=(0.8, 0.4, 0.5), column=(0.3, 0.4, 0.3)))]
+    else:
+        cases = _cases()
+    ref = None
+    for un, pf, rm in ((1, 0, 0), (2, 1, 0), (4, 0, 1), (4, 1, 1)):
+        with gpu.Simulation(cases) as s:
+            s.set_tuning(nbr_lds=4, cell_div=2, nbr_unroll=un, nbr_prefetch=pf, xcd_remap=rm)
+            s.step(3)
+            r = s.eval_rates()
+            s.step(2)
+            r["pos"] = s.pos64()
+        if ref is None:
+            ref = r
+            continue
+        for k in ("drho", "acc", "dofv", "p", "rho", "pos", "keys", "perm"):
+            np.testing.assert_array_equal(r[k], ref[k], err_msg=f"unroll={un} prefetch={pf} remap={rm} {k}")
+    if len(cases) > 1:
+        with gpu.Simulation(cases) as b:
+            b.set_tuning(nbr_lds=4, cell_div=2)
+            b.step(6)
+            pb = b.pos64()
+            offs = b.offsets
+        for k, c in enumerate(cases):
+            with gpu.Simulation(c) as s:
+                s.set_tuning(nbr_lds=4, cell_div=2)
+                s.step(6)
+                np.testing.assert_array_equal(pb[offs[k]:offs[k + 1]], s.pos64(), err_msg=f"instance {k}")
 
 
 def test_c4_sweep_keys_order_and_members(gpu, orc):

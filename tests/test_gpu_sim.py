@@ -56,6 +56,41 @@ def test_eval_rates_match_oracle(gpu, orc, which):
     _check_rates(case, got, want)
 
 
+@pytest.mark.parametrize("which", ["2d_flow", "3d_flow"])
+@pytest.mark.parametrize("cell_div", [1, 2])
+def test_fine_rows_match_oracle(gpu, orc, which, cell_div):
+    """Fine rows (nbr_lds 4) on the instances' grid and on the grid of half
+    the cell edge: keys and order bit-exact against the oracle run on that
+    grid, rates within 1e-5 of it (the oracle visits 2K+1 cells per axis in
+    the same row order)."""
+    case = advanced(orc, case2d(0.02), 200) if which == "2d_flow" else advanced(orc, case3d(0.04), 50)
+    with gpu.Simulation(case) as sim:
+        sim.set_tuning(nbr_lds=4, cell_div=cell_div)
+        got = sim.eval_rates()
+    grid = case.copy()
+    grid.params = gpu.subdivide_params(case.params, cell_div)
+    want = orc.eval_rates(grid.params, grid, mode=1)
+    _check_rates(grid, got, want)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fine_rows_visit_the_same_pairs(gpu, dim):
+    """The in-support pairs the fine rows visit on the halved grid are those
+    of the 2h grid (statistics of both visits), with fewer candidates."""
+    case = case2d(0.01) if dim == 2 else case3d(0.03)
+    with gpu.Simulation(case) as sim:
+        sim.step(3)
+        coarse = sim.pair_stats()
+        sim.set_tuning(nbr_lds=4, cell_div=1)
+        fine1 = sim.pair_stats()
+        sim.set_tuning(nbr_lds=4, cell_div=2)
+        fine2 = sim.pair_stats()
+    print(dim, coarse, fine1, fine2)
+    for k in ("hitsA", "hitsB"):
+        assert fine1[k] == coarse[k] and fine2[k] == coarse[k], k
+    assert fine2["candB"] < coarse["candB"]
+
+
 @pytest.mark.parametrize("dim,steps", [(2, 20), (3, 8)])
 def test_steps_match_oracle(gpu, orc, dim, steps):
     case = advanced(orc, case2d(0.02) if dim == 2 else case3d(0.04), 30)

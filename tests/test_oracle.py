@@ -236,6 +236,14 @@ def test_neighbour_search_independent_of_grid_and_order(orc, dim):
         prm.gridydim = int(np.ceil(prm.gridydim / 1.37)) + 1
     prm.numCells = prm.gridxdim * prm.gridydim * prm.gridzdim
     out = orc.eval_rates(prm, coarse, mode=2)
+    # finer grids (edge h, h/2): the stencil reaches 2 and 4 cells on each side
+    from tmp_sph_amd import sph
+    fine = []
+    for div in (2, 4):
+        f = case.copy()
+        f.params = sph.subdivide_params(case.params, div)
+        assert f.params.numCells == case.params.numCells * div ** dim
+        fine.append(orc.eval_rates(f.params, f, mode=2))
     rng = np.random.default_rng(2)
     perm = rng.permutation(case.n)
     sh = case.copy()
@@ -243,7 +251,7 @@ def test_neighbour_search_independent_of_grid_and_order(orc, dim):
     o2 = orc.eval_rates(case.params, sh, mode=2)
     inv = np.argsort(perm)
     fl = case.type == 1
-    for o, remap in ((out, np.arange(case.n)), (o2, inv)):
+    for o, remap in ((out, np.arange(case.n)), (o2, inv), (fine[0], np.arange(case.n)), (fine[1], np.arange(case.n))):
         assert rel_err(o["drho"][remap][fl], base["drho"][fl], base["mag_drho"][fl]) < 1e-9
         assert rel_err(o["acc"][remap][fl], base["acc"][fl], base["mag_acc"][fl]) < 1e-9
         assert rel_err(o["p"][remap][~fl], base["p"][~fl], 1.0) < 1e-9

@@ -102,6 +102,14 @@ hipError_t launch_nbrlist_fast(const InstTable& T, const FastSet& st, const Fast
                                hipStream_t s);
 hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s);
+// Fine-row loops (tuning nbr_lds 4): any stencil reach, per-lane trimmed
+// candidate ranges, one cell-table read per wave and stencil row.
+hipError_t launch_nsA_fine(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                           hipStream_t s);
+hipError_t launch_nsB_fine(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
+                           hipStream_t s);
+hipError_t launch_pair_stats_fine(const InstTable& T, const FastSet& st, const FastWork& w, unsigned long long* out4,
+                                  int n, hipStream_t s);
 hipError_t launch_pc_fast(const InstTable& T, bool predictor, const FastSet& st, const FastWork& w,
                           const int* perm, int n, hipStream_t s);
 hipError_t launch_upload_fast(int dim, const FastSet& st, const float* pos, const float* vel, const float* rho,

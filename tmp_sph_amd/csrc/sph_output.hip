@@ -32,13 +32,15 @@ namespace {
 
 // ------------------------------------------------------------------ checkpoint
 
-constexpr char kMagic[8] = {'S', 'P', 'H', 'C', 'K', 'P', 'T', '1'};
+constexpr char kMagic[8] = {'S', 'P', 'H', 'C', 'K', 'P', 'T', '2'};
 
 struct CkptHeader {
   char magic[8];
   uint32_t params_bytes;  // sizeof(sph_params) of the writer
   int32_t K, n, dim;
   int64_t steps;
+  int32_t cell_div;       // cell grid the sorted order follows (tuning.cell_div)
+  int32_t reserved;
 };
 
 // FNV-1a, 64-bit, over every byte after the header's magic.
@@ -195,6 +197,7 @@ extern "C" int sph_sim_checkpoint(sph_sim* s, const char* path) {
   h.n = s->n;
   h.dim = s->dim;
   h.steps = s->steps;
+  h.cell_div = s->tun.cell_div;
   Fnv sum;
   sum.add((const char*)&h + 8, sizeof(h) - 8);
   sum.add(s->P.data(), sizeof(sph_params) * s->K);
@@ -223,7 +226,8 @@ extern "C" int sph_sim_restore(const char* path, sph_sim** out) {
   if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0)
     return fail("not a checkpoint");
   if (h.params_bytes != sizeof(sph_params)) return fail("written by a build with another parameter layout");
-  if (h.K <= 0 || h.n <= 0 || h.K > h.n || (h.dim != 2 && h.dim != 3)) return fail("corrupt header");
+  if (h.K <= 0 || h.n <= 0 || h.K > h.n || (h.dim != 2 && h.dim != 3) || (h.cell_div != 1 && h.cell_div != 2))
+    return fail("corrupt header");
   std::vector<sph_params> P(h.K);
   if (std::fread(P.data(), sizeof(sph_params), h.K, f) != (size_t)h.K) return fail("truncated");
   long long total = 0;
@@ -269,6 +273,12 @@ extern "C" int sph_sim_restore(const char* path, sph_sim** out) {
     return set_error(SPH_E_INVAL, "sph_sim_restore: dimension mismatch");
   }
   rc = copy_set(s->set[0], a, n, hipMemcpyHostToDevice);
+  if (rc == SPH_OK && h.cell_div != 1) {  // the grid changes the order of the sums: keep the saved one
+    sph_tuning t = s->tun;
+    t.cell_div = h.cell_div;
+    t.nbr_lds = 4;
+    rc = sph_sim_set_tuning(s, &t);
+  }
   if (rc) {
     sph_sim_destroy(s);
     return rc;

@@ -92,6 +92,12 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
 }
 
 hipError_t rates_phase(sph_sim* s, int set, const FastWork& w, bool timing) {
+  if (s->tun.nbr_lds == 4) {
+    hipError_t e =
+        timed(s, timing, "nsA", [&] { return launch_nsA_fine(s->T, s->set[set], w, s->n, s->tun, s->stream); });
+    if (e != hipSuccess) return e;
+    return timed(s, timing, "nsB", [&] { return launch_nsB_fine(s->T, s->set[set], w, s->n, s->tun, s->stream); });
+  }
   if (s->tun.nbr_list) {
     hipError_t e =
         timed(s, timing, "nbrlist", [&] { return launch_nbrlist_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
@@ -100,6 +106,12 @@ hipError_t rates_phase(sph_sim* s, int set, const FastWork& w, bool timing) {
   hipError_t e = timed(s, timing, "nsA", [&] { return launch_nsA_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
   if (e != hipSuccess) return e;
   return timed(s, timing, "nsB", [&] { return launch_nsB_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
+}
+
+// Selects the cell grid of tuning.cell_div (1: the instances' own, 2: halved).
+void use_grid(sph_sim* s, int cell_div) {
+  s->T = cell_div == 2 ? s->T2 : s->T1;
+  s->bits = cell_div == 2 ? s->bits2 : s->bits1;
 }
 
 hipError_t half_step(sph_sim* s, int from, int to, bool predictor, bool timing) {
@@ -160,6 +172,22 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   }
   if (ntot > 0x7fffffffLL || ctot > (1LL << 30))
     return set_error(SPH_E_INVAL, "sph_sim_create: too many particles or cells");
+  // the same grids with half the cell edge, for tuning.cell_div = 2
+  std::vector<sph_params> P2(K);
+  std::vector<int> off2(K);
+  long long ctot2 = 0;
+  bool fine_ok = true;
+  for (int k = 0; k < K && fine_ok; ++k) {
+    if (sph_params_subdivide(&params[k], 2, &P2[k]) != SPH_OK) {
+      fine_ok = false;  // cell_div 2 stays unavailable; the context itself is fine
+      break;
+    }
+    P2[k].numParticles = counts[k];
+    off2[k] = (int)std::min<long long>(ctot2, 1LL << 30);
+    ctot2 += P2[k].numCells;
+  }
+  if (ctot2 > (1LL << 30)) fine_ok = false;
+  const long long cmax = std::max(ctot, fine_ok ? ctot2 : 0LL);
   const size_t N = (size_t)ntot;
   std::vector<int> inst(N);
   for (int k = 0, o = 0; k < K; ++k)
@@ -178,7 +206,9 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
     s->P[k].numParticles = counts[k];
     c += params[k].numCells;
   }
-  s->bits = key_bits((int)ctot);
+  s->bits1 = key_bits((int)ctot);
+  s->bits2 = fine_ok ? key_bits((int)ctot2) : 1;
+  s->bits = s->bits1;
   // default launch shape: row staging with neighbour data, unroll 2, prefetch
   // (the autotuner's pick for the 2-D and 3-D dam breaks from 0.5 M particles up)
   s->tun.nbr_block = 64;
@@ -189,6 +219,7 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   s->tun.nbr_compact = 0;
   s->tun.nbr_prefetch = 1;
   s->tun.nbr_list = 0;
+  s->tun.cell_div = 1;
   int rc = SPH_OK;
   hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (he != hipSuccess) {
@@ -214,9 +245,9 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   rc = rc ? rc : dalloc(s, &w.idx_alt, N);
   rc = rc ? rc : dalloc(s, &w.hist, (size_t)radix_hist_words(s->n));
   rc = rc ? rc : dalloc(s, &w.totals, 256);
-  rc = rc ? rc : dalloc(s, &w.cellStart, (size_t)ctot);
-  rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)ctot);
-  rc = rc ? rc : dalloc(s, &w.cellFluid, (size_t)ctot);
+  rc = rc ? rc : dalloc(s, &w.cellStart, (size_t)cmax);
+  rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)cmax);
+  rc = rc ? rc : dalloc(s, &w.cellFluid, (size_t)cmax);
   rc = rc ? rc : dalloc(s, &w.DV, N + CAND_PAD);
   w.nl_cap = d3 ? NL_CAP_3D : NL_CAP_2D;
   rc = rc ? rc : dalloc(s, &w.NL, N * (size_t)w.nl_cap);
@@ -228,24 +259,34 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   rc = rc ? rc : dalloc(s, &w.TV, N);
   rc = rc ? rc : dalloc(s, &s->probe, (size_t)std::max(2 * K, 4));
   rc = rc ? rc : dalloc(s, &s->gbuf, (size_t)5 * SPH_MAX_PROBES);
-  sph_params* dP = nullptr;
-  int* dC = nullptr;
+  sph_params *dP = nullptr, *dP2 = nullptr;
+  int *dC = nullptr, *dC2 = nullptr;
   rc = rc ? rc : dalloc(s, &dP, (size_t)K);
   rc = rc ? rc : dalloc(s, &dC, (size_t)K);
+  rc = rc ? rc : dalloc(s, &dP2, (size_t)K);
+  rc = rc ? rc : dalloc(s, &dC2, (size_t)K);
   if (rc) {
     release(s);
     return rc;
   }
-  s->T.P = dP;
-  s->T.cell_off = dC;
-  s->T.K = K;
-  s->T.dim = s->dim;
-  s->T.total_cells = (int)ctot;
+  s->T1.P = dP;
+  s->T1.cell_off = dC;
+  s->T1.K = K;
+  s->T1.dim = s->dim;
+  s->T1.total_cells = (int)ctot;
+  s->T2.P = dP2;
+  s->T2.cell_off = dC2;
+  s->T2.K = fine_ok ? K : 0;
+  s->T2.dim = s->dim;
+  s->T2.total_cells = fine_ok ? (int)ctot2 : 0;
+  s->T = s->T1;
   // stage the host arrays in temporary device buffers, widen on device
   float *dpos = nullptr, *dvel = nullptr, *drho = nullptr, *dprs = nullptr;
   int *dtype = nullptr, *dinst = nullptr;
   he = hipMemcpy(dP, s->P.data(), sizeof(sph_params) * K, hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMemcpy(dC, s->cell_off.data(), sizeof(int) * K, hipMemcpyHostToDevice);
+  if (he == hipSuccess && fine_ok) he = hipMemcpy(dP2, P2.data(), sizeof(sph_params) * K, hipMemcpyHostToDevice);
+  if (he == hipSuccess && fine_ok) he = hipMemcpy(dC2, off2.data(), sizeof(int) * K, hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMalloc(&dpos, N * 3 * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&dvel, N * 3 * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&drho, N * sizeof(float));
@@ -314,15 +355,21 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
     return set_error(SPH_E_INVAL, "nbr_block must be 64, 128, 192 or 256");
   if (t->nbr_unroll != 1 && t->nbr_unroll != 2 && t->nbr_unroll != 4 && t->nbr_unroll != 8)
     return set_error(SPH_E_INVAL, "nbr_unroll must be 1, 2, 4 or 8");
-  if (t->nbr_lds < 0 || t->nbr_lds > 3) return set_error(SPH_E_INVAL, "nbr_lds must be 0, 1, 2 or 3");
+  if (t->nbr_lds < 0 || t->nbr_lds > 4) return set_error(SPH_E_INVAL, "nbr_lds must be 0, 1, 2, 3 or 4");
+  const int cd = t->cell_div <= 0 ? 1 : t->cell_div;
+  if (cd != 1 && cd != 2) return set_error(SPH_E_INVAL, "cell_div must be 1 or 2");
+  if (cd == 2 && t->nbr_lds != 4) return set_error(SPH_E_INVAL, "cell_div 2 needs nbr_lds 4 (fine rows)");
+  if (cd == 2 && s->T2.K == 0) return set_error(SPH_E_INVAL, "cell_div 2: the subdivided grid would be too large");
   if (t->nbr_compact < 0 || t->nbr_compact > 2) return set_error(SPH_E_INVAL, "nbr_compact must be 0, 1 or 2");
   const bool shape_changed = t->nbr_block != s->tun.nbr_block || t->nbr_unroll != s->tun.nbr_unroll ||
                              (t->xcd_remap != 0) != (s->tun.xcd_remap != 0) ||
                              t->nbr_lds != s->tun.nbr_lds ||
                              t->nbr_compact != s->tun.nbr_compact ||
                              (t->nbr_prefetch != 0) != (s->tun.nbr_prefetch != 0) ||
-                             (t->nbr_list != 0) != (s->tun.nbr_list != 0);
+                             (t->nbr_list != 0) != (s->tun.nbr_list != 0) || cd != s->tun.cell_div;
   s->tun = *t;
+  s->tun.cell_div = cd;
+  use_grid(s, cd);
   s->tun.xcd_remap = t->xcd_remap ? 1 : 0;
   s->tun.nbr_prefetch = t->nbr_prefetch ? 1 : 0;
   s->tun.nbr_list = t->nbr_list ? 1 : 0;
@@ -552,7 +599,12 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
   FastWork w;
   SPH_HIP(sort_phase(s, 0, 1, false, &w), "stats sort");
   SPH_HIP(hipMemsetAsync(s->probe, 0, 4 * sizeof(unsigned long long), s->stream), "stats clear");
-  SPH_HIP(launch_pair_stats(s->T, s->set[1], w, s->probe, s->n, s->stream), "stats");
+  if (s->tun.nbr_lds == 4) {
+    // (the visit stages dofv with the candidates; the counts do not use it)
+    SPH_HIP(launch_pair_stats_fine(s->T, s->set[1], w, s->probe, s->n, s->stream), "stats");
+  } else {
+    SPH_HIP(launch_pair_stats(s->T, s->set[1], w, s->probe, s->n, s->stream), "stats");
+  }
   unsigned long long hv[4];
   SPH_HIP(hipMemcpyAsync(hv, s->probe, sizeof(hv), hipMemcpyDeviceToHost, s->stream), "stats copy");
   SPH_HIP(hipStreamSynchronize(s->stream), "stats sync");
@@ -566,13 +618,18 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
 // Autotuner: times kernels A and B on the live state for every launch shape
 // (scan kernels: neighbour loads prefetched or not x plain or compact loop x
 // LDS-staged or global with block 64/128/256; or neighbour lists with block
-// 64/128/256; each x unroll 1/2/4/8 x XCD remap on/off) with HIP events on
-// the context's stream and keeps the fastest.  The neighbour phase is
-// re-evaluated from the same sorted set each time, so the state is unchanged
-// and every candidate sees identical work.
+// 64/128/256; each x unroll 1/2/4/8 x XCD remap on/off; fine rows on the 2h
+// grid and on the halved grid x unroll 1/2/4 x prefetch x remap) with HIP
+// events on the context's stream and keeps the fastest.  The neighbour phase
+// is re-evaluated from the same sorted set of each grid every time, so the
+// state is unchanged and every candidate of a grid sees identical work.  The
+// halved grid visits the same pairs in another order, so picking it moves
+// results by rounding only (within the oracle tolerance on that grid).
 extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* best_us) {
   if (!s) return set_error(SPH_E_INVAL, "null sim");
   if (reps <= 0) reps = 3;
+  const sph_tuning saved = s->tun;
+  use_grid(s, 1);  // the 2h-grid variants first
   FastWork w;
   SPH_HIP(sort_phase(s, 0, 1, false, &w), "autotune sort");
   // kernel A rewrites wall p/rho of set 1 in place from fluid values only, so
@@ -580,7 +637,6 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   hipEvent_t a, b;
   SPH_HIP(hipEventCreate(&a), "hipEventCreate");
   SPH_HIP(hipEventCreate(&b), "hipEventCreate");
-  const sph_tuning saved = s->tun;
   sph_tuning pick = saved;
   double pick_us = 1e300;
   const int blocks[] = {-2, -1, 0, 64, 128, 256};
@@ -621,9 +677,44 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
           pick = t;
         }
       }
+  // fine rows on the 2h grid and on the halved grid (re-sorted for it)
+  for (int cd = 1; cd <= 2 && err == hipSuccess; ++cd) {
+    if (cd == 2 && s->T2.K == 0) break;
+    use_grid(s, cd);
+    FastWork wf;
+    err = sort_phase(s, 0, 1, false, &wf);
+    for (int remap = 0; remap < 2 && err == hipSuccess; ++remap)
+      for (int prefetch = 0; prefetch < 2 && err == hipSuccess; ++prefetch)
+        for (int un : {1, 2, 4}) {
+          if (err != hipSuccess) break;
+          sph_tuning t = saved;
+          t.nbr_block = 64;
+          t.nbr_unroll = un;
+          t.xcd_remap = remap;
+          t.nbr_lds = 4;
+          t.nbr_compact = 0;
+          t.nbr_prefetch = prefetch;
+          t.nbr_list = 0;
+          t.cell_div = cd;
+          s->tun = t;
+          err = rates_phase(s, 1, wf, false);  // warm-up
+          if (err == hipSuccess) err = hipEventRecord(a, s->stream);
+          for (int r = 0; r < reps && err == hipSuccess; ++r) err = rates_phase(s, 1, wf, false);
+          if (err == hipSuccess) err = hipEventRecord(b, s->stream);
+          if (err == hipSuccess) err = hipEventSynchronize(b);
+          float ms = 0.0f;
+          if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+          const double us = 1000.0 * ms / reps;
+          if (err == hipSuccess && us < pick_us) {
+            pick_us = us;
+            pick = t;
+          }
+        }
+  }
   hipEventDestroy(a);
   hipEventDestroy(b);
   s->tun = saved;
+  use_grid(s, saved.cell_div);
   SPH_HIP(err, "sph_sim_autotune");
   int rc = sph_sim_set_tuning(s, &pick);
   if (rc) return rc;

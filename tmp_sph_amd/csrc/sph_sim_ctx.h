@@ -16,11 +16,14 @@ struct OutputPipe;
 struct sph_sim {
   std::vector<sph_params> P;  // per instance (host copy)
   std::vector<int> cell_off;  // per instance
-  sph::InstTable T{};
+  sph::InstTable T{};         // grid in use (T1 or T2, per tuning.cell_div)
+  sph::InstTable T1{};        // the instances' own grids
+  sph::InstTable T2{};        // the same grids subdivided by 2 (K = 0 when too large)
   int K = 1;
   int dim = 3;
   int n = 0;
-  int bits = 1;
+  int bits = 1;               // key bits of the grid in use
+  int bits1 = 1, bits2 = 1;
   hipStream_t stream = nullptr;
   sph::FastSet set[2]{};
   sph::FastWork w{};

@@ -25,7 +25,7 @@ from ._lib import DamBreakCfg, SphParams, SphTuning, call, load
 
 __all__ = [
     "DeviceArray", "SphParams", "DamBreakCfg", "SphTuning", "Simulation", "dambreak_cfg", "make_case",
-    "reference_params", "set_params", "get_params", "calcHash", "sortParticles", "reorder",
+    "reference_params", "subdivide_params", "set_params", "get_params", "calcHash", "sortParticles", "reorder",
     "SPH_NS_simpleversion", "PC_prediction", "PC_correction", "device_count", "synchronize",
 ]
 
@@ -132,6 +132,14 @@ def dambreak_cfg(dim: int = 2, dx: float = 0.01, tank=(4.0, 1.0, 1.0), column=(1
     c.h_over_dx, c.rho0, c.g, c.c0, c.cfl = h_over_dx, rho0, g, c0, cfl
     c.delta, c.alpha, c.gamma, c.grid_pad, c.cell_over_kh = delta, alpha, gamma, grid_pad, cell_over_kh
     return c
+
+
+def subdivide_params(p: SphParams, div: int) -> SphParams:
+    """The parameters on the cell grid divided by ``div`` (1, 2 or 4): the grid
+    a context searches with tuning ``cell_div = div``."""
+    out = SphParams()
+    call("sph_params_subdivide", C.byref(p), int(div), C.byref(out))
+    return out
 
 
 def case_counts(cfg: DamBreakCfg) -> tuple[int, int]:
