@@ -290,7 +290,10 @@ def main() -> int:
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned", "surrogate"])
     ap.add_argument("--train-seconds", type=float, default=120.0, help="learned: GPU training budget")
     ap.add_argument("--learned-t", type=float, default=0.25, help="learned: simulated time of the timed rollout")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every launch shape on the live state first and keep the fastest (off by default: "
+                         "the default shape is the one the autotuner picks for this case)")
+    ap.add_argument("--no-autotune", action="store_true", help="(default; kept for older command lines)")
     ap.add_argument("--timing-steps", type=int, default=10, help="eager steps with per-kernel HIP events")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -330,7 +333,6 @@ def main() -> int:
     print(f"# workload {args.config}: {len(cases)} case(s), N={n_total} fluid={n_fluid} "
           f"(generated in {time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     sim = sph.Simulation(cases if len(cases) > 1 else case)
-    tuned = None
     fixed = {k: v for k, v in (("nbr_block", args.nbr_block), ("nbr_unroll", args.nbr_unroll)) if v > 0}
     fixed.update({k: v for k, v in (("xcd_remap", args.xcd_remap), ("nbr_lds", args.nbr_lds),
                                      ("nbr_compact", args.nbr_compact), ("nbr_prefetch", args.nbr_prefetch),
@@ -340,10 +342,11 @@ def main() -> int:
         fixed["cell_div"] = args.cell_div
     if fixed:
         sim.set_tuning(**fixed)
-        tuned = sim.tuning.as_dict()
-    elif not args.no_autotune:
-        tuned, tuned_us = sim.autotune(reps=3)
-        print(f"# autotune: {tuned} ({tuned_us:.1f} us for kernels A+B)", file=sys.stderr, flush=True)
+    elif args.autotune and not args.no_autotune:
+        _, tuned_us = sim.autotune(reps=3)
+        print(f"# autotune: {sim.tuning.as_dict()} ({tuned_us:.1f} us for kernels A+B)", file=sys.stderr,
+              flush=True)
+    tuned = sim.tuning.as_dict()
     sim.capture()
     sim.replay(args.warmup)
     sim.synchronize()

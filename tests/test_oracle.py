@@ -173,6 +173,63 @@ def test_delta_diffusion_of_quadratic_density(orc, dim):
     np.testing.assert_allclose(out["drho"][m], want, rtol=0.05)
 
 
+
+
+
+
+def _viscous_moment(prm, dim):
+    """J = S_d / (d (d+2)) * int_0^2h r^(d+3) (-fr(r)) / (r^2 + eta^2) dr: the
+    fourth moment of the artificial-viscosity weight, from its closed form
+    (-fr = 5 adh (1 - q/2)^3 / h^2, adh = 21/(16 pi h^3) in 3-D and 7/(4 pi h^2)
+    in 2-D); with eta = 0 it is exactly 1/(d+2)."""
+    h = float(prm.h)
+    eta = float(prm.eta)
+    adh = 21.0 / (16.0 * np.pi * h ** 3) if dim == 3 else 7.0 / (4.0 * np.pi * h ** 2)
+    r = np.linspace(0.0, 2.0 * h, 200_001)
+    mfr = 5.0 * adh * (1.0 - r / (2.0 * h)) ** 3 / h ** 2
+    f = r ** (dim + 3) * mfr / (r ** 2 + eta ** 2)
+    integral = float(np.sum((f[1:] + f[:-1]) * np.diff(r)) / 2.0)
+    surface = 4.0 * np.pi if dim == 3 else 2.0 * np.pi
+    return surface * integral / (dim * (dim + 2))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("field,alpha,damper", [("shear", 0.5, 0.0), ("stretch", 0.5, 0.0), ("stretch", 0.0, 1.0)])
+def test_viscosity_and_acoustic_damper_continuum_limits(orc, dim, field, alpha, damper):
+    """Kernel B's fluid-fluid momentum terms (opt_src_v1/equation/ns.cu:210-213)
+    at uniform rho0 and p = 0, against their continuum limits on a lattice
+    with h = 3 dx, where the lattice's fourth moments are isotropic to a
+    fraction of a percent (second-order Taylor expansion of the velocity):
+      artificial viscosity  h c0 rho0 alpha / rho * J/2 * (2 grad(div v) + lap v)
+      acoustic damper       h c0 rho0 damper / rho * grad(dofv),  dofv -> div v
+    shear   v = (k z^2, 0, 0): div v = 0, lap v = (2k, 0, 0)    -> a_x = h c0 alpha J k
+    stretch v = (k x^2, 0, 0): grad div v = lap v = (2k, 0, 0) -> a_x = h c0 (3 alpha J k + 2 damper k)
+    A prefactor missing rho0, h or c0, a scaled alpha, a dropped factor 2 of
+    grad(div v), or a flipped damper sign each miss by far more than 2 %."""
+    k = 3.0
+    if field == "shear":
+        vf = lambda x: np.stack([k * (x[:, 2] - 0.5) ** 2, 0 * x[:, 0], 0 * x[:, 0]], 1)  # noqa: E731
+    else:
+        vf = lambda x: np.stack([k * (x[:, 0] - 1.2) ** 2, 0 * x[:, 0], 0 * x[:, 0]], 1)  # noqa: E731
+    c = _lattice_case(dim, dx=0.01, side=(0.4, 0.4, 0.4), v_fn=vf, c0=20.0, h_over_dx=3.0)
+    prm = c.params
+    prm.afa = alpha
+    prm.damper = damper
+    out = orc.eval_rates(prm, c, mode=2)
+    # the damper pairs dofv of neighbours, which need full support themselves
+    m = _interior(c, 4.0 * float(prm.h) + 0.01)
+    J = _viscous_moment(prm, dim)
+    hc = float(prm.h) * float(prm.cs)  # rho0 / rho = 1
+    want = hc * (alpha * J * k * (1.0 if field == "shear" else 3.0) + (2.0 * damper * k if field == "stretch" else 0))
+    ax = out["acc"][m, 0]
+    print(dim, field, alpha, damper, "J", J, "want", want, "got", ax.min(), ax.max())
+    np.testing.assert_allclose(ax, want, rtol=0.02)
+    # no cross-stream acceleration; gravity alone along z
+    assert np.abs(out["acc"][m, 2] - float(prm.gravity)).max() < 0.02 * abs(want)
+    if dim == 3:
+        assert np.abs(out["acc"][m, 1]).max() < 0.02 * abs(want)
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_hydrostatic_pressure_balances_gravity(orc, dim):
     """-grad p / rho + g = 0 in the interior of a hydrostatic block."""

@@ -34,8 +34,9 @@ inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) /
 
 
 __global__ void hash_fast_kernel(InstTable T, const float4* __restrict__ PR, const int* __restrict__ TY,
-                                 int* __restrict__ keys, int* __restrict__ idx, int n) {
+                                 int* __restrict__ keys, int* __restrict__ idx, int n, int* __restrict__ zero) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (zero && i == 0) *zero = 0;
   if (i >= n) return;
   const float4 p = PR[i];
   with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
@@ -319,9 +320,11 @@ hipError_t launch_dim(int dim, K2 k2, K3 k3, dim3 g, dim3 b, hipStream_t s, A...
 
 }  // namespace
 
-hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(hash_fast_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, in.PR, in.TY, keys, idx, n);
+hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s,
+                            int* zero) {
+  if (n <= 0 && !zero) return hipSuccess;
+  hipLaunchKernelGGL(hash_fast_kernel, dim3(blocks_for(n > 0 ? n : 1, 256)), dim3(256), 0, s, T, in.PR, in.TY, keys,
+                     idx, n, zero);
   return hipGetLastError();
 }
 

@@ -1,7 +1,8 @@
 // Fine-row neighbour loops (tuning nbr_lds 4): kernels A and B
 // (opt_src_v1/equation/ns.cu:30-134, :137-230) on a cell grid of any stencil
 // reach K (K cells on each side of the target's cell; K = 1 on the
-// reference's 2h grid, K = 2 on a grid of cell edge h).
+// reference's 2h grid, K = 2 on a grid of cell edge h), and the dense cell
+// table they read.
 //
 // On a fine grid most of a (2K+1)^3 cell block lies outside the support, so
 // each target visits, per stencil row (dz, dy), only the cells of that row its
@@ -12,23 +13,29 @@
 // target, which on a grid of edge h leaves about 185 candidates per target in
 // 3-D instead of about 350 on the 2h grid.
 //
-// One wave (one-wave block) per 64 consecutive sorted targets.  A wave serves
-// its targets group by group: the lanes of one instance, one cell row (z, y)
-// and a window of at most 63 - 2K cells in x (waterfall; one group when the
-// 64 targets share a row, the common case).  Per stencil row the group
-//   1. reads the cell table of the window's cells, one lane per cell, and
-//      turns it into dense offsets with a wave suffix minimum: off[t] is the
-//      first sorted index at or after window cell t (empty cells take the
-//      next non-empty cell's start, the tail the last cell's end);
-//   2. gives each lane [off[x0], off[x1 + 1]) for its trimmed cells x0..x1
-//      (cross-lane reads, no table loads per lane);
-//   3. copies the union of the lanes' ranges, with the neighbour data the
-//      pair bodies read, into LDS in chunks of FINE_CAP entries, and runs the
-//      pair loop there.
-// Rows are visited z outer, y inner, and a lane's candidates in ascending
-// sorted index, which is the oracle's order on the same grid; culled cells
-// hold no pair inside the support, so the pair set is the oracle's too.
+// Cell table: first[c] = the first sorted index whose key is >= c, for every
+// cell c of the context and one sentinel (= n).  The x-adjacent cells of a
+// row are adjacent keys, so the candidates of cells x0..x1 of a row are the
+// sorted range [first[row + x0], first[row + x1 + 1]) -- two loads per lane
+// and row, whatever the cells hold (the reference's fill_cell builds a dense
+// form of its table too, opt_src_v4/lib/mesh.cu:204-253).  Kernel A pairs with
+// fluid only: a bit per cell marks cells that hold fluid, and a row whose
+// trimmed cells hold none is skipped.
+//
+// One wave (one-wave block) per 64 consecutive sorted targets, served group
+// by group: the lanes of one instance and one cell row (z, y) (waterfall; one
+// group when the 64 targets share a row, the common case).  Per stencil row
+// the group copies the union of its lanes' ranges, with the neighbour data
+// the pair bodies read, into LDS in chunks of FINE_CAP entries and runs the
+// pair loop there; the next row's range loads are issued before the current
+// row's copy, so a row costs one memory round trip.  Rows are visited z
+// outer, y inner, and a lane's candidates in ascending sorted index, which is
+// the oracle's order on the same grid; culled cells hold no pair inside the
+// support, so the pair set is the oracle's too.
 #include "sph_pair.h"
+
+extern "C" __device__ int __ockl_wfred_min_i32(int);
+extern "C" __device__ int __ockl_wfred_max_i32(int);
 
 namespace sph {
 
@@ -38,24 +45,122 @@ namespace {
 
 inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-constexpr int FINE_CAP = 128;  // staged candidates per chunk
+constexpr int FINE_CAP = 128;  // staged candidates per chunk (2 per lane)
 constexpr int FINE_PAD = 8;    // entries read past a chunk's end by the unrolled candidate loads
+constexpr int GAP_SMALL = 32;  // longer runs of cells are filled block-wide
+
+static_assert(FINE_CAP == 128, "the copy moves two entries per lane");
+
+__device__ __forceinline__ int wmin(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wmax(int v) { return __ockl_wfred_max_i32(v); }
+
+// The wave's LDS accesses are processed in order; this keeps the compiler
+// from moving them across the copy / pair-loop boundary of a one-wave block.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------ dense table
+// Sorted index i (0..n) owns the cells (keys[i-1], keys[i]] (keys[-1] = -1,
+// keys[n] = cells): first[c] = i for them.  The runs partition 0..cells, so
+// every entry is written once; the word of fluid bits of cell c is cleared
+// by the writer of c = 32 w.  Runs longer than GAP_SMALL go to a queue that
+// fill_runs_kernel serves block-wide (empty rows and planes of the padding).
+// The queue counter is zeroed by the hash kernel of the same half-step.
+__global__ void dense_first_kernel(const int* __restrict__ keys, int n, int cells, int* __restrict__ first,
+                                   unsigned* __restrict__ fbits, int* __restrict__ q, int* __restrict__ qn, int qcap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int lo = i == 0 ? 0 : keys[i - 1] + 1;
+  const int hi = i == n ? cells : keys[i];
+  if (hi < lo) return;
+  int slot = qcap;
+  if (hi - lo >= GAP_SMALL) slot = atomicAdd(qn, 1);
+  if (slot < qcap) {
+    q[3 * slot] = lo;
+    q[3 * slot + 1] = hi;
+    q[3 * slot + 2] = i;
+    return;
+  }
+  // short run (or a full queue, which the queue's size rules out)
+  for (int c = lo; c <= hi; ++c) {
+    first[c] = i;
+    if ((c & 31) == 0) fbits[c >> 5] = 0u;
+  }
+}
+
+__global__ void fill_runs_kernel(int* __restrict__ first, unsigned* __restrict__ fbits, const int* __restrict__ q,
+                                 const int* __restrict__ qn, int qcap) {
+  const int cnt = min(*qn, qcap);
+  for (int g = blockIdx.x; g < cnt; g += gridDim.x) {
+    const int lo = q[3 * g], hi = q[3 * g + 1], v = q[3 * g + 2];
+    for (int c = lo + (int)threadIdx.x; c <= hi; c += blockDim.x) {
+      first[c] = v;
+      if ((c & 31) == 0) fbits[c >> 5] = 0u;
+    }
+  }
+}
+
+// Gather into sorted order (as cells_gather_kernel) and set the fluid bit of
+// every cell holding a fluid particle: the keys are sorted, so lanes sharing
+// a word of bits are adjacent and OR their bits together first.
+template <int DIM>
+__global__ void gather_fluidbits_kernel(FastSet in, FastSet out, const int* __restrict__ keys,
+                                        const int* __restrict__ idx, unsigned* __restrict__ fbits, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  const int ic = valid ? i : n - 1;
+  const int h = keys[ic];
+  const int src = idx[ic];
+  const float4 pr = in.PR[src];
+  if (valid) {
+    out.PR[i] = pr;
+    out.VP[i] = in.VP[src];
+    out.X[0][i] = in.X[0][src];
+    if (DIM == 3) out.X[1][i] = in.X[1][src];
+    out.X[2][i] = in.X[2][src];
+    out.ID[i] = in.ID[src];
+    out.TY[i] = in.TY[src];
+  }
+  const int lane = threadIdx.x & 63;
+  const int word = h >> 5;
+  unsigned bits = (valid && pr.w > 0.0f) ? (1u << (h & 31)) : 0u;
+  // segmented OR over runs of equal words (contiguous: the keys are sorted)
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned b = __shfl_down(bits, o);
+    const int wd = __shfl_down(word, o);
+    if (lane + o < 64 && wd == word) bits |= b;
+  }
+  const int wprev = __shfl_up(word, 1);
+  const bool head = lane == 0 || wprev != word;
+  if (valid && head && bits) atomicOr(&fbits[word], bits);
+}
+
+// ------------------------------------------------------------ visit
+// This lane's candidate range in one stencil row, loads issued and not yet
+// waited for (kernel A: with the fluid bits of the trimmed cells).
+struct Pending {
+  int s, e;
+  unsigned w0, w1, m0, m1;
+};
 
 // Candidate visit of one group of lanes (see above).  Called by all 64 lanes
 // of the wave; lanes outside the group (on = false) get empty ranges and only
-// help with the table reads and the copies.
+// help with the copies.
 template <int DIM, int U, bool PF, bool KA>
 struct FineVisit {
   // wave-uniform
-  int X, Y, Z, XY, coff, K, ucy, ucz, cA, cB;
+  int X, Y, Z, XY, coff, K, ucy, ucz, n;
   float gs, inv_gs, gxmin, cut, mx;
   // per lane
   bool on, inside;
   int cx;
   float px, yl, yr, zl, zr;  // target x; its distances to its cell's y/z faces, shrunk (0 outside the grid)
-  const int* __restrict__ cs;
-  const int* __restrict__ ce;
-  const int* __restrict__ fc;  // kernel A: cells holding fluid
+  const int* __restrict__ first;
+  const unsigned* __restrict__ fbits;
   const float4* PR;
   const float4* VP;
   const float* DV;
@@ -63,158 +168,148 @@ struct FineVisit {
   float4* tv;
   float* td;
 
+  // Issues the loads of this lane's range in stencil row r (dz outer, dy inner).
+  __device__ __forceinline__ Pending issue(int r) const {
+    Pending p{0, 0, 0u, 0u, 0u, 0u};
+    const int side = 2 * K + 1;
+    const int dz = (DIM == 3 ? r / side : r) - K;
+    const int dy = DIM == 3 ? r - (dz + K) * side - K : 0;
+    const int z = ucz + dz, y = ucy + dy;
+    if (z < 0 || z >= Z || y < 0 || y >= Y) return p;  // wave-uniform
+    const float fz = dz < 0 ? zl + (float)(-dz - 1) * gs : (dz > 0 ? zr + (float)(dz - 1) * gs : 0.0f);
+    const float fy = dy < 0 ? yl + (float)(-dy - 1) * gs : (dy > 0 ? yr + (float)(dy - 1) * gs : 0.0f);
+    const float dyz = inside ? fz * fz + fy * fy : 0.0f;
+    int x0 = cx - K, x1 = cx + K;
+    if (inside) {
+      const float w = __builtin_sqrtf(fmaxf(cut - dyz, 0.0f));
+      x0 = max(x0, (int)floorf((px - w - mx - gxmin) * inv_gs));
+      x1 = min(x1, (int)floorf((px + w + mx - gxmin) * inv_gs));
+    }
+    x0 = max(x0, 0);
+    x1 = min(x1, X - 1);
+    if (!on || dyz > cut || x0 > x1) return p;
+    const int rb = coff + z * XY + y * X;
+    const int c0 = rb + x0, c1 = rb + x1;
+    // table entries lie in 0..n; the clamp keeps every staged load inside the
+    // particle arrays even for a table that was not built for this sort
+    p.s = min(max(first[c0], 0), n);
+    p.e = min(max(first[c1 + 1], 0), n);
+    if (KA) {  // x1 - x0 <= 2K < 32: the cells span at most two words
+      p.w0 = fbits[c0 >> 5];
+      p.w1 = fbits[c1 >> 5];
+      const int b0 = c0 & 31, b1 = c1 & 31;
+      const unsigned lo = ~((1u << b0) - 1u), hi = b1 == 31 ? ~0u : ((2u << b1) - 1u);
+      if ((c0 >> 5) == (c1 >> 5)) {
+        p.m0 = lo & hi;
+      } else {
+        p.m0 = lo;
+        p.m1 = hi;
+      }
+    }
+    return p;
+  }
+
   template <class Accept, class Extra, class Pair>
   __device__ __forceinline__ void operator()(Accept&& accept, Extra&& extra, Pair&& pair) {
     const int lane = threadIdx.x & 63;
-    const int L = cB - cA + 1;  // window cells (L + 1 <= 64)
+    const int R = DIM == 3 ? (2 * K + 1) * (2 * K + 1) : 2 * K + 1;
+    Pending nx = issue(0);
 #pragma unroll 1
-    for (int dz = -K; dz <= K; ++dz) {
-      const int z = ucz + dz;
-      if (z < 0 || z >= Z) continue;
-      const float fz = dz < 0 ? zl + (float)(-dz - 1) * gs : (dz > 0 ? zr + (float)(dz - 1) * gs : 0.0f);
+    for (int r = 0; r < R; ++r) {
+      int s = nx.s, e = nx.e;
+      if (KA && ((nx.w0 & nx.m0) | (nx.w1 & nx.m1)) == 0u) s = e = 0;  // no fluid in the trimmed cells
+      if (r + 1 < R) nx = issue(r + 1);  // in flight during this row
+      const int us = wmin(s < e ? s : 0x7fffffff);
+      const int ue = wmax(s < e ? e : -1);
 #pragma unroll 1
-      for (int dy = (DIM == 3 ? -K : 0); dy <= (DIM == 3 ? K : 0); ++dy) {
-        const int y = ucy + dy;
-        if (y < 0 || y >= Y) continue;
-        const float fy = dy < 0 ? yl + (float)(-dy - 1) * gs : (dy > 0 ? yr + (float)(dy - 1) * gs : 0.0f);
-        const float dyz = inside ? fz * fz + fy * fy : 0.0f;
-        const bool lon = on && dyz <= cut;
-        if (__ballot(lon) == 0ull) continue;
-        // 1. cell table of the window, dense offsets
-        const int rb = coff + z * XY + y * X + cA;
-        int st = -1, en = -1;
-        if (lane < L) {
-          st = cs[rb + lane];
-          if (KA && st != -1 && fc[rb + lane] == 0) st = -1;  // kernel A pairs with fluid only
-          if (st != -1) en = ce[rb + lane];
-        }
-        const int E = wave_max(en);  // end of the window's last (fluid) cell
-        if (E < 0) continue;
-        int v = st != -1 ? st : 0x7fffffff;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-          const int w = __shfl_down(v, o);
-          if (lane + o < 64) v = min(v, w);
-        }
-        const int off = min(v, E);
-        // 2. this lane's cells and range
-        int x0 = cx - K, x1 = cx + K;
-        if (inside) {
-          const float w = __builtin_sqrtf(fmaxf(cut - dyz, 0.0f));
-          x0 = max(x0, (int)floorf((px - w - mx - gxmin) * inv_gs));
-          x1 = min(x1, (int)floorf((px + w + mx - gxmin) * inv_gs));
-        }
-        x0 = max(x0, cA);
-        x1 = min(x1, cB);
-        const int a = __shfl(off, min(max(x0 - cA, 0), 63));
-        const int b = __shfl(off, min(max(x1 + 1 - cA, 0), 63));
-        int s = 0, e = 0;
-        if (lon && x0 <= x1) {
-          s = a;
-          e = b;
-        }
-        // 3. stage the union chunk by chunk, run the pair loop from LDS
-        const int us = wave_min(s < e ? s : 0x7fffffff);
-        const int ue = wave_max(s < e ? e : -1);
-#pragma unroll 1
-        for (int c0 = us; c0 < ue; c0 += FINE_CAP) {
-          const int len = min(FINE_CAP, ue - c0);
-          __syncthreads();  // the previous chunk is consumed
-          {
-            // FINE_CAP = 2 x 64: every load is issued before the first store
-            const int t0 = lane, t1 = lane + 64;
-            const bool h0 = t0 < len, h1 = t1 < len;
-            float4 a0, a1, b0, b1;
-            float d0 = 0.0f, d1 = 0.0f;
-            if (h0) {
-              a0 = PR[c0 + t0];
-              b0 = VP[c0 + t0];
-              if (!KA) d0 = DV[c0 + t0];
-            }
-            if (h1) {
-              a1 = PR[c0 + t1];
-              b1 = VP[c0 + t1];
-              if (!KA) d1 = DV[c0 + t1];
-            }
-            if (h0) {
-              tp[t0] = a0;
-              tv[t0] = b0;
-              if (!KA) td[t0] = d0;
-            }
-            if (h1) {
-              tp[t1] = a1;
-              tv[t1] = b1;
-              if (!KA) td[t1] = d1;
-            }
+      for (int c0 = us; c0 < ue; c0 += FINE_CAP) {
+        const int len = min(FINE_CAP, ue - c0);
+        {
+          // every load is issued before the first store (entries past the
+          // chunk re-read its first one and are not stored)
+          const int t0 = lane, t1 = lane + 64;
+          const bool h0 = t0 < len, h1 = t1 < len;
+          const unsigned o0 = (unsigned)(c0 + (h0 ? t0 : 0)), o1 = (unsigned)(c0 + (h1 ? t1 : 0));
+          const float4 a0 = ld16(PR, o0 * 16u), a1 = ld16(PR, o1 * 16u);
+          const float4 b0 = ld16(VP, o0 * 16u), b1 = ld16(VP, o1 * 16u);
+          float d0 = 0.0f, d1 = 0.0f;
+          if (!KA) {
+            d0 = ld4(DV, o0 * 4u);
+            d1 = ld4(DV, o1 * 4u);
           }
-          __syncthreads();
-          const int js = max(s, c0), je = min(e, c0 + len);
-          for (int j0 = js; j0 < je; j0 += U) {
-            const int t = j0 - c0;
-            float4 pjs[U];
-            decltype(extra(0)) es[U];
+          if (h0) {
+            tp[t0] = a0;
+            tv[t0] = b0;
+            if (!KA) td[t0] = d0;
+          }
+          if (h1) {
+            tp[t1] = a1;
+            tv[t1] = b1;
+            if (!KA) td[t1] = d1;
+          }
+        }
+        wave_lds_sync();
+        const int js = max(s, c0), je = min(e, c0 + len);
+        for (int j0 = js; j0 < je; j0 += U) {
+          const int t = j0 - c0;
+          float4 pjs[U];
+          decltype(extra(0)) es[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) pjs[u] = tp[t + u];
-            if constexpr (PF) {
+          for (int u = 0; u < U; ++u) pjs[u] = tp[t + u];
+          if constexpr (PF) {
 #pragma unroll
-              for (int u = 0; u < U; ++u) es[u] = extra.from(tv, td, t + u);
-            }
-            bool acc[U];
+            for (int u = 0; u < U; ++u) es[u] = extra.from(tv, td, t + u);
+          }
+          bool acc[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc[u] = j0 + u < je && accept(j0 + u, pjs[u]);
-            if constexpr (!PF) {
-#pragma unroll
-              for (int u = 0; u < U; ++u)
-                if (acc[u]) es[u] = extra.from(tv, td, t + u);
-            }
+          for (int u = 0; u < U; ++u) acc[u] = j0 + u < je && accept(j0 + u, pjs[u]);
+          if constexpr (!PF) {
 #pragma unroll
             for (int u = 0; u < U; ++u)
-              if (acc[u]) pair(j0 + u, pjs[u], es[u]);
+              if (acc[u]) es[u] = extra.from(tv, td, t + u);
           }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (acc[u]) pair(j0 + u, pjs[u], es[u]);
         }
+        wave_lds_sync();  // every lane is done with the chunk before the next copy
       }
     }
   }
 };
 
-// Splits the wave's targets into groups (one instance, one cell row, an x
-// window of at most 63 - 2K cells) and calls body(P, visit, in_group) for
-// each with every lane of the wave.  want: the lane has a target to serve.
+// Splits the wave's targets into groups (one instance, one cell row) and
+// calls body(P, visit, in_group) for each with every lane of the wave.
+// want: the lane has a target to serve.
 template <int DIM, int U, bool PF, bool KA, class Body>
-__device__ __forceinline__ void fine_groups(const InstTable& T, int inst, int key, const float4& p, bool want,
+__device__ __forceinline__ void fine_groups(const InstTable& T, int n, int inst, int key, const float4& p, bool want,
                                             const float4* PR, const float4* VP, const float* DV,
-                                            const int* __restrict__ cs, const int* __restrict__ ce,
-                                            const int* __restrict__ fc, float4* tp, float4* tv, float* td,
-                                            Body&& body) {
+                                            const int* __restrict__ first, const unsigned* __restrict__ fbits,
+                                            float4* tp, float4* tv, float* td, Body&& body) {
   bool todo = want;
   for (;;) {
     const unsigned long long pend = __ballot(todo);
     if (pend == 0ull) break;
-    const int first = __builtin_ctzll(pend);
-    const int u = __builtin_amdgcn_readlane(inst, first);
+    const int lead = __builtin_ctzll(pend);
+    const int u = __builtin_amdgcn_readlane(inst, lead);
     const sph_params& P = T.P[u];
     const int coff = T.cell_off[u];
-    const int X = P.gridxdim, Y = P.gridydim;
+    const int X = P.gridxdim, Y = P.gridydim, Z = P.gridzdim;
     const int hh = key - coff;  // meaningful for the lanes of instance u
     const int row = hh / X;     // z * Y + y
     const int cx = hh - row * X;
-    const int urow = __builtin_amdgcn_readlane(row, first);
-    const int ucx = __builtin_amdgcn_readlane(cx, first);
-    const int K = stencil_reach(P.kh, P.gridSize);
-    // lanes of one row come in ascending cx, so ucx is the group's smallest
-    const bool grp = todo && inst == u && row == urow && cx - ucx <= 62 - 2 * K;
-    const int cxhi = wave_max(grp ? cx : -1);
+    const int urow = __builtin_amdgcn_readlane(row, lead);
+    const bool grp = todo && inst == u && row == urow;
     FineVisit<DIM, U, PF, KA> vis;
     vis.X = X;
     vis.Y = Y;
-    vis.Z = P.gridzdim;
+    vis.Z = Z;
     vis.XY = X * Y;
     vis.coff = coff;
-    vis.K = K;
+    vis.n = n;
+    vis.K = stencil_reach(P.kh, P.gridSize);
     vis.ucz = urow / Y;
     vis.ucy = urow - vis.ucz * Y;
-    vis.cA = max(ucx - K, 0);
-    vis.cB = min(cxhi + K, X - 1);
     const float gs = P.gridSize;
     vis.gs = gs;
     vis.inv_gs = 1.0f / gs;
@@ -227,7 +322,6 @@ __device__ __forceinline__ void fine_groups(const InstTable& T, int inst, int ke
     vis.cx = cx;
     vis.px = p.x;
     const float m = gs * (1.0f / 1024.0f);
-    const int Z = P.gridzdim;
     const bool inside = p.x >= P.gridxmin && p.x <= P.gridxmin + (float)X * gs && p.z >= P.gridzmin &&
                         p.z <= P.gridzmin + (float)Z * gs &&
                         (DIM == 2 || (p.y >= P.gridymin && p.y <= P.gridymin + (float)Y * gs));
@@ -237,9 +331,8 @@ __device__ __forceinline__ void fine_groups(const InstTable& T, int inst, int ke
     vis.yr = inside && DIM == 3 ? fmaxf(ly + gs - p.y - m, 0.0f) : 0.0f;
     vis.zl = inside ? fmaxf(p.z - lz - m, 0.0f) : 0.0f;
     vis.zr = inside ? fmaxf(lz + gs - p.z - m, 0.0f) : 0.0f;
-    vis.cs = cs;
-    vis.ce = ce;
-    vis.fc = fc;
+    vis.first = first;
+    vis.fbits = fbits;
     vis.PR = PR;
     vis.VP = VP;
     vis.DV = DV;
@@ -251,11 +344,12 @@ __device__ __forceinline__ void fine_groups(const InstTable& T, int inst, int ke
   }
 }
 
+// ------------------------------------------------------------ kernels
 template <int DIM, int U, bool PF>
 __global__ __launch_bounds__(64) void nsA_fine_kernel(InstTable T, float4* PR, float4* VP, const int* __restrict__ TY,
                                                       float* __restrict__ DV, const int* __restrict__ keys,
-                                                      const int* __restrict__ cs, const int* __restrict__ ce,
-                                                      const int* __restrict__ fc, int n, int remap) {
+                                                      const int* __restrict__ first,
+                                                      const unsigned* __restrict__ fbits, int n, int remap) {
   __shared__ float4 tp[FINE_CAP + FINE_PAD];
   __shared__ float4 tv[FINE_CAP + FINE_PAD];
   __shared__ float td[1];  // kernel A reads no neighbour's dofv
@@ -265,7 +359,7 @@ __global__ __launch_bounds__(64) void nsA_fine_kernel(InstTable T, float4* PR, f
   const float4 pi = PR[ic];
   const float4 vi = VP[ic];
   const int key = keys[ic];
-  fine_groups<DIM, U, PF, true>(T, inst_of(TY[ic]), key, pi, valid, PR, VP, nullptr, cs, ce, fc, tp, tv, td,
+  fine_groups<DIM, U, PF, true>(T, n, inst_of(TY[ic]), key, pi, valid, PR, VP, nullptr, first, fbits, tp, tv, td,
                                 [&](const sph_params& P, auto visit, bool grp) {
                                   nsA_target<DIM>(P, i, pi, vi, visit, PR, VP, DV, grp);
                                 });
@@ -275,8 +369,8 @@ template <int DIM, int U, bool PF>
 __global__ __launch_bounds__(64) void nsB_fine_kernel(InstTable T, const float4* __restrict__ PR,
                                                       const float4* __restrict__ VP, const int* __restrict__ TY,
                                                       const float* __restrict__ DV, const int* __restrict__ keys,
-                                                      const int* __restrict__ cs, const int* __restrict__ ce,
-                                                      float4* __restrict__ RATE, int n, int remap) {
+                                                      const int* __restrict__ first, float4* __restrict__ RATE,
+                                                      int n, int remap) {
   __shared__ float4 tp[FINE_CAP + FINE_PAD];
   __shared__ float4 tv[FINE_CAP + FINE_PAD];
   __shared__ float td[FINE_CAP + FINE_PAD];
@@ -289,7 +383,7 @@ __global__ __launch_bounds__(64) void nsB_fine_kernel(InstTable T, const float4*
   const float4 vi = VP[ic];
   const float dvi = DV[ic];
   const int key = keys[ic];
-  fine_groups<DIM, U, PF, false>(T, inst_of(TY[ic]), key, pi, valid && fluid, PR, VP, DV, cs, ce, nullptr, tp, tv,
+  fine_groups<DIM, U, PF, false>(T, n, inst_of(TY[ic]), key, pi, valid && fluid, PR, VP, DV, first, nullptr, tp, tv,
                                  td, [&](const sph_params& P, auto visit, bool grp) {
                                    nsB_target<DIM>(P, i, pi, vi, dvi, visit, VP, DV, RATE, grp);
                                  });
@@ -304,8 +398,8 @@ __global__ __launch_bounds__(64) void pair_stats_fine_kernel(InstTable T, const 
                                                              const float4* __restrict__ VP,
                                                              const float* __restrict__ DV,
                                                              const int* __restrict__ TY, const int* __restrict__ keys,
-                                                             const int* __restrict__ cs, const int* __restrict__ ce,
-                                                             const int* __restrict__ fc,
+                                                             const int* __restrict__ first,
+                                                             const unsigned* __restrict__ fbits,
                                                              unsigned long long* __restrict__ out, int n) {
   __shared__ float4 tp[FINE_CAP + FINE_PAD];
   __shared__ float4 tv[FINE_CAP + FINE_PAD];
@@ -318,7 +412,7 @@ __global__ __launch_bounds__(64) void pair_stats_fine_kernel(InstTable T, const 
   const int key = keys[ic];
   const int inst = inst_of(TY[ic]);
   unsigned long long cA = 0, hA = 0, c = 0, hB = 0;
-  fine_groups<DIM, 1, false, true>(T, inst, key, pi, valid, PR, VP, nullptr, cs, ce, fc, tp, tv, td,
+  fine_groups<DIM, 1, false, true>(T, n, inst, key, pi, valid, PR, VP, nullptr, first, fbits, tp, tv, td,
                                    [&](const sph_params&, auto visit, bool) {
                                      visit(
                                          [&](int j, const float4&) {
@@ -327,7 +421,7 @@ __global__ __launch_bounds__(64) void pair_stats_fine_kernel(InstTable T, const 
                                          },
                                          ExtraNone{}, [&](int, const float4&, const NoExtra&) {});
                                    });
-  fine_groups<DIM, 1, false, false>(T, inst, key, pi, valid, PR, VP, DV, cs, ce, nullptr, tp, tv, td,
+  fine_groups<DIM, 1, false, false>(T, n, inst, key, pi, valid, PR, VP, DV, first, nullptr, tp, tv, td,
                                     [&](const sph_params& P, auto visit, bool) {
                                       const float cut = P.rr_cut;
                                       visit(
@@ -356,12 +450,10 @@ hipError_t nsA_fine_v(const InstTable& T, const FastSet& st, const FastWork& w, 
   const dim3 g(blocks_for(n, 64)), b(64);
   if (T.dim == 3)
     hipLaunchKernelGGL((nsA_fine_kernel<3, U, PF>), g, b, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
-                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.cellFluid,
-                       n, remap);
+                       (const int*)w.keys, (const int*)w.cellFirst, (const unsigned*)w.fluidBits, n, remap);
   else
     hipLaunchKernelGGL((nsA_fine_kernel<2, U, PF>), g, b, 0, s, T, st.PR, st.VP, (const int*)st.TY, w.DV,
-                       (const int*)w.keys, (const int*)w.cellStart, (const int*)w.cellEnd, (const int*)w.cellFluid,
-                       n, remap);
+                       (const int*)w.keys, (const int*)w.cellFirst, (const unsigned*)w.fluidBits, n, remap);
   return hipGetLastError();
 }
 
@@ -370,16 +462,38 @@ hipError_t nsB_fine_v(const InstTable& T, const FastSet& st, const FastWork& w, 
   const dim3 g(blocks_for(n, 64)), b(64);
   if (T.dim == 3)
     hipLaunchKernelGGL((nsB_fine_kernel<3, U, PF>), g, b, 0, s, T, (const float4*)st.PR, (const float4*)st.VP,
-                       (const int*)st.TY, (const float*)w.DV, (const int*)w.keys, (const int*)w.cellStart,
-                       (const int*)w.cellEnd, w.RATE, n, remap);
+                       (const int*)st.TY, (const float*)w.DV, (const int*)w.keys, (const int*)w.cellFirst, w.RATE,
+                       n, remap);
   else
     hipLaunchKernelGGL((nsB_fine_kernel<2, U, PF>), g, b, 0, s, T, (const float4*)st.PR, (const float4*)st.VP,
-                       (const int*)st.TY, (const float*)w.DV, (const int*)w.keys, (const int*)w.cellStart,
-                       (const int*)w.cellEnd, w.RATE, n, remap);
+                       (const int*)st.TY, (const float*)w.DV, (const int*)w.keys, (const int*)w.cellFirst, w.RATE,
+                       n, remap);
   return hipGetLastError();
 }
 
 }  // namespace
+
+int dense_runs_cap(long long cells) { return (int)((cells + 1) / (GAP_SMALL + 1) + 2); }
+
+hipError_t launch_cells_dense(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
+                              const int* idx, const FastWork& w, int n, hipStream_t s) {
+  const int cells = T.total_cells;
+  hipLaunchKernelGGL(dense_first_kernel, dim3(blocks_for((long long)n + 1, 256)), dim3(256), 0, s, keys, n, cells,
+                     w.cellFirst, w.fluidBits, w.runs, w.runCount, w.runs_cap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fill_runs_kernel, dim3(1024), dim3(256), 0, s, w.cellFirst, w.fluidBits, (const int*)w.runs,
+                     (const int*)w.runCount, w.runs_cap);
+  e = hipGetLastError();
+  if (e != hipSuccess || n <= 0) return e;
+  if (T.dim == 3)
+    hipLaunchKernelGGL(gather_fluidbits_kernel<3>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, out, keys, idx,
+                       w.fluidBits, n);
+  else
+    hipLaunchKernelGGL(gather_fluidbits_kernel<2>, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, out, keys, idx,
+                       w.fluidBits, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_nsA_fine(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s) {
@@ -411,12 +525,12 @@ hipError_t launch_pair_stats_fine(const InstTable& T, const FastSet& st, const F
   const dim3 g(blocks_for(n, 64)), b(64);
   if (T.dim == 3)
     hipLaunchKernelGGL(pair_stats_fine_kernel<3>, g, b, 0, s, T, (const float4*)st.PR, (const float4*)st.VP,
-                       (const float*)w.DV, (const int*)st.TY, (const int*)w.keys, (const int*)w.cellStart,
-                       (const int*)w.cellEnd, (const int*)w.cellFluid, out4, n);
+                       (const float*)w.DV, (const int*)st.TY, (const int*)w.keys, (const int*)w.cellFirst,
+                       (const unsigned*)w.fluidBits, out4, n);
   else
     hipLaunchKernelGGL(pair_stats_fine_kernel<2>, g, b, 0, s, T, (const float4*)st.PR, (const float4*)st.VP,
-                       (const float*)w.DV, (const int*)st.TY, (const int*)w.keys, (const int*)w.cellStart,
-                       (const int*)w.cellEnd, (const int*)w.cellFluid, out4, n);
+                       (const float*)w.DV, (const int*)st.TY, (const int*)w.keys, (const int*)w.cellFirst,
+                       (const unsigned*)w.fluidBits, out4, n);
   return hipGetLastError();
 }
 

@@ -69,9 +69,18 @@ struct FastWork {
   double* TX[3];   // positions at the start of the step (predictor slot order)
   float4* TV;      // (vx, vy, vz, rho) at the start of the step
   int* cellFluid;  // 1 for cells holding a fluid particle (rebuilt with the cell ranges)
-  int* NL;         // neighbour lists, slot-major: NL[k * n + i] (k < nl_cap)
+  int* NL;         // neighbour lists, slot-major: NL[k * n + i] (k < nl_cap); null until list mode is used
   int* NC;         // neighbour count of each target (> nl_cap: list overflowed)
   int nl_cap;
+  // dense cell table of the fine-row loops (nbr_lds 4): cellFirst[c] = first
+  // sorted index with key >= c, for every cell and a sentinel; fluidBits: one
+  // bit per cell holding fluid; runs / runCount: queue of long runs of cells
+  // filled block-wide (runs_cap entries of 3 ints)
+  int* cellFirst;
+  unsigned* fluidBits;
+  int* runs;
+  int* runCount;
+  int runs_cap;
 };
 
 // Neighbour-list capacity per target: 3-D supports hold ~74 neighbours on a
@@ -89,7 +98,15 @@ struct InstTable {
   int total_cells;
 };
 
-hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s);
+// zero (may be null): one word set to 0 by the launch (the dense table's run counter).
+hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s,
+                            int* zero = nullptr);
+// Dense cell table (cellFirst, fluidBits) of the sorted keys, and the gather
+// into sorted order: the cell build of the fine-row loops.
+hipError_t launch_cells_dense(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
+                              const int* idx, const FastWork& w, int n, hipStream_t s);
+// Capacity of the run queue for a table of `cells` cells.
+int dense_runs_cap(long long cells);
 hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
                                     const int* idx, int* cellStart, int* cellEnd, int* cellFluid, int n,
                                     hipStream_t s);

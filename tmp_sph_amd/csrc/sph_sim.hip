@@ -69,7 +69,10 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
   const int n = s->n;
   FastWork w = s->w;
   hipError_t e;
-  e = timed(s, timing, "hash", [&] { return launch_hash_fast(s->T, s->set[from], w.keys, w.idx, n, s->stream); });
+  const bool dense = s->tun.nbr_lds == 4;  // the fine-row loops read the dense cell table
+  e = timed(s, timing, "hash", [&] {
+    return launch_hash_fast(s->T, s->set[from], w.keys, w.idx, n, s->stream, dense ? w.runCount : nullptr);
+  });
   if (e != hipSuccess) return e;
   bool alt = false;
   e = timed(s, timing, "radix_sort", [&] {
@@ -83,9 +86,9 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
     w.idx_alt = s->w.idx;
   }
   e = timed(s, timing, "cells_gather", [&] {
+    if (dense) return launch_cells_dense(s->T, s->set[from], s->set[to], w.keys, w.idx, w, n, s->stream);
     return launch_cells_gather_fast(s->T, s->set[from], s->set[to], w.keys, w.idx, w.cellStart, w.cellEnd,
-                                    w.cellFluid, n,
-                                     s->stream);
+                                    w.cellFluid, n, s->stream);
   });
   *out = w;
   return e;
@@ -99,6 +102,8 @@ hipError_t rates_phase(sph_sim* s, int set, const FastWork& w, bool timing) {
     return timed(s, timing, "nsB", [&] { return launch_nsB_fine(s->T, s->set[set], w, s->n, s->tun, s->stream); });
   }
   if (s->tun.nbr_list) {
+    // the lists are allocated by set_tuning; `w` must be a copy taken after that
+    if (!w.NL || !w.NC) return hipErrorInvalidValue;
     hipError_t e =
         timed(s, timing, "nbrlist", [&] { return launch_nbrlist_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
     if (e != hipSuccess) return e;
@@ -106,6 +111,32 @@ hipError_t rates_phase(sph_sim* s, int set, const FastWork& w, bool timing) {
   hipError_t e = timed(s, timing, "nsA", [&] { return launch_nsA_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
   if (e != hipSuccess) return e;
   return timed(s, timing, "nsB", [&] { return launch_nsB_fast(s->T, s->set[set], w, s->n, s->tun, s->stream); });
+}
+
+// Neighbour lists (N * nl_cap ints, 1.75 GB at 3.4 M particles in 3-D) exist
+// only while list mode is in use.
+int ensure_lists(sph_sim* s) {
+  if (s->w.NL) return SPH_OK;
+  const size_t N = (size_t)s->n;
+  void *nl = nullptr, *nc = nullptr;
+  hipError_t e = hipMalloc(&nl, N * (size_t)s->w.nl_cap * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&nc, N * sizeof(int));
+  if (e != hipSuccess) {
+    if (nl) (void)hipFree(nl);
+    return hip_status(e, "neighbour lists");
+  }
+  s->w.NL = (int*)nl;
+  s->w.NC = (int*)nc;
+  return SPH_OK;
+}
+
+void drop_lists(sph_sim* s) {
+  if (!s->w.NL) return;
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(s->w.NL);
+  (void)hipFree(s->w.NC);
+  s->w.NL = nullptr;
+  s->w.NC = nullptr;
 }
 
 // Selects the cell grid of tuning.cell_div (1: the instances' own, 2: halved).
@@ -140,6 +171,7 @@ void drop_graph(sph_sim* s) {
 void release(sph_sim* s) {
   output_release(s);
   drop_graph(s);
+  drop_lists(s);
   for (hipEvent_t e : s->event_pool) hipEventDestroy(e);
   for (void* p : s->allocs) hipFree(p);
   if (s->stream) hipStreamDestroy(s->stream);
@@ -249,9 +281,12 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   rc = rc ? rc : dalloc(s, &w.cellEnd, (size_t)cmax);
   rc = rc ? rc : dalloc(s, &w.cellFluid, (size_t)cmax);
   rc = rc ? rc : dalloc(s, &w.DV, N + CAND_PAD);
-  w.nl_cap = d3 ? NL_CAP_3D : NL_CAP_2D;
-  rc = rc ? rc : dalloc(s, &w.NL, N * (size_t)w.nl_cap);
-  rc = rc ? rc : dalloc(s, &w.NC, N);
+  w.nl_cap = d3 ? NL_CAP_3D : NL_CAP_2D;  // the lists themselves are allocated when list mode is first used
+  rc = rc ? rc : dalloc(s, &w.cellFirst, (size_t)cmax + 1);
+  rc = rc ? rc : dalloc(s, &w.fluidBits, (size_t)(cmax >> 5) + 2);
+  w.runs_cap = dense_runs_cap(cmax);
+  rc = rc ? rc : dalloc(s, &w.runs, 3 * (size_t)w.runs_cap);
+  rc = rc ? rc : dalloc(s, &w.runCount, 1);
   rc = rc ? rc : dalloc(s, &w.RATE, N);
   rc = rc ? rc : dalloc(s, &w.TX[0], N);
   if (d3) rc = rc ? rc : dalloc(s, &w.TX[1], N);
@@ -361,6 +396,11 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
   if (cd == 2 && t->nbr_lds != 4) return set_error(SPH_E_INVAL, "cell_div 2 needs nbr_lds 4 (fine rows)");
   if (cd == 2 && s->T2.K == 0) return set_error(SPH_E_INVAL, "cell_div 2: the subdivided grid would be too large");
   if (t->nbr_compact < 0 || t->nbr_compact > 2) return set_error(SPH_E_INVAL, "nbr_compact must be 0, 1 or 2");
+  const bool lists = t->nbr_list != 0 && t->nbr_lds != 4;
+  if (lists) {
+    const int rc = ensure_lists(s);
+    if (rc) return rc;
+  }
   const bool shape_changed = t->nbr_block != s->tun.nbr_block || t->nbr_unroll != s->tun.nbr_unroll ||
                              (t->xcd_remap != 0) != (s->tun.xcd_remap != 0) ||
                              t->nbr_lds != s->tun.nbr_lds ||
@@ -374,6 +414,7 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
   s->tun.nbr_prefetch = t->nbr_prefetch ? 1 : 0;
   s->tun.nbr_list = t->nbr_list ? 1 : 0;
   if (shape_changed) drop_graph(s);  // a captured graph holds the old launch shapes
+  if (!lists) drop_lists(s);         // and no graph holds the list kernels any more
   std::fill(s->tsum_ms.begin(), s->tsum_ms.end(), 0.0);
   std::fill(s->tcount.begin(), s->tcount.end(), 0);
   return SPH_OK;
@@ -617,8 +658,8 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
 
 // Autotuner: times kernels A and B on the live state for every launch shape
 // (scan kernels: neighbour loads prefetched or not x plain or compact loop x
-// LDS-staged or global with block 64/128/256; or neighbour lists with block
-// 64/128/256; each x unroll 1/2/4/8 x XCD remap on/off; fine rows on the 2h
+// LDS-staged or global with block 64/128/256; each x unroll 1/2/4/8 x XCD
+// remap on/off; fine rows on the 2h
 // grid and on the halved grid x unroll 1/2/4 x prefetch x remap) with HIP
 // events on the context's stream and keeps the fastest.  The neighbour phase
 // is re-evaluated from the same sorted set of each grid every time, so the
@@ -645,9 +686,10 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   // the LDS-staged variants always run one-wave blocks: each is tried once
   // per unroll/remap pair (bl == 0: whole stencil, -1: plane by plane,
   // -2: row by row with the neighbour data)
-  for (int list = 0; list < 2 && err == hipSuccess; ++list)
-  for (int prefetch = 0; prefetch < 2 - list && err == hipSuccess; ++prefetch)
-  for (int compact = 0; compact < (list ? 1 : 3) && err == hipSuccess; ++compact)
+  // neighbour-list mode is not among the candidates (set_tuning can still
+  // select it; it wins only for cases of a few thousand particles)
+  for (int prefetch = 0; prefetch < 2 && err == hipSuccess; ++prefetch)
+  for (int compact = 0; compact < 3 && err == hipSuccess; ++compact)
   for (int remap = 0; remap < 2 && err == hipSuccess; ++remap)
     for (int bl : blocks)
       for (int un : unrolls) {
@@ -659,8 +701,7 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
         t.nbr_lds = bl == 0 ? 1 : bl == -1 ? 2 : bl == -2 ? 3 : 0;
         t.nbr_compact = compact;
         t.nbr_prefetch = prefetch;
-        t.nbr_list = list;
-        if (list && bl <= 0) continue;     // the list kernels have no LDS-staged form
+        t.nbr_list = 0;
         if (compact == 1 && bl < 0) continue;  // plane and row staging have no list loop
         if (compact == 2 && (bl == 0 || prefetch)) continue;  // mask groups: unstaged or planes, own loads
         s->tun = t;
@@ -681,6 +722,11 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   for (int cd = 1; cd <= 2 && err == hipSuccess; ++cd) {
     if (cd == 2 && s->T2.K == 0) break;
     use_grid(s, cd);
+    // the sort phase builds the cell table of the tuning in force: the dense
+    // table the fine-row loops read
+    s->tun = saved;
+    s->tun.nbr_lds = 4;
+    s->tun.cell_div = cd;
     FastWork wf;
     err = sort_phase(s, 0, 1, false, &wf);
     for (int remap = 0; remap < 2 && err == hipSuccess; ++remap)
