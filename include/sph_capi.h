@@ -152,6 +152,18 @@ int sph_calcHash(const float* pos, int* particleHash, int* particleIndex, int nu
 /* sortParticles (src/header/mesh.cuh:23, src/lib/mesh.cu:73-86): stable ascending sort of
    the keys carrying the indices, in place.  Keys must lie in [0, numCells). */
 int sph_sortParticles(int* particleHash, int* particleIndex, int numParticles);
+/* sortParticles that allocates nothing: no reference counterpart (the
+   reference's thrust::sort_by_key allocates its temporary storage).  For
+   callers that cannot allocate device memory -- code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+   arguments and results as sph_sortParticles, on the sph_set_stream stream,
+   without waiting for the sort to finish.  Its scratch is one file-scope
+   device array per GPU (48 MB, held until the process ends) sized for
+   numParticles <= SPH_NOALLOC_MAX_PARTICLES; larger sorts are refused with
+   SPH_E_INVAL.  Every call on a GPU shares that GPU's array, so calls that
+   overlap on one GPU (other streams, other host threads) race. */
+#define SPH_NOALLOC_MAX_PARTICLES (1 << 22)
+int sph_sortParticles_noalloc(int* particleHash, int* particleIndex, int numParticles);
 /* reorder (opt_src_v0/header/mesh.cuh:38, opt_src_v0/lib/mesh.cu:106-177): cellStart is
    set to -1 for every cell, then cellStart/cellEnd mark the sorted range of every
    non-empty cell (cellEnd of empty cells is left untouched, as in the reference), and

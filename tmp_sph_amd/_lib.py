@@ -140,6 +140,7 @@ _PROTOS = {
     "sph_case_dambreak": [C.POINTER(DamBreakCfg), I, P, P, P, P, P, C.POINTER(SphParams), IP, IP],
     "sph_calcHash": [P, P, P, I],
     "sph_sortParticles": [P, P, I],
+    "sph_sortParticles_noalloc": [P, P, I],
     "sph_reorder": [P] * 16 + [I, P, P],
     "sph_recover": [P] * 4 + [I] + [P] * 9,
     "sph_SPH_NS_simpleversion": [P] * 9 + [I, P, I, P, P, P],

@@ -231,6 +231,70 @@ extern "C" int sph_sortParticles(int* hash, int* idx, int n) {
   return SPH_OK;
 }
 
+// ------------------------------------------- sortParticles without allocation
+// Scratch of sph_sortParticles_noalloc: a file-scope device array (one copy
+// per GPU, resident for the life of the process) holding the alternate keys,
+// the alternate indices, the per-tile digit histogram and the digit totals of
+// a sort of up to SPH_NOALLOC_MAX_PARTICLES pairs.  Host code never uses the
+// array's name as an address: each device's copy is found once with
+// hipGetSymbolAddress and that pointer is what the kernels receive.
+namespace {
+constexpr long long kNoAllocWords = 3LL * SPH_NOALLOC_MAX_PARTICLES + 256;
+static __device__ __align__(256) int scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+int* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// The calling thread's current device's copy of `scratch`.
+int scratch_of_current_device(int** out) {
+  int dev = 0;
+  SPH_HIP(hipGetDevice(&dev), "sph_sortParticles_noalloc: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return set_error(SPH_E_INVAL, "sph_sortParticles_noalloc: device index too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SPH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "sph_sortParticles_noalloc: hipGetSymbolAddress");
+    if (!p) return set_error(SPH_E_STATE, "sph_sortParticles_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SPH_OK;
+}
+}  // namespace
+
+extern "C" int sph_sortParticles_noalloc(int* hash, int* idx, int n) {
+  sph_params P;
+  hipStream_t s;
+  int rc = current_params(&P, &s);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!hash || !idx))) return set_error(SPH_E_INVAL, "sph_sortParticles_noalloc: bad argument");
+  if (n > SPH_NOALLOC_MAX_PARTICLES)
+    return set_error(SPH_E_INVAL, "sph_sortParticles_noalloc: more than SPH_NOALLOC_MAX_PARTICLES particles");
+  if (n <= 1) return SPH_OK;
+  int* base = nullptr;
+  rc = scratch_of_current_device(&base);
+  if (rc) return rc;
+  // every word of the histogram and the totals is written by the sort before it is read
+  int* keys_alt = base;
+  int* vals_alt = base + SPH_NOALLOC_MAX_PARTICLES;
+  unsigned* hist = reinterpret_cast<unsigned*>(base + 2LL * SPH_NOALLOC_MAX_PARTICLES);
+  unsigned* totals = reinterpret_cast<unsigned*>(base + 3LL * SPH_NOALLOC_MAX_PARTICLES);
+  static_assert(3LL * SPH_NOALLOC_MAX_PARTICLES + 256 == kNoAllocWords, "scratch layout");
+  if ((long long)radix_hist_words(n) > (long long)SPH_NOALLOC_MAX_PARTICLES)
+    return set_error(SPH_E_INVAL, "sph_sortParticles_noalloc: histogram exceeds the scratch array");
+  bool alt = false;
+  SPH_HIP(radix_sort_pairs(hash, idx, keys_alt, vals_alt, hist, totals, n, key_bits(P.numCells), s, &alt),
+          "sortParticles_noalloc");
+  if (alt) {
+    SPH_HIP(hipMemcpyAsync(hash, keys_alt, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s),
+            "sortParticles_noalloc copy");
+    SPH_HIP(hipMemcpyAsync(idx, vals_alt, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s),
+            "sortParticles_noalloc copy");
+  }
+  return SPH_OK;
+}
+
 extern "C" int sph_reorder(const float* oPos, const float* oVel, const float* oRho, const float* oP,
                            const int* oType, const int* oZone, float* sPos, float* sVel, float* sRho,
                            float* sP, int* sType, int* sZone, const int* hash, const int* idx,
