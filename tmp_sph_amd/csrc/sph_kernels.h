@@ -73,11 +73,13 @@ struct FastWork {
   int* NC;         // neighbour count of each target (> nl_cap: list overflowed)
   int nl_cap;
   // dense cell table of the fine-row loops (nbr_lds 4): cellFirst[c] = first
-  // sorted index with key >= c, for every cell and a sentinel; fluidBits: one
-  // bit per cell holding fluid; runs / runCount: queue of long runs of cells
-  // filled block-wide (runs_cap entries of 3 ints)
+  // sorted index with key >= c, for every cell and a sentinel; fluidC/nearC:
+  // coarse-cell fluid flags that let kernel A skip walls away from the water;
+  // runs / runCount: queue of the longest runs of cells (runs_cap entries of
+  // 3 ints)
   int* cellFirst;
-  unsigned* fluidBits;
+  unsigned char* fluidC;  // per cell of the instances' own (2h) grid: holds fluid
+  unsigned char* nearC;   // the same: fluid within one cell
   int* runs;
   int* runCount;
   int runs_cap;
@@ -101,10 +103,11 @@ struct InstTable {
 // zero (may be null): one word set to 0 by the launch (the dense table's run counter).
 hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s,
                             int* zero = nullptr);
-// Dense cell table (cellFirst, fluidBits) of the sorted keys, and the gather
-// into sorted order: the cell build of the fine-row loops.
-hipError_t launch_cells_dense(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
-                              const int* idx, const FastWork& w, int n, hipStream_t s);
+// Dense cell table (cellFirst) of the sorted keys, the gather into sorted
+// order and the coarse fluid flags (Tc: the instances' own grids): the cell
+// build of the fine-row loops.
+hipError_t launch_cells_dense(const InstTable& T, const InstTable& Tc, const FastSet& in, const FastSet& out,
+                              const int* keys, const int* idx, const FastWork& w, int n, hipStream_t s);
 // Capacity of the run queue for a table of `cells` cells.
 int dense_runs_cap(long long cells);
 hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
@@ -121,12 +124,12 @@ hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork
                            hipStream_t s);
 // Fine-row loops (tuning nbr_lds 4): any stencil reach, per-lane trimmed
 // candidate ranges, one cell-table read per wave and stencil row.
-hipError_t launch_nsA_fine(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
-                           hipStream_t s);
+hipError_t launch_nsA_fine(const InstTable& T, const InstTable& Tc, const FastSet& st, const FastWork& w, int n,
+                           const sph_tuning& tu, hipStream_t s);
 hipError_t launch_nsB_fine(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s);
-hipError_t launch_pair_stats_fine(const InstTable& T, const FastSet& st, const FastWork& w, unsigned long long* out4,
-                                  int n, hipStream_t s);
+hipError_t launch_pair_stats_fine(const InstTable& T, const InstTable& Tc, const FastSet& st, const FastWork& w,
+                                  unsigned long long* out4, int n, hipStream_t s);
 hipError_t launch_pc_fast(const InstTable& T, bool predictor, const FastSet& st, const FastWork& w,
                           const int* perm, int n, hipStream_t s);
 hipError_t launch_upload_fast(int dim, const FastSet& st, const float* pos, const float* vel, const float* rho,

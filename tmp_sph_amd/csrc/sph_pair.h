@@ -39,8 +39,10 @@ __device__ __forceinline__ float geom(const float4& pi, const float4& pj, float&
   return DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
 }
 
-// Kernel A for target i (fluid neighbours only).
-template <int DIM, class Visit>
+// Kernel A for target i (fluid neighbours only).  WALL_RHO: also store a
+// wall target's density from its pressure (the inverse EOS, in double for the
+// Tait form); otherwise only the pressure, and wall_density_kernel follows.
+template <int DIM, bool WALL_RHO = true, class Visit>
 __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const float4& pi, const float4& vi,
                                            Visit visit, float4* PR, float4* VP, float* __restrict__ DV,
                                            bool store = true) {
@@ -81,7 +83,7 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
     float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
     if (pw < 0.0f) pw = 0.0f;
     reinterpret_cast<float*>(VP)[4 * i + 3] = pw;
-    reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
+    if (WALL_RHO) reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
   }
 }
 

@@ -86,7 +86,7 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
     w.idx_alt = s->w.idx;
   }
   e = timed(s, timing, "cells_gather", [&] {
-    if (dense) return launch_cells_dense(s->T, s->set[from], s->set[to], w.keys, w.idx, w, n, s->stream);
+    if (dense) return launch_cells_dense(s->T, s->T1, s->set[from], s->set[to], w.keys, w.idx, w, n, s->stream);
     return launch_cells_gather_fast(s->T, s->set[from], s->set[to], w.keys, w.idx, w.cellStart, w.cellEnd,
                                     w.cellFluid, n, s->stream);
   });
@@ -97,7 +97,7 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
 hipError_t rates_phase(sph_sim* s, int set, const FastWork& w, bool timing) {
   if (s->tun.nbr_lds == 4) {
     hipError_t e =
-        timed(s, timing, "nsA", [&] { return launch_nsA_fine(s->T, s->set[set], w, s->n, s->tun, s->stream); });
+        timed(s, timing, "nsA", [&] { return launch_nsA_fine(s->T, s->T1, s->set[set], w, s->n, s->tun, s->stream); });
     if (e != hipSuccess) return e;
     return timed(s, timing, "nsB", [&] { return launch_nsB_fine(s->T, s->set[set], w, s->n, s->tun, s->stream); });
   }
@@ -283,7 +283,8 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   rc = rc ? rc : dalloc(s, &w.DV, N + CAND_PAD);
   w.nl_cap = d3 ? NL_CAP_3D : NL_CAP_2D;  // the lists themselves are allocated when list mode is first used
   rc = rc ? rc : dalloc(s, &w.cellFirst, (size_t)cmax + 1);
-  rc = rc ? rc : dalloc(s, &w.fluidBits, (size_t)(cmax >> 5) + 2);
+  rc = rc ? rc : dalloc(s, &w.fluidC, (size_t)ctot);
+  rc = rc ? rc : dalloc(s, &w.nearC, (size_t)ctot);
   w.runs_cap = dense_runs_cap(cmax);
   rc = rc ? rc : dalloc(s, &w.runs, 3 * (size_t)w.runs_cap);
   rc = rc ? rc : dalloc(s, &w.runCount, 1);
@@ -642,7 +643,7 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
   SPH_HIP(hipMemsetAsync(s->probe, 0, 4 * sizeof(unsigned long long), s->stream), "stats clear");
   if (s->tun.nbr_lds == 4) {
     // (the visit stages dofv with the candidates; the counts do not use it)
-    SPH_HIP(launch_pair_stats_fine(s->T, s->set[1], w, s->probe, s->n, s->stream), "stats");
+    SPH_HIP(launch_pair_stats_fine(s->T, s->T1, s->set[1], w, s->probe, s->n, s->stream), "stats");
   } else {
     SPH_HIP(launch_pair_stats(s->T, s->set[1], w, s->probe, s->n, s->stream), "stats");
   }
