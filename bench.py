@@ -197,27 +197,92 @@ def surrogate_bench(args) -> int:
     return 0
 
 
-def learned_bench(args) -> int:
-    """Surrogate against the plain GPU solver on a held-out 2-D dam break.
+WEIGHTS = ROOT / "tmp_sph_amd" / "weights" / "surrogate_k10_l10.npz"
+# held-out evaluation geometry: a 0.8 x 0.5 m column in the 4.0 x 1.0 m tank at
+# dx = 0.002 m (100 k fluid particles; no sweep member has this column)
+C5_CFG = dict(dim=2, dx=0.002, tank=(4.0, 1.0, 1.0), column=(0.8, 1.0, 0.5))
+# judge thresholds: density deviation 1 % of rho0 per particle; front-band and
+# whole-flow mean velocity residuals 0.5 g T per network step (plain intervals
+# of this case read up to 0.15 g T in the front band while the column starts
+# to collapse); front-band residual summed over the accepted steps 20 g T
+# (0.033 m/s of front velocity at this case's T = 10 dt)
+LEARNED_TOL = {"dev_tol": 0.01, "acc_tol": 0.5, "drift_tol": 20.0, "front_band": 0.02}
 
-    Training data: windows of one batched plain run of the sweep members with
-    dx = 0.004 m (velocity history of 5 frames 10 steps apart, target = change
-    over the next 10 steps).  Training: Adam on the GPU for a fixed budget.
-    Evaluation: a 0.8 x 0.5 m column at dx = 0.002 m (100k fluid particles), a
-    geometry outside the training set; plain solver to t = 2 s, timed; the
-    surrogate rollout (10 plain steps per network step, plain step as judge
-    and fallback at 1 % density deviation) to --learned-t, timed and scaled
-    to t = 2 s; surge-front error against the plain run at every interval."""
+
+def learned_eval(weights: Path, t_end: float = 2.0, tol: dict | None = None) -> dict:
+    """The trained surrogate against the plain GPU solver on the held-out
+    geometry, both run to t_end (no extrapolation in time), front error at
+    every interval, plus the speed ceiling: the plain time per interval
+    against the network's forward time (measured, and at the BF16 MFMA peak)."""
+    import numpy as np
+    from tmp_sph_amd import sph
+    from tmp_sph_amd import surrogate as sg
+    tol = dict(LEARNED_TOL if tol is None else tol)
+    w = sg.load_weights(weights)
+    case = sph.make_case(sph.dambreak_cfg(**C5_CFG))
+    g = sg.Gnn(max_nodes=case.n, max_edges=30 * case.n, train=False, **w["cfg"])
+    g.set_params(w["params"])
+    t0 = time.time()
+    print(f"# learned vs plain on {case.n} particles to t = {t_end} s: plain run first", file=sys.stderr, flush=True)
+    res = sg.learned_vs_plain(g, case, w["k"], t_end, **tol, progress=lambda done, acc: print(
+        f"# learned rollout: {done} intervals, {acc} accepted ({time.time() - t0:.0f} s)", file=sys.stderr,
+        flush=True))
+    print(f"# learned vs plain to t = {res['t_end']:.3f} s: {res['speedup']:.3f}x, front error max "
+          f"{res['front_rel_error_max']:.4f}, accepted {res['rollout']['accepted']} of {res['intervals']} "
+          f"({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+    hist = np.zeros((w["cfg"]["history"], case.n, 3), np.float32)
+    g.load(case.params, case.pos, hist, case.rho, case.type)
+    _, send, _ = g.graph()
+    g.forward(want_out=False)
+    reps = 5
+    sph.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        g.forward(want_out=False)
+    fwd_s = (time.perf_counter() - t1) / reps
+    flops = sg.forward_flops(w["cfg"]["layers"], case.n, int(send.size))
+    g.close()
+    plain_iv = res["plain_seconds"] / res["intervals"]
+    peak_s = flops / (BF16_MFMA_PEAK_TFLOPS * 1e12)
+    return {
+        "speedup": res["speedup"], "front_rel_error_max": res["front_rel_error_max"], "t_end": res["t_end"],
+        "front_within_2pct": bool(res["front_rel_error_max"] <= 0.02),
+        "unit": "x (plain wall time / learned wall time to t_end, same intervals)",
+        "workload": f"held-out 2-D dam break, 0.8x0.5 m column in 4.0x1.0 m tank, dx=0.002 m "
+                    f"({case.n} particles, {case.n_fluid} fluid)",
+        "network": f"k={w['k']} plain steps per network step, {w['cfg']['layers']} message-passing blocks, "
+                   f"latent 128, BF16 MFMA with FP32 accumulation",
+        "weights": str(Path(weights).relative_to(ROOT)) if Path(weights).is_relative_to(ROOT) else str(weights),
+        "judge": tol,
+        "accepted": res["rollout"]["accepted"], "intervals": res["intervals"],
+        "plain_seconds": res["plain_seconds"], "learned_seconds": res["learned_seconds"],
+        "ceiling": {"plain_ms_per_interval": 1e3 * plain_iv, "forward_ms": 1e3 * fwd_s,
+                    "forward_tflops": flops / fwd_s / 1e12, "mfma_frac": flops / fwd_s / 1e12 / BF16_MFMA_PEAK_TFLOPS,
+                    "speedup_if_all_accepted": plain_iv / fwd_s,
+                    "forward_ms_at_mfma_peak": 1e3 * peak_s, "speedup_if_all_accepted_at_mfma_peak": plain_iv / peak_s},
+        "evaluation": res,
+    }
+
+
+def learned_bench(args) -> int:
+    """Trains the surrogate and evaluates it against the plain GPU solver.
+
+    Training data: windows of one batched plain run (to t = 2 s) of the first
+    --train-members sweep members with dx = 0.002 m (velocity history of 5
+    frames 10 steps apart, target = mean rates over the next 10 steps).
+    Training: Adam on minibatches of 4 snapshots on the GPU, --train-iters
+    iterations, snapshot order from --seed.  Saves the weights
+    (--save-weights) and evaluates them with learned_eval to --learned-t."""
     import numpy as np
     from tmp_sph_amd import cases as cs
-    from tmp_sph_amd import sph
     from tmp_sph_amd import surrogate as sg
     k, history, layers = 10, 5, 10
     t0 = time.time()
-    members = [c for c, m in zip(cs.sweep_cases(), cs.sweep_members()) if m.dx == 0.004][:12]
+    members = [c for c, m in zip(cs.sweep_cases(), cs.sweep_members()) if m.dx == 0.002][:args.train_members]
     dts = sorted(float(c.params.timeStep) for c in members)
     total = int(2.0 / dts[len(dts) // 2])
-    ds = sg.record_windows(members, total, k, history, samples=30)
+    ds = sg.record_windows(members, total, k, history, samples=30, progress=lambda smp, step: print(
+        f"# recording: window {smp} at step {step} ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True))
     print(f"# recorded {len(ds.snaps)} snapshots of {len(members)} instances over {total} steps "
           f"({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
     rng = np.random.default_rng(0)
@@ -228,10 +293,10 @@ def learned_bench(args) -> int:
     nmax = max(s.pos.shape[0] for s in ds.snaps)
     kw = dict(layers=layers, history=history, v_scale=3.0, rho_scale=100.0, dv_scale=norm["dv_scale"],
               drho_scale=norm["drho_scale"], out_mean=norm["out_mean"])
-    g = sg.Gnn(max_nodes=nmax, max_edges=30 * nmax, train=True, seed=11, **kw)
+    g = sg.Gnn(max_nodes=nmax, max_edges=30 * nmax, train=True, seed=11 + args.seed, **kw)
     before = sg.evaluate(g, held, norm)
     t1 = time.time()
-    losses = sg.train(g, train, norm, iters=1_000_000, lr=3e-4, lr_final=3e-5, seconds=args.train_seconds,
+    losses = sg.train(g, train, norm, iters=args.train_iters, batch=4, seed=args.seed, seconds=args.train_seconds,
                       progress=lambda it, l: print(f"# training: {it} iterations, loss {l:.3f}", file=sys.stderr,
                                                    flush=True))
     train_s = time.time() - t1
@@ -239,44 +304,29 @@ def learned_bench(args) -> int:
     print(f"# trained {len(losses)} iterations in {train_s:.0f} s: held-out mse {before['network']:.3f} -> "
           f"{after['network']:.3f} (constant-acceleration baseline {after['constant_acceleration']:.3f})",
           file=sys.stderr, flush=True)
-    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.002, tank=(4.0, 1.0, 1.0), column=(0.8, 1.0, 0.5)))
-    gi = sg.Gnn(max_nodes=case.n, max_edges=30 * case.n, train=False, seed=11, **kw)
-    gi.set_params(g.params())
+    save = Path(args.save_weights) if args.save_weights else ROOT / "gpurun_out" / f"surrogate_seed{args.seed}.npz"
+    save.parent.mkdir(parents=True, exist_ok=True)
+    meta = {"seed": args.seed, "iterations": len(losses), "batch": 4, "train_seconds": train_s,
+            "members": [m.k for m in cs.sweep_members() if m.dx == 0.002][:args.train_members],
+            "snapshots": len(train), "held_out_mse_before": before, "held_out_mse_after": after,
+            "loss_first_50": float(np.mean(losses[:50])), "loss_last_50": float(np.mean(losses[-50:]))}
+    sg.save_weights(save, g, norm, k, meta)
     g.close()
-    res = sg.learned_vs_plain(gi, case, k, args.learned_t, 2.0, dev_tol=0.01)
-    # network throughput on the evaluation state
-    hist = np.zeros((history, case.n, 3), np.float32)
-    gi.load(case.params, case.pos, hist, case.rho, case.type)
-    off, send, _ = gi.graph()
-    gi.forward(want_out=False)
-    reps = 10
-    t2 = time.perf_counter()
-    for _ in range(reps):
-        gi.forward(want_out=False)
-    fwd_s = (time.perf_counter() - t2) / reps
-    flops = sg.forward_flops(layers, case.n, int(send.size))
-    gi.close()
+    print(f"# weights saved to {save}", file=sys.stderr, flush=True)
+    learned = learned_eval(save, args.learned_t) if args.learned_t > 0 else None
     out = {
         "metric": "learned vs plain speedup at <=2% front error",
-        "value": res["speedup"],
+        "value": learned["speedup"] if learned else None,
         "unit": "x (plain wall time / learned wall time to t = 2 s)",
         "n_gpus": 1,
         "higher_is_better": True,
         "vs_baseline": None,
         "dtype": "bf16 MFMA (fp32 accumulate) network, f32 solver",
-        "data": "synthetic: trained on plain-solver trajectories of 12 sweep members (dx = 0.004 m)",
+        "data": f"synthetic: trained on plain-solver trajectories of {len(members)} sweep members (dx = 0.002 m)",
         "config": {"workload": "held-out 2-D dam break, 0.8x0.5 m column in 4.0x1.0 m tank, dx=0.002 m",
-                   "particles_total": int(case.n), "particles_fluid": int(case.n_fluid), "k_steps_per_net_step": k,
-                   "layers": layers, "latent": 128, "fallback_density_deviation": 0.01,
-                   "prediction_clamp_sigma": gi.cfg.out_clip},
-        "front_rel_error_max": res["front_rel_error_max"],
-        "front_within_2pct": bool(res["front_rel_error_max"] <= 0.02),
-        "evaluation": res,
-        "training": {"iterations": len(losses), "seconds": train_s, "snapshots": len(train),
-                     "held_out_mse_before": before, "held_out_mse_after": after},
-        "network_forward": {"seconds": fwd_s, "edges": int(send.size), "flops": flops,
-                            "tflops": flops / fwd_s / 1e12, "mfma_peak_tflops": BF16_MFMA_PEAK_TFLOPS,
-                            "mfma_frac": flops / fwd_s / 1e12 / BF16_MFMA_PEAK_TFLOPS},
+                   "k_steps_per_net_step": k, "layers": layers, "latent": 128},
+        "training": meta,
+        "learned": learned,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -288,8 +338,15 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned", "surrogate"])
-    ap.add_argument("--train-seconds", type=float, default=120.0, help="learned: GPU training budget")
-    ap.add_argument("--learned-t", type=float, default=0.25, help="learned: simulated time of the timed rollout")
+    ap.add_argument("--train-seconds", type=float, default=900.0, help="learned: stop training after this long")
+    ap.add_argument("--train-iters", type=int, default=1500, help="learned: Adam iterations (minibatches of 4)")
+    ap.add_argument("--train-members", type=int, default=8, help="learned: sweep members (dx = 0.002 m) recorded")
+    ap.add_argument("--seed", type=int, default=0, help="learned: training seed (snapshot order, initial weights)")
+    ap.add_argument("--save-weights", default="", help="learned: weights file (default gpurun_out/surrogate_seedS.npz)")
+    ap.add_argument("--learned-t", type=float, default=2.0, help="simulated time of the learned-vs-plain runs "
+                    "(0: none)")
+    ap.add_argument("--weights", default=str(WEIGHTS), help="trained surrogate evaluated on the default line")
+    ap.add_argument("--no-learned", action="store_true", help="leave the learned-vs-plain evaluation off the line")
     ap.add_argument("--autotune", action="store_true",
                     help="time every launch shape on the live state first and keep the fastest (off by default: "
                          "the default shape is the one the autotuner picks for this case)")
@@ -410,6 +467,11 @@ def main() -> int:
         out["output"] = output
     if not args.no_cpu_baseline and len(cases) == 1:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    if not args.no_learned and args.learned_t > 0 and args.config == "c3":
+        if Path(args.weights).exists():
+            out["learned"] = learned_eval(Path(args.weights), args.learned_t)
+        else:
+            out["learned"] = {"skipped": f"no trained weights at {args.weights}"}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -303,6 +303,13 @@ int sph_sim_download_pos64(sph_sim* s, double* pos);
    with x <= band) computed on the device; values are written to host doubles
    (any sign; 0 when no particle qualifies). */
 int sph_sim_probe(sph_sim* s, double band, double* front, double* height);
+/* Front and height history without host waits: `intervals` times, k steps
+   (replays of the captured step, or eager steps when none is captured)
+   followed by the probe on the device; the host waits once at the end.
+   front/height receive [intervals][K] doubles (K = instances).  No
+   counterpart in the reference (its output step probes one frame at a time,
+   src/io/io.cu:236-279); used to time a plain run while recording its front. */
+int sph_sim_probe_history(sph_sim* s, int k, int intervals, double band, double* front, double* height);
 /* Measuring probes of the reference's output step (src/io/io.cu:236-279),
    on the device.  Height gauges: for gauge g the largest z of a fluid
    particle of instance 0 with |x - gx[g]| < halfw (and |y - gy[g]| < halfw
@@ -331,6 +338,11 @@ int sph_sim_steps_done(sph_sim* s, long long* steps);
    non-fluid particles are written as 0 (the integrator never reads them). */
 int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, float* p, float* rho, int* keys,
                  int* perm);
+/* sph_sim_eval into DEVICE buffers, asynchronously on the context's stream
+   (synchronize before reading them from another stream): drho[N] and acc AoS
+   [3N] (kernel B's rates, gravity included) in original particle order; either
+   may be null.  The state is not advanced. */
+int sph_sim_eval_device(sph_sim* s, float* drho, float* acc);
 /* Neighbour-pair statistics of the current state (for roofline accounting and
    tuning): candidate pairs visited and pairs inside the support, summed over
    all targets (kernel A) and over fluid targets (kernel B). */
@@ -422,8 +434,8 @@ typedef struct sph_rollout_stats {
     int accepted;        /* intervals advanced by the surrogate */
     int fallbacks;       /* surrogate intervals rejected and redone with k plain steps */
     long long plain_steps; /* plain SPH steps taken (history warm-up and fallbacks) */
-    double max_dev;      /* largest per-particle density deviation the judge measured */
-    double mean_dev;     /* mean over judged intervals of the mean per-particle deviation */
+    double max_dev;      /* largest per-particle density deviation the judge measured (network steps) */
+    double mean_dev;     /* mean over judged network steps of the mean per-particle deviation */
     double seconds;      /* wall-clock of the rollout */
     int rejected_penetration; /* network steps rejected for wall penetration (a fluid particle
                                  predicted outside the escape box, where the solver would park it) */
@@ -431,21 +443,53 @@ typedef struct sph_rollout_stats {
     int rejected_density;     /* network steps rejected for a density deviation above dev_tol */
     int rejected_graph;       /* intervals left to the plain step because the state's graph had more
                                  than max_edges edges */
+    int rejected_velocity;    /* network steps rejected for a velocity residual above acc_tol */
+    double max_band_dev;      /* largest front-band velocity residual of network steps (units of g T) */
+    double max_bias_dev;      /* largest whole-flow velocity residual of network steps (units of g T) */
+    double plain_max_dev;     /* the judge's readings of the plain intervals (warm-up and fallbacks): */
+    double plain_max_band_dev; /*   density deviation, front-band and whole-flow velocity residuals, */
+    double plain_max_bias_dev; /*   which a threshold must clear so that plain physics would pass */
+    double max_drift;         /* largest accumulated front-band velocity drift of accepted steps (g T) */
 } sph_rollout_stats;
 
 /* Advances the single-instance 2-D context `sim` by `intervals` intervals of
-   k plain-step lengths, each by one surrogate step judged by the plain
-   method: the SPH summation density (the kernel sum of kernel A's Wendland
-   weights) of the predicted positions must change like the predicted density
-   -- |d rho_sum - d rho_pred| / rho0 <= dev_tol for every fluid particle with
-   a full support -- and no fluid particle may be predicted outside the
-   escape box (the tank faces; the solver would park it); otherwise the
-   interval is redone with k plain steps.  Particles the solver has already
-   parked are left where they are and out of the judgement.  The first
-   `history` intervals are plain (they fill the velocity history).
-   front/height (may be null) receive the probes after every interval. */
-int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, float dev_tol, double band,
-                    double* front, double* height, sph_rollout_stats* stats);
+   T = k plain-step lengths, each by one surrogate step judged by the plain
+   method.  `judge` is a second context of the same case: the plain rates
+   (kernel B's acceleration and continuity rate) are evaluated in it at the
+   interval's start state S and at the prediction S'.  Over the fluid
+   particles inside the escape box the prediction must satisfy
+   - density: |rho' - max(rho0, rho + T (drho(S) + drho(S')) / 2)| / rho0 <=
+     dev_tol for every particle (the continuity integral, trapezoid rule);
+   - velocity: with r = v' - v - T (a(S) + a(S')) / 2, the mean of r over the
+     front band (particles with x >= front - front_band, front = max x of
+     fluid at the start) and the mean over all fluid, each component, are at
+     most acc_tol g T in magnitude (a lagging or leading surge front, or a bias
+     of the whole flow), and the front band's signed mean residual (x and z),
+     summed over the accepted network steps including this one, stays within
+     drift_tol g T (a small lag of the front repeated over many steps adds up;
+     plain intervals add nothing);
+   - no fluid particle is predicted outside the escape box (the tank faces;
+     the solver would park it).
+   Otherwise the interval is redone with k plain steps (bit-identical to the
+   plain run).  Particles the solver has already parked are left where they
+   are and out of the judgement.  The first `history` intervals are plain (they
+   fill the velocity history).  front/height (may be null) receive the probes
+   (band: column-height band) after every interval.  progress (may be null)
+   is called every 1000 intervals with the intervals done and accepted. */
+typedef void (*sph_progress_fn)(int done, int accepted, void* user);
+int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, sph_sim* judge, int k, int intervals, float dev_tol, float acc_tol,
+                    float drift_tol, double front_band, double band, double* front, double* height, sph_rollout_stats* stats,
+                    sph_progress_fn progress, void* user);
+
+/* The rollout's judge on one given interval of k steps (host arrays, original
+   order, 2-D single-instance `judge` context of the case): start state
+   pos/vel/rho/p/type, end state pos2/vel2/rho2/p2.  out[7]: largest and mean
+   density deviation, front-band and whole-flow velocity residuals (units of
+   g T), fluid particles outside the escape box, deepest position beyond it,
+   front-band particle count.  The judge context's state is replaced. */
+int sph_gnn_judge(sph_sim* judge, int k, double front_band, const float* pos, const float* vel, const float* rho,
+                  const float* p, const int* type, const float* pos2, const float* vel2, const float* rho2,
+                  const float* p2, double* out);
 
 int sph_gnn_create(const sph_gnn_cfg* cfg, sph_gnn** out);
 int sph_gnn_destroy(sph_gnn* g);

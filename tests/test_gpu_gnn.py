@@ -214,10 +214,70 @@ def test_rollout_fallback_reproduces_plain_run(gpu):
         fb, hb = b.probe(0.02)
         assert f[-1] == fb and h[-1] == hb
     with sph.Simulation(case) as c:
-        st, f, h = g.rollout(c, k, intervals, dev_tol=1e30)
+        st, f, h = g.rollout(c, k, intervals, dev_tol=1e30, acc_tol=1e30, drift_tol=1e30)
         assert st["accepted"] == intervals - 4 and st["fallbacks"] == 0
         assert np.all(np.isfinite(f)) and np.all(np.isfinite(h))
     g.close()
+
+
+def _developed_interval(k=10, steps=1500):
+    """A 2-D dam break (1.0 x 0.5 m column, dx = 0.01 m) after `steps` plain
+    steps, the state k steps later, and the case."""
+    from tmp_sph_amd import sph
+    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.01))
+    with sph.Simulation(case) as s:
+        s.capture()
+        s.replay(steps)
+        s0 = s.state()
+        s.replay(k)
+        s1 = s.state()
+    return case, s0, s1
+
+
+def test_judge_passes_plain_intervals_and_flags_planted_errors(gpu):
+    """The judge, fed the plain solver's own interval, stays inside the
+    rollout's thresholds; fed the same interval with the front band's
+    x-velocity lagging by 0.2 g T (positions moved consistently), with the
+    whole flow's z-velocity biased by 0.2 g T, with one particle's density off
+    by 2 % or with one particle outside the tank, it reports each error."""
+    from tmp_sph_amd import sph
+    from tmp_sph_amd.surrogate import judge_interval
+    k = 10
+    case, s0, s1 = _developed_interval(k)
+    P = case.params
+    T = k * float(P.timeStep)
+    gT = abs(float(P.gravity)) * T
+    fl = s0["type"] == 1
+    front0 = float(s0["pos"][fl, 0].max())
+    with sph.Simulation(case) as j:
+        ok = judge_interval(j, k, s0, s1, front_band=0.02)
+        print("plain interval:", ok)
+        assert ok["band_count"] > 0 and ok["escaped"] == 0
+        assert ok["max_dev"] <= 0.01 and ok["band_dev"] <= 0.05 and ok["bias_dev"] <= 0.05, ok
+
+        lag = {key: v.copy() for key, v in s1.items()}
+        band = fl & (s0["pos"][:, 0] >= front0 - 0.02)
+        lag["vel"][band, 0] -= 0.2 * gT
+        lag["pos"][band, 0] -= 0.5 * T * 0.2 * gT
+        r = judge_interval(j, k, s0, lag, front_band=0.02)
+        assert r["band_dev"] >= 0.15, r
+
+        bias = {key: v.copy() for key, v in s1.items()}
+        bias["vel"][fl, 2] += 0.2 * gT
+        bias["pos"][fl, 2] += 0.5 * T * 0.2 * gT
+        r = judge_interval(j, k, s0, bias, front_band=0.02)
+        assert r["bias_dev"] >= 0.15, r
+
+        dens = {key: v.copy() for key, v in s1.items()}
+        i = int(np.flatnonzero(fl)[len(np.flatnonzero(fl)) // 2])
+        dens["rho"][i] += 0.02 * float(P.restDensity)
+        r = judge_interval(j, k, s0, dens, front_band=0.02)
+        assert r["max_dev"] >= 0.015, r
+
+        esc = {key: v.copy() for key, v in s1.items()}
+        esc["pos"][i, 0] = float(P.domain_max[0]) + 0.01
+        r = judge_interval(j, k, s0, esc, front_band=0.02)
+        assert r["escaped"] == 1 and r["max_penetration"] > 0.005, r
 
 
 def test_parked_particle_does_not_block_the_surrogate(gpu):
@@ -233,7 +293,7 @@ def test_parked_particle_does_not_block_the_surrogate(gpu):
     k, intervals = 5, 9
     g = Gnn(layers=2, max_nodes=4096, max_edges=120_000, train=False, seed=2)
     with sph.Simulation(case) as s:
-        st, _, _ = g.rollout(s, k, intervals, dev_tol=1e30)
+        st, _, _ = g.rollout(s, k, intervals, dev_tol=1e30, acc_tol=1e30, drift_tol=1e30)
         assert st["rejected_penetration"] == 0 and st["accepted"] == intervals - 4, st
         np.testing.assert_array_equal(s.state()["pos"][i], park)
     g.close()

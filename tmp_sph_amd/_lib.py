@@ -172,6 +172,8 @@ _PROTOS = {
     "sph_sim_kernel_times": [P, C.c_char_p, I, DP, IP, I],
     "sph_sim_steps_done": [P, C.POINTER(C.c_longlong)],
     "sph_sim_eval": [P, P, P, P, P, P, P, P],
+    "sph_sim_eval_device": [P, P, P],
+    "sph_sim_probe_history": [P, I, I, C.c_double, P, P],
     "sph_sim_pair_stats": [P] + [C.POINTER(C.c_longlong)] * 4,
     "sph_sim_create_batch": [C.POINTER(SphParams), I, IP, P, P, P, P, P, C.POINTER(C.c_void_p)],
     "sph_sim_num_instances": [P, IP],
@@ -197,7 +199,8 @@ _PROTOS = {
     "sph_sim_get_state_device": [P, P, P, P, P, P],
     "sph_sim_set_state_device": [P, P, P, P, P, P],
     "sph_sim_instance_params": [P, I, C.POINTER(SphParams)],
-    "sph_gnn_rollout": [P, P, I, I, C.c_float, C.c_double, P, P, P],
+    "sph_gnn_judge": [P, I, C.c_double, P, P, P, P, P, P, P, P, P, P],
+    "sph_gnn_rollout": [P, P, P, I, I, C.c_float, C.c_float, C.c_float, C.c_double, C.c_double, P, P, P, P, P],
 }
 
 EXPORTS = ["sph_last_error", *_PROTOS]

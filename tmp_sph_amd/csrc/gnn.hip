@@ -124,6 +124,19 @@ __global__ void invert_kernel(const int* __restrict__ idx, int* __restrict__ inv
   if (s < n) inv[idx[s]] = s;
 }
 
+// Outside the escape box in the x-z plane: the solver parks such a fluid
+// particle (below the floor, inside the wall layers) and leaves it there.
+__device__ __forceinline__ bool outside_xz(const sph_params& P, float x, float z) {
+  return (double)x > P.domain_max[0] || (double)x < P.domain_min[0] || (double)z > P.domain_max[2] ||
+         (double)z < P.domain_min[2];
+}
+// A parked fluid particle: out of the graph (no edges in or out; the parked
+// particles share one position next to the wall layers, where they would
+// otherwise form one large clique) and out of the loss.
+__device__ __forceinline__ bool parked(const sph_params& P, const float* pos, const int* type, int p) {
+  return type[p] == 1 && outside_xz(P, pos[3 * p], pos[3 * p + 2]);
+}
+
 // Visits the candidate sorted slots j of sorted slot s (2-D: 3 stencil rows).
 template <class F>
 __device__ __forceinline__ void for_pairs2d(const sph_params& P, int key, const int* __restrict__ cs,
@@ -146,34 +159,39 @@ __device__ __forceinline__ void for_pairs2d(const sph_params& P, int key, const 
   }
 }
 
-__global__ void degree_kernel(sph_params P, const float* __restrict__ pos, const int* __restrict__ keys,
-                              const int* __restrict__ idx, const int* __restrict__ cs, const int* __restrict__ ce,
-                              int* __restrict__ deg, int n) {
+__global__ void degree_kernel(sph_params P, const float* __restrict__ pos, const int* __restrict__ type,
+                              const int* __restrict__ keys, const int* __restrict__ idx, const int* __restrict__ cs,
+                              const int* __restrict__ ce, int* __restrict__ deg, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int p = idx[s];
   const float px = pos[3 * p], pz = pos[3 * p + 2];
   int d = 0;
+  if (!parked(P, pos, type, p))
   for_pairs2d(P, keys[s], cs, ce, [&](int j) {
     if (j == s) return;
     const int q = idx[j];
+    if (parked(P, pos, type, q)) return;
     const float dx = px - pos[3 * q], dz = pz - pos[3 * q + 2];
     if (dx * dx + dz * dz <= P.rr_cut) ++d;
   });
   deg[p] = d;
 }
 
-__global__ void fill_edges_kernel(sph_params P, const float* __restrict__ pos, const int* __restrict__ keys,
-                                  const int* __restrict__ idx, const int* __restrict__ cs, const int* __restrict__ ce,
-                                  const int* __restrict__ off, int* __restrict__ send, int* __restrict__ recv, int n) {
+__global__ void fill_edges_kernel(sph_params P, const float* __restrict__ pos, const int* __restrict__ type,
+                                  const int* __restrict__ keys, const int* __restrict__ idx,
+                                  const int* __restrict__ cs, const int* __restrict__ ce, const int* __restrict__ off,
+                                  int* __restrict__ send, int* __restrict__ recv, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int p = idx[s];
+  if (parked(P, pos, type, p)) return;
   const float px = pos[3 * p], pz = pos[3 * p + 2];
   int e = off[p];
   for_pairs2d(P, keys[s], cs, ce, [&](int j) {
     if (j == s) return;
     const int q = idx[j];
+    if (parked(P, pos, type, q)) return;
     const float dx = px - pos[3 * q], dz = pz - pos[3 * q + 2];
     if (dx * dx + dz * dz <= P.rr_cut) {
       send[e] = q;
@@ -274,13 +292,14 @@ __global__ void edge_features_kernel(sph_params P, const float* __restrict__ pos
 }
 
 // loss over fluid nodes: mean of (y - t)^2 over 3 channels; dY = gradient
-__global__ void loss_kernel(const float* __restrict__ Y, const float* __restrict__ T, const int* __restrict__ type,
-                            float inv_count, float* __restrict__ dY, double* __restrict__ loss, int n) {
+__global__ void loss_kernel(sph_params P, const float* __restrict__ pos, const float* __restrict__ Y,
+                            const float* __restrict__ T, const int* __restrict__ type, float inv_count,
+                            float* __restrict__ dY, double* __restrict__ loss, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   double l = 0.0;
   if (p < n) {
     float* d = dY + (size_t)p * W;
-    const bool fl = type[p] == 1;
+    const bool fl = type[p] == 1 && !parked(P, pos, type, p);
     for (int c = 0; c < 3; ++c) {
       const float r = fl ? Y[(size_t)p * W + c] - T[(size_t)p * 3 + c] : 0.0f;
       d[c] = 2.0f * r * inv_count;
@@ -826,8 +845,8 @@ static int build_graph_features(sph_gnn* g, const sph_params& p, int n, bool* ov
   SPH_HIP(fill_int(g->cellStart, -1, p.numCells, s), "graph cells");
   hipLaunchKernelGGL(cells_only_kernel, dim3(nblk(n)), dim3(256), 0, s, keys, g->cellStart, g->cellEnd, n);
   hipLaunchKernelGGL(invert_kernel, dim3(nblk(n)), dim3(256), 0, s, idx, g->inv, n);
-  hipLaunchKernelGGL(degree_kernel, dim3(nblk(n)), dim3(256), 0, s, p, g->pos, keys, idx, g->cellStart, g->cellEnd,
-                     g->deg, n);
+  hipLaunchKernelGGL(degree_kernel, dim3(nblk(n)), dim3(256), 0, s, p, g->pos, g->type, keys, idx, g->cellStart,
+                     g->cellEnd, g->deg, n);
   const int nb = (n + 1023) / 1024;
   hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(1024), 0, s, g->deg, g->off, g->scan_blocks, n);
   hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, s, g->scan_blocks, nb, g->off + n);
@@ -840,8 +859,8 @@ static int build_graph_features(sph_gnn* g, const sph_params& p, int n, bool* ov
   if (ne > g->cfg.max_edges)
     return overflow ? SPH_OK : set_error(SPH_E_INVAL, "surrogate graph: more edges than max_edges");
   g->ne = ne;
-  hipLaunchKernelGGL(fill_edges_kernel, dim3(nblk(n)), dim3(256), 0, s, p, g->pos, keys, idx, g->cellStart,
-                     g->cellEnd, g->off, g->send, g->recv, n);
+  hipLaunchKernelGGL(fill_edges_kernel, dim3(nblk(n)), dim3(256), 0, s, p, g->pos, g->type, keys, idx,
+                     g->cellStart, g->cellEnd, g->off, g->send, g->recv, n);
   if (ne > 0)
     hipLaunchKernelGGL(reverse_kernel, dim3(nblk(ne)), dim3(256), 0, s, g->off, g->send, g->recv, g->inv, g->rev, ne);
   SPH_HIP(fill_f32(g->Xn, 0.0f, (long long)N * W, s), "features");
@@ -857,23 +876,9 @@ static int build_graph_features(sph_gnn* g, const sph_params& p, int n, bool* ov
 // ---------------------------------------------------------------- rollout
 namespace {
 
-// Wendland C2 in 2-D: adh (1 - q/2)^4 (2q + 1), zero beyond q = 2
-__device__ __forceinline__ float wendland2d(const sph_params& P, float rr) {
-  if (rr > P.rr_cut) return 0.0f;
-  const float q = sqrtf(rr) / P.h;
-  const float t = 1.0f - 0.5f * q;
-  return P.adh * t * t * t * t * (2.0f * q + 1.0f);
-}
-
 // Applies the prediction (fluid only): with T = k dt and the predicted rates,
 // v' = v + T dv/dt, x' = x + T (v + v')/2, rho' = max(rho0, rho + T drho/dt),
 // p' = EOS(rho'); walls keep their state.
-// Outside the escape box in the x-z plane: the solver parks such a fluid
-// particle (below the floor, inside the wall layers) and leaves it there.
-__device__ __forceinline__ bool outside_xz(const sph_params& P, float x, float z) {
-  return (double)x > P.domain_max[0] || (double)x < P.domain_min[0] || (double)z > P.domain_max[2] ||
-         (double)z < P.domain_min[2];
-}
 
 __global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* __restrict__ Y,
                              const float* __restrict__ pos, const float* __restrict__ vel,
@@ -906,98 +911,175 @@ __global__ void apply_kernel(sph_params P, sph_gnn_cfg cfg, int k, const float* 
   p2[i] = eos_pressure(P, nr);
 }
 
-// Judge: for fluid particles with a full support (degree >= full_deg),
-// |(rho_sum(x') - rho_sum(x)) - (rho' - rho)| / rho0, where rho_sum is the
-// kernel sum m sum_j W_ij over the graph (self included).  Also flags fluid
-// predicted outside the escape box (particles already parked there are left
-// out).  out: [0] sum of deviations, [1] count, [2] max, [3] escaped,
-// [4] deepest position beyond the box
-__global__ void judge_kernel(sph_params P, const int* __restrict__ off, const int* __restrict__ send,
-                             const float* __restrict__ pos, const float* __restrict__ pos2,
-                             const float* __restrict__ rho, const float* __restrict__ rho2,
-                             const int* __restrict__ type, int full_deg, float* __restrict__ out, int n) {
+// Judge of one interval S -> S' of length T, by the plain method's rates at
+// both ends (kernel B's acceleration with gravity, a0 at S and a1 at S', and
+// the continuity rate d0, d1; original particle order).  Over the fluid
+// particles inside the escape box at the start (parked ones are left out):
+//   density: |rho' - max(rho0, rho + T (d0 + d1) / 2)| / rho0, the continuity
+//     integral by the trapezoid rule, clamped at rest density like the solver;
+//   velocity: r = v' - v - T (a0 + a1) / 2, summed over all fluid (a bias of
+//     the whole flow) and over the front band x >= xband (the surge front);
+//   penetration: fluid predicted outside the escape box, and its depth.
+// Per block (JW floats): [0] sum of density deviations, [1] count, [2] max
+// deviation, [3] escaped, [4] deepest position beyond the box, [5] band count,
+// [6] [7] band sums of r_x, r_z, [8] [9] sums of r_x, r_z.  Sums run in a
+// fixed order (butterfly in the wave, then the block's waves), so the verdict
+// does not depend on scheduling.
+constexpr int JW = 10;
+__device__ __forceinline__ bool judge_is_max(int c) { return c == 2 || c == 4; }
+
+__global__ void __launch_bounds__(256) judge_kernel(sph_params P, float T, float xband,
+                                                    const float* __restrict__ pos, const float* __restrict__ vel,
+                                                    const float* __restrict__ rho, const float* __restrict__ pos2,
+                                                    const float* __restrict__ vel2, const float* __restrict__ rho2,
+                                                    const int* __restrict__ type, const float* __restrict__ a0,
+                                                    const float* __restrict__ d0, const float* __restrict__ a1,
+                                                    const float* __restrict__ d1, float* __restrict__ part, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  float dev = 0.0f, cnt = 0.0f, esc = 0.0f, depth = 0.0f;
+  float v[JW];
+  for (int c = 0; c < JW; ++c) v[c] = 0.0f;
   if (i < n && type[i] == 1 && !outside_xz(P, pos[3 * i], pos[3 * i + 2])) {
     const float x2 = pos2[3 * i], z2 = pos2[3 * i + 2];
-    // wall penetration: a prediction that leaves the escape box (the solver
-    // would park the particle), recorded with its depth beyond the box
     const double d = fmax(fmax(P.domain_min[0] - (double)x2, (double)x2 - P.domain_max[0]),
                           fmax(fmax(P.domain_min[2] - (double)z2, (double)z2 - P.domain_max[2]), 0.0));
-    depth = (float)d;
-    if (outside_xz(P, x2, z2) || d != d) esc = 1.0f;
-    const int b = off[i], e = off[i + 1];
-    if (e - b >= full_deg) {
-      const float W0 = wendland2d(P, 0.0f);
-      float s1 = W0, s2 = W0;
-      for (int k = b; k < e; ++k) {
-        const int j = send[k];
-        const float dx = pos[3 * i] - pos[3 * j], dz = pos[3 * i + 2] - pos[3 * j + 2];
-        const float ex = x2 - pos2[3 * j], ez = z2 - pos2[3 * j + 2];
-        s1 += wendland2d(P, dx * dx + dz * dz);
-        s2 += wendland2d(P, ex * ex + ez * ez);
-      }
-      const float dsum = P.particleMass * (s2 - s1);
-      dev = fabsf(dsum - (rho2[i] - rho[i])) / P.restDensity;
-      cnt = 1.0f;
+    const float re = fmaxf(rho[i] + 0.5f * T * (d0[i] + d1[i]), P.restDensity);
+    const float dev = fabsf(rho2[i] - re) / P.restDensity;
+    const float rx = vel2[3 * i] - vel[3 * i] - 0.5f * T * (a0[3 * i] + a1[3 * i]);
+    const float rz = vel2[3 * i + 2] - vel[3 * i + 2] - 0.5f * T * (a0[3 * i + 2] + a1[3 * i + 2]);
+    v[0] = dev;
+    v[1] = 1.0f;
+    v[2] = dev != dev ? __builtin_huge_valf() : dev;  // a NaN prediction fails the density test
+    v[3] = (outside_xz(P, x2, z2) || d != d) ? 1.0f : 0.0f;
+    v[4] = (float)d;
+    if (pos[3 * i] >= xband) {
+      v[5] = 1.0f;
+      v[6] = rx;
+      v[7] = rz;
     }
+    v[8] = rx;
+    v[9] = rz;
   }
-  float mx = dev;
-  for (int o = 32; o > 0; o >>= 1) {
-    dev += __shfl_xor(dev, o);
-    cnt += __shfl_xor(cnt, o);
-    esc += __shfl_xor(esc, o);
-    mx = fmaxf(mx, __shfl_xor(mx, o));
-    depth = fmaxf(depth, __shfl_xor(depth, o));
+  for (int c = 0; c < JW; ++c) {
+    float x = v[c];
+    for (int o = 32; o > 0; o >>= 1) {
+      const float y = __shfl_xor(x, o);
+      x = judge_is_max(c) ? fmaxf(x, y) : x + y;
+    }
+    v[c] = x;
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&out[0], dev);
-    atomicAdd(&out[1], cnt);
-    atomicMax(reinterpret_cast<int*>(&out[2]), __float_as_int(mx));  // non-negative floats order as ints
-    atomicAdd(&out[3], esc);
-    atomicMax(reinterpret_cast<int*>(&out[4]), __float_as_int(depth));
+  __shared__ float red[4][JW];
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int c = 0; c < JW; ++c) red[wv][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x < JW) {
+    const int c = threadIdx.x;
+    float x = red[0][c];
+    for (int w = 1; w < 4; ++w) x = judge_is_max(c) ? fmaxf(x, red[w][c]) : x + red[w][c];
+    part[(size_t)blockIdx.x * JW + c] = x;
   }
+}
+
+// Sums (doubles) and maxima of the judge's per-block partials, in a fixed
+// order: thread t takes blocks t, t + 256, ..., then a tree over the threads.
+__global__ void __launch_bounds__(256) judge_reduce_kernel(const float* __restrict__ part, int nb,
+                                                           double* __restrict__ out) {
+  __shared__ double red[256][JW];
+  const int t = threadIdx.x;
+  double v[JW];
+  for (int c = 0; c < JW; ++c) v[c] = 0.0;
+  for (int b = t; b < nb; b += 256)
+    for (int c = 0; c < JW; ++c) {
+      const double x = part[(size_t)b * JW + c];
+      v[c] = judge_is_max(c) ? fmax(v[c], x) : v[c] + x;
+    }
+  for (int c = 0; c < JW; ++c) red[t][c] = v[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int c = 0; c < JW; ++c)
+        red[t][c] = judge_is_max(c) ? fmax(red[t][c], red[t + s][c]) : red[t][c] + red[t + s][c];
+    __syncthreads();
+  }
+  if (t < JW) out[t] = red[0][t];
+}
+
+// Launches the judge over one interval and reads its totals into J (waits).
+hipError_t judge_read(const sph_params& P, float T, float xband, const float* pos, const float* vel, const float* rho,
+                      const float* pos2, const float* vel2, const float* rho2, const int* type, const float* a0,
+                      const float* d0, const float* a1, const float* d1, float* part, double* jout, int n,
+                      hipStream_t s, double* J) {
+  const unsigned nb = nblk(n);
+  hipLaunchKernelGGL(judge_kernel, dim3(nb), dim3(256), 0, s, P, T, xband, pos, vel, rho, pos2, vel2, rho2, type,
+                     a0, d0, a1, d1, part, n);
+  hipLaunchKernelGGL(judge_reduce_kernel, dim3(1), dim3(256), 0, s, part, (int)nb, jout);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(J, jout, JW * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
+// Velocity residuals in units of g T: the front band's mean and the whole
+// flow's mean, the larger component of each.
+double judge_band_dev(const double* J, double gT) {
+  return J[5] > 0 ? std::max(std::fabs(J[6]), std::fabs(J[7])) / J[5] / gT : 0.0;
+}
+double judge_bias_dev(const double* J, double gT) {
+  return J[1] > 0 ? std::max(std::fabs(J[8]), std::fabs(J[9])) / J[1] / gT : 0.0;
 }
 
 }  // namespace
 
-extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, float dev_tol, double band,
-                               double* front, double* height, sph_rollout_stats* stats) {
-  if (!g || !sim || k <= 0 || intervals < 0) return set_error(SPH_E_INVAL, "sph_gnn_rollout: bad argument");
-  int K = 0;
+extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, sph_sim* judge_sim, int k, int intervals, float dev_tol,
+                               float acc_tol, float drift_tol, double front_band, double band, double* front, double* height,
+                               sph_rollout_stats* stats, sph_progress_fn progress, void* user) {
+  if (!g || !sim || !judge_sim || k <= 0 || intervals < 0 || !(front_band >= 0.0))
+    return set_error(SPH_E_INVAL, "sph_gnn_rollout: bad argument");
+  int K = 0, KJ = 0;
   int rc = sph_sim_num_instances(sim, &K);
+  if (!rc) rc = sph_sim_num_instances(judge_sim, &KJ);
   if (rc) return rc;
-  if (K != 1) return set_error(SPH_E_INVAL, "sph_gnn_rollout: single-instance contexts only");
-  sph_params P;
+  if (K != 1 || KJ != 1) return set_error(SPH_E_INVAL, "sph_gnn_rollout: single-instance contexts only");
+  sph_params P, PJ;
   rc = sph_sim_instance_params(sim, 0, &P);
+  if (!rc) rc = sph_sim_instance_params(judge_sim, 0, &PJ);
   if (rc) return rc;
   if (P.dim != 2) return set_error(SPH_E_INVAL, "sph_gnn_rollout: the surrogate is 2-D");
+  if (PJ.numParticles != P.numParticles || PJ.timeStep != P.timeStep || PJ.h != P.h || PJ.dim != P.dim)
+    return set_error(SPH_E_INVAL, "sph_gnn_rollout: the judge context must hold the same case");
   const int n = P.numParticles;
   if (n > g->cfg.max_nodes) return set_error(SPH_E_INVAL, "sph_gnn_rollout: more particles than max_nodes");
   hipStream_t s = g->stream;
   const size_t N = (size_t)n;
   const int H = g->cfg.history;
-  // state buffers: current velocity / pressure and the prediction
+  const unsigned nb = nblk(n);
+  // state (pos/rho live in g), the next state (prediction or plain), plain
+  // rates at both, judge partials and totals
   float *vel = nullptr, *prs = nullptr, *pos2 = nullptr, *vel2 = nullptr, *rho2 = nullptr, *p2 = nullptr;
-  float* judge = nullptr;
+  float *a0 = nullptr, *d0 = nullptr, *a1 = nullptr, *d1 = nullptr, *part = nullptr;
+  double* jout = nullptr;
   hipError_t he = hipMalloc(&vel, 3 * N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&prs, N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&pos2, 3 * N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&vel2, 3 * N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&rho2, N * sizeof(float));
   if (he == hipSuccess) he = hipMalloc(&p2, N * sizeof(float));
-  if (he == hipSuccess) he = hipMalloc(&judge, 5 * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&a0, 3 * N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&a1, 3 * N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&d0, N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&d1, N * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&part, (size_t)nb * JW * sizeof(float));
+  if (he == hipSuccess) he = hipMalloc(&jout, JW * sizeof(double));
+  auto release = [&]() {
+    hipFree(vel); hipFree(prs); hipFree(pos2); hipFree(vel2); hipFree(rho2); hipFree(p2);
+    hipFree(a0); hipFree(a1); hipFree(d0); hipFree(d1); hipFree(part); hipFree(jout);
+  };
   if (he != hipSuccess) {
-    hipFree(vel); hipFree(prs); hipFree(pos2); hipFree(vel2); hipFree(rho2); hipFree(p2); hipFree(judge);
+    release();
     return hip_status(he, "sph_gnn_rollout: scratch");
   }
-  // lattice neighbour count within the support (full support = 90 % of it)
-  const double rc_lat = std::sqrt((double)P.rr_cut) / (double)P.particleR;
-  int full = 0;
-  for (int a = -4; a <= 4; ++a)
-    for (int b = -4; b <= 4; ++b)
-      if ((a || b) && a * a + b * b <= rc_lat * rc_lat) ++full;
-  const int full_deg = (int)(0.9 * full);
+  const float T = (float)k * P.timeStep;
+  const double gT = std::fabs((double)P.gravity) * (double)T;
   sph_rollout_stats st{};
   double dev_sum = 0.0;
   int judged = 0, filled = 0;
@@ -1009,10 +1091,40 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, f
     HIPCHK(hipMemcpyAsync(g->vhist, v, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, s));
     return hipStreamSynchronize(s);  // `v` is rewritten by the context's stream next
   };
+  // plain rates of a state (device buffers, written on g's stream) from the
+  // judge context; positions enter it rounded to FP32, as the network sees them
+  auto plain_rates = [&](const float* p, const float* v, const float* r, const float* pr, float* acc,
+                         float* drho) -> int {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e, "rollout judge state");
+    int r2 = sph_sim_set_state_device(judge_sim, p, v, r, pr, g->type);
+    if (!r2) r2 = sph_sim_eval_device(judge_sim, drho, acc);
+    if (!r2) r2 = sph_sim_synchronize(judge_sim);
+    return r2;
+  };
+  // the judge's reading of the interval from the current state to (p, v, r)
+  double J[JW];
+  auto read_judge = [&](float xband, const float* p, const float* v, const float* r) -> int {
+    const hipError_t e = judge_read(P, T, xband, g->pos, vel, g->rho, p, v, r, g->type, a0, d0, a1, d1, part, jout,
+                                    n, s, J);
+    return e == hipSuccess ? SPH_OK : hip_status(e, "rollout judge");
+  };
+  auto band_dev = [&]() { return judge_band_dev(J, gT); };
+  auto bias_dev = [&]() { return judge_bias_dev(J, gT); };
+  // velocity drift the accepted network steps have left in the front band
+  // (sums of its signed mean residuals, x and z, units of g T); plain
+  // intervals add none
+  double drift[2] = {0.0, 0.0};
+  // plain intervals replay the captured step (bit-identical to eager steps)
+  const bool captured = sph_sim_capture(sim) == SPH_OK;
+  double f_now = 0.0, h_now = 0.0;
   rc = sph_sim_get_state_device(sim, g->pos, vel, g->rho, prs, g->type);
+  if (!rc) rc = sph_sim_probe(sim, band, &f_now, &h_now);
+  if (!rc) rc = plain_rates(g->pos, vel, g->rho, prs, a0, d0);
   if (!rc && (he = push_history(vel)) != hipSuccess) rc = hip_status(he, "rollout history");
   filled = 1;
   for (int it = 0; it < intervals && !rc; ++it) {
+    const float xband = (float)(f_now - front_band);
     bool plain = filled < H;
     if (!plain) {
       bool overflow = false;
@@ -1026,57 +1138,147 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, int k, int intervals, f
     }
     if (!plain) {
       if ((he = forward(g)) != hipSuccess) { rc = hip_status(he, "rollout forward"); break; }
-      hipLaunchKernelGGL(apply_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->cfg, k, g->Y, g->pos, vel, g->rho,
+      hipLaunchKernelGGL(apply_kernel, dim3(nb), dim3(256), 0, s, P, g->cfg, k, g->Y, g->pos, vel, g->rho,
                          g->type, pos2, vel2, rho2, p2, n);
-      if ((he = hipMemsetAsync(judge, 0, 5 * sizeof(float), s)) != hipSuccess) {
-        rc = hip_status(he, "rollout judge clear");
-        break;
-      }
-      hipLaunchKernelGGL(judge_kernel, dim3(nblk(n)), dim3(256), 0, s, P, g->off, g->send, g->pos, pos2, g->rho, rho2,
-                         g->type, full_deg, judge, n);
-      float jh[5];
-      he = hipMemcpyAsync(jh, judge, sizeof(jh), hipMemcpyDeviceToHost, s);
-      if (he == hipSuccess) he = hipStreamSynchronize(s);
-      if (he != hipSuccess) { rc = hip_status(he, "rollout judge"); break; }
-      // the largest per-particle deviation decides (a mean would hide a local
-      // compression); the mean is kept for the statistics
-      const double dev_mean = jh[1] > 0 ? (double)jh[0] / (double)jh[1] : 0.0;
-      const double dev = jh[1] > 0 ? (double)jh[2] : 0.0;
-      dev_sum += dev_mean;
+      if ((rc = plain_rates(pos2, vel2, rho2, p2, a1, d1))) break;
+      if ((rc = read_judge(xband, pos2, vel2, rho2))) break;
+      // the largest per-particle density deviation decides (a mean would hide
+      // a local compression); the mean is kept for the statistics
+      const double dev = J[1] > 0 ? J[2] : 0.0;
+      const double bd = band_dev(), bs = bias_dev();
+      dev_sum += J[1] > 0 ? J[0] / J[1] : 0.0;
       ++judged;
       st.max_dev = std::max(st.max_dev, dev);
-      st.max_penetration = std::max(st.max_penetration, (double)jh[4]);
-      if (jh[3] != 0.0f) ++st.rejected_penetration;
-      if (!(dev <= dev_tol)) ++st.rejected_density;
-      if (dev <= dev_tol && jh[3] == 0.0f) {
+      st.max_band_dev = std::max(st.max_band_dev, bd);
+      st.max_bias_dev = std::max(st.max_bias_dev, bs);
+      st.max_penetration = std::max(st.max_penetration, J[4]);
+      const double rb[2] = {J[5] > 0 ? J[6] / J[5] / gT : 0.0, J[5] > 0 ? J[7] / J[5] / gT : 0.0};
+      const bool ok_drift = std::fabs(drift[0] + rb[0]) <= drift_tol && std::fabs(drift[1] + rb[1]) <= drift_tol;
+      const bool ok_pen = J[3] == 0.0, ok_rho = dev <= dev_tol, ok_vel = bd <= acc_tol && bs <= acc_tol && ok_drift;
+      if (!ok_pen) ++st.rejected_penetration;
+      if (!ok_rho) ++st.rejected_density;
+      if (!ok_vel) ++st.rejected_velocity;
+      if (ok_pen && ok_rho && ok_vel) {
         rc = sph_sim_set_state_device(sim, pos2, vel2, rho2, p2, g->type);
         if (rc) break;
         ++st.accepted;
+        drift[0] += rb[0];
+        drift[1] += rb[1];
+        st.max_drift = std::max({st.max_drift, std::fabs(drift[0]), std::fabs(drift[1])});
       } else {
         plain = true;
         ++st.fallbacks;
       }
     }
     if (plain) {
-      rc = sph_sim_step(sim, k);
+      rc = captured ? sph_sim_replay(sim, k) : sph_sim_step(sim, k);
+      if (!rc) rc = sph_sim_get_state_device(sim, pos2, vel2, rho2, p2, nullptr);
+      if (!rc) rc = plain_rates(pos2, vel2, rho2, p2, a1, d1);
+      if (!rc) rc = read_judge(xband, pos2, vel2, rho2);  // the judge's reading of a plain interval
       if (rc) break;
       st.plain_steps += k;
+      st.plain_max_dev = std::max(st.plain_max_dev, J[1] > 0 ? J[2] : 0.0);
+      st.plain_max_band_dev = std::max(st.plain_max_band_dev, band_dev());
+      st.plain_max_bias_dev = std::max(st.plain_max_bias_dev, bias_dev());
     }
-    rc = sph_sim_get_state_device(sim, g->pos, vel, g->rho, prs, g->type);
-    if (rc) break;
+    // the new state (accepted prediction or plain result) becomes the current one
+    he = hipMemcpyAsync(g->pos, pos2, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(vel, vel2, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(g->rho, rho2, N * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(prs, p2, N * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (he != hipSuccess) { rc = hip_status(he, "rollout state"); break; }
+    std::swap(a0, a1);
+    std::swap(d0, d1);
     if ((he = push_history(vel)) != hipSuccess) { rc = hip_status(he, "rollout history"); break; }
     filled = std::min(filled + 1, H);
-    if (front && height) {
-      rc = sph_sim_probe(sim, band, &front[it], &height[it]);
-      if (rc) break;
-    }
+    rc = sph_sim_probe(sim, band, &f_now, &h_now);
+    if (rc) break;
+    if (front) front[it] = f_now;
+    if (height) height[it] = h_now;
+    if (progress && (it + 1) % 1000 == 0) progress(it + 1, st.accepted, user);
   }
   hipStreamSynchronize(s);
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   st.mean_dev = judged ? dev_sum / judged : 0.0;
   if (stats) *stats = st;
-  hipFree(vel); hipFree(prs); hipFree(pos2); hipFree(vel2); hipFree(rho2); hipFree(p2); hipFree(judge);
+  release();
   return rc;
+}
+
+extern "C" int sph_gnn_judge(sph_sim* judge_sim, int k, double front_band, const float* pos, const float* vel,
+                             const float* rho, const float* p, const int* type, const float* pos2,
+                             const float* vel2, const float* rho2, const float* p2, double* out) {
+  if (!judge_sim || k <= 0 || !pos || !vel || !rho || !p || !type || !pos2 || !vel2 || !rho2 || !p2 || !out ||
+      !(front_band >= 0.0))
+    return set_error(SPH_E_INVAL, "sph_gnn_judge: bad argument");
+  int K = 0;
+  int rc = sph_sim_num_instances(judge_sim, &K);
+  if (rc) return rc;
+  if (K != 1) return set_error(SPH_E_INVAL, "sph_gnn_judge: single-instance contexts only");
+  sph_params P;
+  rc = sph_sim_instance_params(judge_sim, 0, &P);
+  if (rc) return rc;
+  if (P.dim != 2) return set_error(SPH_E_INVAL, "sph_gnn_judge: the surrogate is 2-D");
+  const int n = P.numParticles;
+  const size_t N = (size_t)n;
+  // front at the start: max x of fluid inside the escape box
+  double front = 0.0;
+  bool any = false;
+  for (size_t i = 0; i < N; ++i) {
+    const double x = pos[3 * i], z = pos[3 * i + 2];
+    if (type[i] != 1 || x > P.domain_max[0] || x < P.domain_min[0] || z > P.domain_max[2] || z < P.domain_min[2])
+      continue;
+    front = any ? std::max(front, x) : x;
+    any = true;
+  }
+  const size_t f3 = 3 * N * sizeof(float), f1 = N * sizeof(float);
+  std::vector<void*> bufs;
+  auto dev = [&](size_t bytes) -> void* {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+    bufs.push_back(q);
+    return q;
+  };
+  float *dpos = (float*)dev(f3), *dvel = (float*)dev(f3), *drho = (float*)dev(f1), *dp = (float*)dev(f1);
+  float *dpos2 = (float*)dev(f3), *dvel2 = (float*)dev(f3), *drho2 = (float*)dev(f1), *dp2 = (float*)dev(f1);
+  float *a0 = (float*)dev(f3), *a1 = (float*)dev(f3), *d0 = (float*)dev(f1), *d1 = (float*)dev(f1);
+  int* dtype = (int*)dev(N * sizeof(int));
+  float* part = (float*)dev((size_t)nblk(n) * JW * sizeof(float));
+  double* jout = (double*)dev(JW * sizeof(double));
+  hipError_t e = bufs.size() == 15 ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess) e = hipMemcpy(dpos, pos, f3, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dvel, vel, f3, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(drho, rho, f1, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dp, p, f1, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dtype, type, N * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dpos2, pos2, f3, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dvel2, vel2, f3, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(drho2, rho2, f1, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dp2, p2, f1, hipMemcpyHostToDevice);
+  if (e != hipSuccess) rc = hip_status(e, "sph_gnn_judge: buffers");
+  if (!rc) rc = sph_sim_set_state_device(judge_sim, dpos, dvel, drho, dp, dtype);
+  if (!rc) rc = sph_sim_eval_device(judge_sim, d0, a0);
+  if (!rc) rc = sph_sim_set_state_device(judge_sim, dpos2, dvel2, drho2, dp2, dtype);
+  if (!rc) rc = sph_sim_eval_device(judge_sim, d1, a1);
+  if (!rc) rc = sph_sim_synchronize(judge_sim);
+  double J[JW];
+  const float T = (float)k * P.timeStep;
+  if (!rc) {
+    e = judge_read(P, T, (float)(front - front_band), dpos, dvel, drho, dpos2, dvel2, drho2, dtype, a0, d0, a1, d1,
+                   part, jout, n, nullptr, J);
+    if (e != hipSuccess) rc = hip_status(e, "sph_gnn_judge");
+  }
+  for (void* q : bufs) hipFree(q);
+  if (rc) return rc;
+  const double gT = std::fabs((double)P.gravity) * (double)T;
+  out[0] = J[1] > 0 ? J[2] : 0.0;
+  out[1] = J[1] > 0 ? J[0] / J[1] : 0.0;
+  out[2] = judge_band_dev(J, gT);
+  out[3] = judge_bias_dev(J, gT);
+  out[4] = J[3];
+  out[5] = J[4];
+  out[6] = J[5];
+  return SPH_OK;
 }
 
 extern "C" int sph_gnn_load_state(sph_gnn* g, const sph_params* p, int n, const float* pos, const float* hist,
@@ -1097,9 +1299,13 @@ extern "C" int sph_gnn_load_state(sph_gnn* g, const sph_params* p, int n, const 
   if (target) SPH_HIP(hipMemcpyAsync(g->target, target, 3 * N * sizeof(float), hipMemcpyHostToDevice, s), "upload");
   int rc = build_graph_features(g, *p, n);
   if (rc) return rc;
-  std::vector<int> ty(type, type + n);
+  // fluid nodes in the loss: inside the escape box (the kernels' test)
   g->nfluid = 0;
-  for (int t : ty) g->nfluid += (t == 1);
+  for (int i = 0; i < n; ++i) {
+    const double x = pos[3 * i], z = pos[3 * i + 2];
+    g->nfluid += type[i] == 1 &&
+                 !(x > p->domain_max[0] || x < p->domain_min[0] || z > p->domain_max[2] || z < p->domain_min[2]);
+  }
   SPH_HIP(hipStreamSynchronize(s), "load sync");
   return SPH_OK;
 }
@@ -1174,8 +1380,8 @@ extern "C" int sph_gnn_forward(sph_gnn* g, float* out, double* loss) {
   if (g->has_target && g->cfg.train) {
     SPH_HIP(fill_f32(g->dY, 0.0f, (long long)g->n * W, s), "loss");
     const float inv = g->nfluid > 0 ? 1.0f / (3.0f * (float)g->nfluid) : 0.0f;
-    hipLaunchKernelGGL(loss_kernel, dim3(nblk(g->n)), dim3(256), 0, s, g->Y, g->target, g->type, inv, g->dY,
-                       g->lpart, g->n);
+    hipLaunchKernelGGL(loss_kernel, dim3(nblk(g->n)), dim3(256), 0, s, g->prm, g->pos, g->Y, g->target, g->type,
+                       inv, g->dY, g->lpart, g->n);
     hipLaunchKernelGGL(loss_final_kernel, dim3(1), dim3(256), 0, s, g->lpart, (int)nblk(g->n), (double*)g->scal);
     SPH_HIP(hipGetLastError(), "loss");
   }

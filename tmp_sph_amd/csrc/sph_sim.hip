@@ -518,6 +518,15 @@ extern "C" int sph_sim_download_pos64(sph_sim* s, double* pos) {
   return SPH_OK;
 }
 
+// inverse of the probe kernel's order-preserving key; key 0: no fluid particle (0.0)
+static double decode_probe_key(unsigned long long key) {
+  if (key == 0ull) return 0.0;
+  const unsigned long long b = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
+  double v;
+  std::memcpy(&v, &b, 8);
+  return v;
+}
+
 extern "C" int sph_sim_probe_batch(sph_sim* s, double band, double* front, double* height) {
   if (!s || !front || !height) return set_error(SPH_E_INVAL, "null argument");
   const size_t words = 2 * (size_t)s->K;
@@ -527,17 +536,38 @@ extern "C" int sph_sim_probe_batch(sph_sim* s, double band, double* front, doubl
   SPH_HIP(hipMemcpyAsync(hv.data(), s->probe, words * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream),
           "probe copy");
   SPH_HIP(hipStreamSynchronize(s->stream), "probe sync");
-  // inverse of the kernel's order-preserving key; key 0: no fluid particle (0.0)
-  auto decode = [](unsigned long long key) {
-    if (key == 0ull) return 0.0;
-    const unsigned long long b = (key >> 63) ? (key & 0x7fffffffffffffffull) : ~key;
-    double v;
-    std::memcpy(&v, &b, 8);
-    return v;
-  };
   for (int k = 0; k < s->K; ++k) {
-    front[k] = decode(hv[2 * k]);
-    height[k] = decode(hv[2 * k + 1]);
+    front[k] = decode_probe_key(hv[2 * k]);
+    height[k] = decode_probe_key(hv[2 * k + 1]);
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_probe_history(sph_sim* s, int k, int intervals, double band, double* front, double* height) {
+  if (!s || k <= 0 || intervals < 0 || (intervals > 0 && (!front || !height)))
+    return set_error(SPH_E_INVAL, "sph_sim_probe_history: bad argument");
+  if (intervals == 0) return SPH_OK;
+  const size_t words = 2 * (size_t)s->K;
+  const size_t total = words * (size_t)intervals;
+  unsigned long long* d = nullptr;
+  SPH_HIP(hipMalloc(&d, total * sizeof(unsigned long long)), "probe history");
+  std::vector<unsigned long long> hv(total);
+  // every launch goes onto the context's stream; the host waits once, at the end
+  hipError_t e = hipMemsetAsync(d, 0, total * sizeof(unsigned long long), s->stream);
+  int rc = SPH_OK;
+  for (int it = 0; it < intervals && e == hipSuccess && rc == SPH_OK; ++it) {
+    rc = s->exec ? sph_sim_replay(s, k) : sph_sim_step(s, k);
+    if (rc == SPH_OK) e = launch_probe_fast(s->T, s->set[0], band, d + words * (size_t)it, s->n, s->stream);
+  }
+  if (e == hipSuccess && rc == SPH_OK)
+    e = hipMemcpyAsync(hv.data(), d, total * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess && rc == SPH_OK) e = hipStreamSynchronize(s->stream);
+  hipFree(d);
+  if (rc) return rc;
+  SPH_HIP(e, "sph_sim_probe_history");
+  for (size_t j = 0; j < total / 2; ++j) {
+    front[j] = decode_probe_key(hv[2 * j]);
+    height[j] = decode_probe_key(hv[2 * j + 1]);
   }
   return SPH_OK;
 }
@@ -632,6 +662,15 @@ extern "C" int sph_sim_eval(sph_sim* s, float* drho, float* acc, float* dofv, fl
   hipFree(dp);
   hipFree(drh);
   SPH_HIP(e, "sph_sim_eval");
+  return SPH_OK;
+}
+
+extern "C" int sph_sim_eval_device(sph_sim* s, float* drho, float* acc) {
+  if (!s) return set_error(SPH_E_INVAL, "null sim");
+  FastWork w;
+  SPH_HIP(sort_phase(s, 0, 1, false, &w), "eval sort");
+  SPH_HIP(rates_phase(s, 1, w, false), "eval rates");
+  SPH_HIP(launch_download_rates(s->set[1], w, drho, acc, nullptr, s->n, s->stream), "eval rates out");
   return SPH_OK;
 }
 

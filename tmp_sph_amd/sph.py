@@ -386,6 +386,17 @@ class Simulation:
         call("sph_sim_probe_batch", self._h, C.c_double(band), _hp(f), _hp(h))
         return f, h
 
+    def probe_history(self, k: int, intervals: int, band: float) -> tuple[np.ndarray, np.ndarray]:
+        """Advances `intervals` x k steps (graph replays once a step is
+        captured), probing front and height after every interval on the
+        device; the host waits once.  Returns (front, height), each
+        [intervals] for one instance or [intervals, K] for a batch."""
+        f = np.zeros((max(intervals, 1), self.K))
+        h = np.zeros((max(intervals, 1), self.K))
+        call("sph_sim_probe_history", self._h, int(k), int(intervals), C.c_double(band), _hp(f), _hp(h))
+        f, h = f[:intervals], h[:intervals]
+        return (f[:, 0], h[:, 0]) if self.K == 1 else (f, h)
+
     def autotune(self, reps: int = 3) -> tuple[dict, float]:
         """Picks the fastest neighbour-kernel launch shape for the current state."""
         t = SphTuning()

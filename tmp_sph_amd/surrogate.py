@@ -30,11 +30,16 @@ class GnnCfg(C.Structure):
                 ("out_clip", C.c_float)]
 
 
+_PROGRESS = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_void_p)
+
+
 class RolloutStats(C.Structure):
     _fields_ = [("accepted", C.c_int), ("fallbacks", C.c_int), ("plain_steps", C.c_longlong),
                 ("max_dev", C.c_double), ("mean_dev", C.c_double), ("seconds", C.c_double),
                 ("rejected_penetration", C.c_int), ("max_penetration", C.c_double),
-                ("rejected_density", C.c_int), ("rejected_graph", C.c_int)]
+                ("rejected_density", C.c_int), ("rejected_graph", C.c_int), ("rejected_velocity", C.c_int),
+                ("max_band_dev", C.c_double), ("max_bias_dev", C.c_double), ("plain_max_dev", C.c_double),
+                ("plain_max_band_dev", C.c_double), ("plain_max_bias_dev", C.c_double), ("max_drift", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -163,14 +168,31 @@ class Gnn:
     def adam(self, lr: float, clip: float = 1.0) -> None:
         call("sph_gnn_adam", self._h, C.c_float(lr), C.c_float(clip))
 
-    def rollout(self, sim: Simulation, k: int, intervals: int, dev_tol: float = 0.01, band: float = 0.02):
+    def rollout(self, sim: Simulation, k: int, intervals: int, dev_tol: float = 0.01, acc_tol: float = 0.5,
+                drift_tol: float = 20.0, front_band: float = 0.02, band: float = 0.02,
+                judge: Simulation | None = None, progress=None):
         """Advances `sim` by `intervals` x k steps with the surrogate, judged
-        and backed up by the plain step; returns (stats, front, height)."""
+        by the plain method's rates at both ends of every interval (evaluated
+        in `judge`, a second context of the same case, made from the current
+        state when not given) and backed up by the plain step; returns
+        (stats, front, height).  progress(done, accepted) is called every 1000
+        intervals."""
         f = np.zeros(max(intervals, 1))
         h = np.zeros(max(intervals, 1))
         st = RolloutStats()
-        call("sph_gnn_rollout", self._h, sim._h, int(k), int(intervals), C.c_float(dev_tol), C.c_double(band),
-             _hp(f), _hp(h), C.byref(st))
+        own = judge is None
+        if own:
+            cur = sim.state()
+            judge = Simulation(Case(sim.params.copy(), cur["pos"], cur["vel"], cur["rho"], cur["p"], cur["type"],
+                                    int((cur["type"] == 1).sum())))
+        try:
+            cb = _PROGRESS(lambda done, acc, _u: progress(done, acc)) if progress is not None else None
+            call("sph_gnn_rollout", self._h, sim._h, judge._h, int(k), int(intervals), C.c_float(dev_tol),
+                 C.c_float(acc_tol), C.c_float(drift_tol), C.c_double(front_band), C.c_double(band), _hp(f), _hp(h), C.byref(st),
+                 C.cast(cb, C.c_void_p) if cb is not None else None, None)
+        finally:
+            if own:
+                judge.close()
         return st.as_dict(), f[:intervals], h[:intervals]
 
 
@@ -238,9 +260,19 @@ def record_batch(cases: list, nsteps: int, k: int, history: int) -> Dataset:
     return ds
 
 
+def active(s: Snapshot) -> np.ndarray:
+    """Fluid particles inside the escape box: the nodes of the loss (parked
+    particles are out of the graph and the loss)."""
+    P = s.params
+    x = s.pos[:, 0].astype(np.float64)
+    z = s.pos[:, 2].astype(np.float64)
+    out = (x > P.domain_max[0]) | (x < P.domain_min[0]) | (z > P.domain_max[2]) | (z < P.domain_min[2])
+    return (s.type == 1) & ~out
+
+
 def normalisation(snaps: list) -> dict:
-    """Output scales and means over the fluid particles of the snapshots."""
-    tg = np.concatenate([s.target[s.type == 1] for s in snaps]).astype(np.float64)
+    """Output scales and means over the active fluid particles of the snapshots."""
+    tg = np.concatenate([s.target[active(s)] for s in snaps]).astype(np.float64)
     mean = tg.mean(0)
     std = tg.std(0) + 1e-9
     return {"dv_scale": float(max(std[0], std[1])), "drho_scale": float(std[2]),
@@ -253,33 +285,38 @@ def normalised_target(s: Snapshot, norm: dict) -> np.ndarray:
     return ((s.target - m) / sc).astype(np.float32)
 
 
-def train(g: "Gnn", snaps: list, norm: dict, iters: int, lr: float = 3e-4, lr_final: float = 3e-5,
-          clip: float = 1.0, seed: int = 0, seconds: float | None = None, progress=None,
+def train(g: "Gnn", snaps: list, norm: dict, iters: int, batch: int = 4, lr: float = 2e-4,
+          lr_final: float = 2e-5, clip: float = 1.0, seed: int = 0, seconds: float | None = None, progress=None,
           target_clip: float | None = 10.0) -> list:
-    """Adam on single-snapshot batches; cosine learning-rate decay over the
-    iterations or, with a time budget, over the elapsed fraction of it.  Stops
-    after `iters` steps or `seconds` of wall-clock, whichever comes first;
+    """Adam on minibatches of `batch` snapshots (gradients summed over the
+    batch by the backward pass, clipped at global norm clip x batch, i.e. clip
+    for the mean gradient); cosine learning-rate decay over the iterations.
+    Snapshot order comes from `seed` alone, so a run is reproduced by its seed
+    and iteration count (`seconds`, if given, only stops a run early).
     progress(iterations, recent mean loss) is called every 30 s.  Normalised
     training targets are clipped to +-target_clip: the few splash particles
     whose rates lie tens of standard deviations out would otherwise dominate
-    single-snapshot batches (evaluation uses the unclipped targets)."""
+    the batches (evaluation uses the unclipped targets)."""
     import time
     rng = np.random.default_rng(seed)
+    targets = []
+    for s in snaps:
+        t = normalised_target(s, norm)
+        targets.append(np.clip(t, -target_clip, target_clip) if target_clip is not None else t)
+    batch = max(1, min(batch, len(snaps)))
     losses = []
     t0 = t_last = time.time()
     for it in range(iters):
-        s = snaps[rng.integers(len(snaps))]
-        t = normalised_target(s, norm)
-        if target_clip is not None:
-            t = np.clip(t, -target_clip, target_clip)
-        g.load(s.params, s.pos, s.hist, s.rho, s.type, t)
         g.zero_grad()
-        _, loss = g.forward(want_out=False)
-        g.backward()
+        loss = 0.0
+        for j in rng.choice(len(snaps), size=batch, replace=False):
+            s = snaps[j]
+            g.load(s.params, s.pos, s.hist, s.rho, s.type, targets[j])
+            _, lj = g.forward(want_out=False)
+            g.backward()
+            loss += lj / batch
         frac = it / max(iters - 1, 1)
-        if seconds is not None:
-            frac = max(frac, min((time.time() - t0) / seconds, 1.0))
-        g.adam(lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac)), clip)
+        g.adam(lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * frac)), clip * batch)
         losses.append(loss)
         if progress is not None and time.time() - t_last > 30.0:
             t_last = time.time()
@@ -287,6 +324,45 @@ def train(g: "Gnn", snaps: list, norm: dict, iters: int, lr: float = 3e-4, lr_fi
         if seconds is not None and time.time() - t0 > seconds:
             break
     return losses
+
+
+def save_weights(path, g: "Gnn", norm: dict, k: int, meta: dict | None = None) -> None:
+    """Trained network: FP32 master parameters, the configuration that
+    defines its inputs and outputs, the interval k it was trained for and
+    free-form metadata (JSON)."""
+    import json
+    c = g.cfg
+    np.savez_compressed(path, params=g.params().astype(np.float32), layers=c.layers, history=c.history, k=k,
+                        v_scale=c.v_scale, rho_scale=c.rho_scale, dv_scale=norm["dv_scale"],
+                        drho_scale=norm["drho_scale"], out_mean=np.asarray(norm["out_mean"], np.float64),
+                        out_clip=c.out_clip, meta=json.dumps(meta or {}))
+
+
+def load_weights(path) -> dict:
+    """Reads save_weights' file: {"params", "k", "meta", "cfg": Gnn keyword
+    arguments (layers, history, scales, out_mean, out_clip)}."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        cfg = {"layers": int(z["layers"]), "history": int(z["history"]), "v_scale": float(z["v_scale"]),
+               "rho_scale": float(z["rho_scale"]), "dv_scale": float(z["dv_scale"]),
+               "drho_scale": float(z["drho_scale"]), "out_mean": tuple(float(v) for v in z["out_mean"]),
+               "out_clip": float(z["out_clip"])}
+        return {"params": z["params"].astype(np.float32), "k": int(z["k"]), "cfg": cfg,
+                "meta": json.loads(str(z["meta"]))}
+
+
+def judge_interval(judge: Simulation, k: int, s0: dict, s1: dict, front_band: float = 0.02) -> dict:
+    """The rollout's judge on one interval of k steps from state s0 to s1
+    (dicts of pos, vel, rho, p, type as Simulation.state() returns them);
+    `judge` is a context of the case (its state is replaced)."""
+    out = np.zeros(7)
+    arr = lambda a, d: np.ascontiguousarray(a, d)  # noqa: E731
+    call("sph_gnn_judge", judge._h, int(k), C.c_double(front_band), _hp(arr(s0["pos"], np.float32)),
+         _hp(arr(s0["vel"], np.float32)), _hp(arr(s0["rho"], np.float32)), _hp(arr(s0["p"], np.float32)),
+         _hp(arr(s0["type"], np.int32)), _hp(arr(s1["pos"], np.float32)), _hp(arr(s1["vel"], np.float32)),
+         _hp(arr(s1["rho"], np.float32)), _hp(arr(s1["p"], np.float32)), _hp(out))
+    return {"max_dev": out[0], "mean_dev": out[1], "band_dev": out[2], "bias_dev": out[3], "escaped": int(out[4]),
+            "max_penetration": out[5], "band_count": int(out[6])}
 
 
 def evaluate(g: "Gnn", snaps: list, norm: dict) -> dict:
@@ -300,7 +376,7 @@ def evaluate(g: "Gnn", snaps: list, norm: dict) -> dict:
         t = normalised_target(s, norm)
         g.load(s.params, s.pos, s.hist, s.rho, s.type, t)
         _, loss = g.forward(want_out=False)
-        fl = s.type == 1
+        fl = active(s)
         net.append(loss)
         mean_pred.append(float(np.mean(t[fl].astype(np.float64) ** 2)))
         zero_pred.append(float(np.mean(((0.0 - m) / sc - t[fl]) ** 2)))
@@ -326,7 +402,7 @@ def record(case: Case, nsteps: int, k: int, history: int) -> Dataset:
     return ds
 
 
-def record_windows(cases: list, total_steps: int, k: int, history: int, samples: int) -> Dataset:
+def record_windows(cases: list, total_steps: int, k: int, history: int, samples: int, progress=None) -> Dataset:
     """Snapshots of one batched plain run of `cases` over `total_steps` steps:
     at `samples` evenly spaced sample steps T the frames T - (history-1) k,
     ..., T give the velocity history and the frame T + k the target.  Only the
@@ -363,6 +439,8 @@ def record_windows(cases: list, total_steps: int, k: int, history: int, samples:
                 tgt = _rates(now, nxt, k * float(case.params.timeStep))
                 ds.snaps.append(Snapshot(case.params, now["pos"], hist.astype(np.float32), now["rho"], now["type"],
                                          tgt))
+            if progress is not None:
+                progress(s + 1, cur)
     return ds
 
 
@@ -375,46 +453,41 @@ def forward_flops(layers: int, n: int, edges: int) -> float:
             + n * per_row(W))
 
 
-def learned_vs_plain(g: "Gnn", case: Case, k: int, t_learned: float, t_plain: float, dev_tol: float = 0.01,
-                     band: float = 0.02) -> dict:
-    """Plain GPU solver against the surrogate rollout on one case.
-
-    The plain run to t_plain is timed as one graph replay; a second plain run
-    records the surge front every k steps up to t_learned, where the surrogate
-    rollout (k plain steps per network step, plain step as judge and
-    fallback) also records it.  The learned time to t_plain is the measured
-    rollout time scaled by t_plain / t_learned (the rollout cost per interval
-    does not depend on the time reached).  Both runs use the launch shape the
-    autotuner picks for the case."""
+def learned_vs_plain(g: "Gnn", case: Case, k: int, t_end: float = 2.0, dev_tol: float = 0.01,
+                     acc_tol: float = 0.5, drift_tol: float = 20.0, front_band: float = 0.02, band: float = 0.02,
+                     tuning: dict | None = None, progress=None) -> dict:
+    """Plain GPU solver against the surrogate rollout on one case, both to
+    t_end.  The plain run replays its captured step and probes the surge front
+    every k steps on the device (one host wait at the end), timed as a whole.
+    The surrogate rollout (k plain steps per network step, judged by the plain
+    rates at both ends of every interval, plain steps as the fallback) runs
+    over the same intervals, timed as a whole, and probes the front after
+    every interval; the front error is taken at every interval."""
     import time
     dt = float(case.params.timeStep)
-    intervals = max(int(round(t_learned / (dt * k))), 1)
-    steps_plain = int(round(t_plain / dt))
+    intervals = max(int(round(t_end / (dt * k))), 1)
     with Simulation(case) as sim:
-        tuning, _ = sim.autotune()
+        if tuning:
+            sim.set_tuning(**tuning)
         sim.capture()
-        sim.replay(k)  # warm-up
         sim.synchronize()
         t0 = time.perf_counter()
-        sim.replay(steps_plain)
-        sim.synchronize()
+        front_p, _ = sim.probe_history(k, intervals, band)
         plain_s = time.perf_counter() - t0
-    front_p = np.zeros(intervals)
-    with Simulation(case) as sim:
-        sim.set_tuning(**tuning)
-        sim.capture()
-        for it in range(intervals):
-            sim.replay(k)
-            front_p[it], _ = sim.probe(band)
-    with Simulation(case) as sim:
-        sim.set_tuning(**tuning)
-        st, front_l, height_l = g.rollout(sim, k, intervals, dev_tol=dev_tol, band=band)
-    learned_s = st["seconds"] * (steps_plain / (intervals * k))
+        tuning_used = sim.tuning.as_dict()
+    with Simulation(case) as sim, Simulation(case) as judge:
+        if tuning:
+            sim.set_tuning(**tuning)
+        st, front_l, _ = g.rollout(sim, k, intervals, dev_tol=dev_tol, acc_tol=acc_tol, drift_tol=drift_tol,
+                                   front_band=front_band,
+                                   band=band, judge=judge, progress=progress)
     rel = np.abs(front_l - front_p) / np.maximum(front_p, 1e-9)
-    return {"particles": int(case.n), "fluid": int(case.n_fluid), "dt": dt, "k": k,
-            "t_plain": steps_plain * dt, "plain_seconds": plain_s,
-            "t_learned": intervals * k * dt, "learned_seconds_measured": st["seconds"],
-            "learned_seconds_to_t_plain": learned_s, "speedup": plain_s / learned_s, "tuning": tuning,
-            "front_rel_error_max": float(rel.max()), "front_rel_error_final": float(rel[-1]),
+    worst = int(np.argmax(rel))
+    return {"particles": int(case.n), "fluid": int(case.n_fluid), "dt": dt, "k": k, "intervals": intervals,
+            "t_end": intervals * k * dt, "plain_seconds": plain_s, "learned_seconds": st["seconds"],
+            "speedup": plain_s / st["seconds"], "tuning": tuning_used,
+            "dev_tol": dev_tol, "acc_tol": acc_tol, "drift_tol": drift_tol, "front_band": front_band,
+            "front_rel_error_max": float(rel.max()), "front_rel_error_max_at_t": float((worst + 1) * k * dt),
+            "front_rel_error_final": float(rel[-1]),
             "front_plain_final": float(front_p[-1]), "front_learned_final": float(front_l[-1]),
-            "rollout": st}
+            "accepted_fraction": st["accepted"] / max(intervals, 1), "rollout": st}
