@@ -58,7 +58,7 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
       [&](int j, const float4& pj) {
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        return rr <= cut && j != i && pj.w > 0.0f;
+        return (rr <= cut) & (j != i) & (pj.w > 0.0f);  // masks, no short-circuit branches
       },
       ExtraA{VP},
       [&](int j, const float4& pj, const float4& vj) {
@@ -126,7 +126,7 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
       [&](int j, const float4& pj) {
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        return rr <= cut && j != i;
+        return (rr <= cut) & (j != i);
       },
       ExtraB{VP, DV},
       [&](int j, const float4& pj, const NbrB& nb) {
