@@ -57,16 +57,109 @@ def _algorithmic_bytes(name: str, dim: int, n: int, nf: int) -> float:
     raise KeyError(name)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu), for the baseline's record."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+ORACLE_BUILD = "g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp, FP32 fields (oracle mode 0)"
+
+
+def _advanced_case(case, state):
+    out = case.copy()
+    out.pos, out.vel, out.rho, out.p, out.type = state["pos"], state["vel"], state["rho"], state["p"], state["type"]
+    return out
+
+
+def cpu_rate(case, steps: int, runs: int, threads: int, warmup: int = 0) -> dict:
+    """The oracle's FP32 step on `threads` host threads: `runs` runs of
+    `steps` steps each (the step loop timed, steady_clock-style wall time),
+    started after `warmup` untimed steps from rest; median fluid
+    particle-steps/s."""
+    from oracle import oracle as orc
+    import numpy as np
+    old = orc.num_threads(0)
+    orc.num_threads(threads)
+    try:
+        start = case
+        if warmup > 0:
+            st, _, _, _ = orc.run(case.params, case, warmup, mode=0)
+            start = _advanced_case(case, st)
+        secs = [orc.run(start.params, start, steps, mode=0)[3] for _ in range(runs)]
+    finally:
+        orc.num_threads(old)
+    rates = [case.n_fluid * steps / s for s in secs]
+    return {"value": float(np.median(rates)), "runs": rates, "steps_per_run": steps, "warmup_steps": warmup,
+            "threads": threads, "seconds": secs}
+
+
 def cpu_baseline(case, seconds_target: float) -> dict:
+    """Bounded sample of the same workload on the host cores: all threads
+    (median of 5 runs) and one thread (one step), with the CPU model."""
     from oracle import oracle as orc
     threads = orc.num_threads(0)
-    # one step first, then as many as fit in the time target (at least 1 more)
-    _, _, _, s1 = orc.run(case.params, case, 1, mode=0)
-    k = max(1, min(20, int(seconds_target / max(s1, 1e-9))))
-    _, _, _, sk = orc.run(case.params, case, k, mode=0)
-    return {"value": case.n_fluid * k / sk, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP32 reference step (serial C++ + OpenMP), {k} full steps of the same case "
-                      f"from rest, {case.n} particles ({case.n_fluid} fluid), {sk:.1f} s"}
+    _, _, _, s1 = orc.run(case.params, case, 1, mode=0)  # also the first-touch warm-up
+    k = max(1, min(20, int(seconds_target / 5 / max(s1, 1e-9))))
+    allc = cpu_rate(case, k, 5, threads)
+    one = cpu_rate(case, 1, 1, 1)
+    return {"value": allc["value"], "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 reference step (C++ + OpenMP) on {threads} threads, median of 5 runs of {k} "
+                      f"step(s) from rest, {case.n} particles ({case.n_fluid} fluid)",
+            "runs": allc["runs"], "serial": {"value": one["value"], "cores": 1, "steps": 1,
+                                             "seconds": one["seconds"][0]},
+            "cpu_model": cpu_model(), "build": ORACLE_BUILD}
+
+
+def cpu_plan(args) -> int:
+    """The host baseline table of every plain configuration: C1 (2000 steps),
+    C2 and C3 (20 steps after 5 warm-up steps), C4 (every sweep member, 20
+    steps after 5), all threads and one thread, median of 5 runs where the
+    time allows; one JSON line."""
+    from oracle import oracle as orc
+    from tmp_sph_amd import cases as cs, sph
+    threads = orc.num_threads(0)
+    out = {"metric": "host baseline (oracle FP32 step)", "unit": "fluid particle-steps/s", "cpu_model": cpu_model(),
+           "threads_all": threads, "build": ORACLE_BUILD, "configs": {}}
+
+    def row(name, case, steps, warmup, runs_all, runs_one, steps_one=None):
+        t0 = time.time()
+        r = {"particles": case.n, "fluid": case.n_fluid,
+             "all": cpu_rate(case, steps, runs_all, threads, warmup),
+             "one": cpu_rate(case, steps_one or steps, runs_one, 1, warmup)}
+        out["configs"][name] = r
+        print(f"# {name}: {r['all']['value']:.4g} ({threads} threads), {r['one']['value']:.4g} (1 thread) "
+              f"fluid particle-steps/s ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+
+    c1 = sph.make_case(cs.named("c1")[1])
+    row("c1", c1, 2000, 0, 5, 5)
+    c2 = sph.make_case(cs.named("c2")[1])
+    row("c2", c2, 20, 5, 5, 1)
+    c3 = sph.make_case(cs.named("c3")[1])
+    row("c3", c3, 20, 5, 5, 1, steps_one=3)
+    t0 = time.time()
+    secs, fl = 0.0, 0
+    for c in cs.sweep_cases():
+        st, _, _, _ = orc.run(c.params, c, 5, mode=0)
+        secs += orc.run(c.params, _advanced_case(c, st), 20, mode=0)[3]
+        fl += c.n_fluid
+    out["configs"]["c4"] = {"instances": len(cs.sweep_cases()), "fluid": fl,
+                            "all": {"value": fl * 20 / secs, "steps_per_run": 20, "warmup_steps": 5,
+                                    "threads": threads, "seconds": secs,
+                                    "note": "members run one after another (the oracle has no batched mode); "
+                                            "one run"}}
+    print(f"# c4: {fl * 20 / secs:.4g} fluid particle-steps/s ({time.time() - t0:.0f} s)", file=sys.stderr,
+          flush=True)
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def copy_probe(mib: int) -> int:
@@ -196,6 +289,9 @@ def surrogate_bench(args) -> int:
     print(json.dumps(out), flush=True)
     return 0
 
+
+# steps of each plain configuration's whole run (the default timed length)
+RUN_STEPS = {"c1": 2000, "c2": 20000, "c3": 10000, "c4": 5000}
 
 WEIGHTS = ROOT / "tmp_sph_amd" / "weights" / "surrogate_k10_l10.npz"
 # held-out evaluation geometry: a 0.8 x 0.5 m column in the 4.0 x 1.0 m tank at
@@ -335,9 +431,15 @@ def learned_bench(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed steps (default: the configuration's whole run: c1 2000, c2 20000, c3 10000, "
+                         "c4 5000)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned", "surrogate"])
+    ap.add_argument("--window", type=int, default=500, help="steps per timed window (the run's time profile)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the timed steps one kernel at a time instead of replaying the captured graph "
+                         "(for kernel-trace profiling: rocprofv3's tracer crashes on long graph replays)")
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned", "surrogate", "cpu-plan"])
     ap.add_argument("--train-seconds", type=float, default=900.0, help="learned: stop training after this long")
     ap.add_argument("--train-iters", type=int, default=1500, help="learned: Adam iterations (minibatches of 4)")
     ap.add_argument("--train-members", type=int, default=8, help="learned: sweep members (dx = 0.002 m) recorded")
@@ -347,14 +449,11 @@ def main() -> int:
                     "(0: none)")
     ap.add_argument("--weights", default=str(WEIGHTS), help="trained surrogate evaluated on the default line")
     ap.add_argument("--no-learned", action="store_true", help="leave the learned-vs-plain evaluation off the line")
-    ap.add_argument("--autotune", action="store_true",
-                    help="time every launch shape on the live state first and keep the fastest (off by default: "
-                         "the default shape is the one the autotuner picks for this case)")
-    ap.add_argument("--no-autotune", action="store_true", help="(default; kept for older command lines)")
-    ap.add_argument("--timing-steps", type=int, default=10, help="eager steps with per-kernel HIP events")
+    ap.add_argument("--timing-steps", type=int, default=1,
+                    help="eager steps with per-kernel HIP events after every timed window")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--nbr-block", type=int, default=0, help="fixed launch shape (skips the autotuner)")
+    ap.add_argument("--nbr-block", type=int, default=0, help="fixed launch shape (default: the context's own)")
     ap.add_argument("--nbr-unroll", type=int, default=0)
     ap.add_argument("--xcd-remap", type=int, default=-1)
     ap.add_argument("--nbr-lds", type=int, default=-1)
@@ -375,6 +474,8 @@ def main() -> int:
         return learned_bench(args)
     if args.config == "surrogate":
         return surrogate_bench(args)
+    if args.config == "cpu-plan":
+        return cpu_plan(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): there is no multi-GPU path; "
               "the slab decomposition it would use is described in the design notes", file=sys.stderr)
@@ -399,28 +500,44 @@ def main() -> int:
         fixed["cell_div"] = args.cell_div
     if fixed:
         sim.set_tuning(**fixed)
-    elif args.autotune and not args.no_autotune:
-        _, tuned_us = sim.autotune(reps=3)
-        print(f"# autotune: {sim.tuning.as_dict()} ({tuned_us:.1f} us for kernels A+B)", file=sys.stderr,
-              flush=True)
     tuned = sim.tuning.as_dict()
     sim.capture()
+    if args.steps <= 0:
+        args.steps = RUN_STEPS[args.config]
     sim.replay(args.warmup)
     sim.synchronize()
-    t0 = time.perf_counter()
-    sim.replay(args.steps)
-    sim.synchronize()
-    elapsed = time.perf_counter() - t0
+    # the timed steps in windows (one synchronisation each): the time profile
+    # of the run, from the lattice at rest to the developed flow.  After every
+    # window, outside the timed region, `timing_steps` eager steps record the
+    # per-kernel device time with HIP events on the simulation stream, so the
+    # kernel averages sample the whole run.
+    sim.set_tuning(flags=1)
+    windows = []
+    elapsed = 0.0
+    done = 0
+    extra = 0
+    while done < args.steps:
+        k = min(args.window, args.steps - done)
+        tw = time.perf_counter()
+        if args.eager:
+            sim.step(k)  # (with per-kernel events when timing is on; only for profiler runs)
+        else:
+            sim.replay(k)
+        sim.synchronize()
+        dt_w = time.perf_counter() - tw
+        elapsed += dt_w
+        windows.append({"first_step": args.warmup + done + extra, "steps": k, "ms_per_step": 1000.0 * dt_w / k})
+        done += k
+        sim.step(args.timing_steps)
+        sim.synchronize()
+        extra += args.timing_steps
     ms_per_step = 1000.0 * elapsed / args.steps
     value = n_fluid * args.steps / elapsed
-    print(f"# timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+    print(f"# timed {args.steps} steps: {ms_per_step:.3f} ms/step (first window {windows[0]['ms_per_step']:.3f}, "
+          f"last {windows[-1]['ms_per_step']:.3f})", file=sys.stderr, flush=True)
 
     output = output_bench(sim, args.output_every, args.output_frames, ms_per_step) if args.output_every > 0 else None
 
-    # per-kernel device time (HIP events on the simulation stream, eager steps)
-    sim.set_tuning(flags=1)
-    sim.step(args.timing_steps)
-    sim.synchronize()
     kt = sim.kernel_times()
     stats = sim.pair_stats()
     front, height = sim.probe(band=0.02)
@@ -449,7 +566,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (f64 positions)",
-        "data": "synthetic dam-break lattice at rest, hydrostatic pressure",
+        "data": "synthetic dam-break lattice released from rest (hydrostatic pressure); the timed steps cover "
+                "the collapse and the surge along the tank",
         "config": {"workload": desc, "instances": len(cases), "particles_total": n_total,
                    "particles_fluid": n_fluid, "dt": prm.timeStep if len(cases) == 1 else "per instance",
                    "kernel_h_over_dx": 1.3, "eos": "tait7" if prm.eos == 1 else "linear",
@@ -457,9 +575,16 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": us,
                      "algorithmic_bytes_per_launch": byts,
+                     "limiter": "VALU instruction issue with divergent lanes (the pair body runs whenever any "
+                                "lane of a wave accepts a candidate), not HBM",
                      "valu": {"achieved_tflops": valu, "peak_tflops": FP32_VALU_PEAK_TFLOPS,
-                              "frac": valu / FP32_VALU_PEAK_TFLOPS, "flops_per_launch": flops}},
+                              "frac": valu / FP32_VALU_PEAK_TFLOPS,
+                              "frac_of_non_packed_peak": valu / (FP32_VALU_PEAK_TFLOPS / 2),
+                              "flops_per_launch": flops}},
+        "windows": windows,
+        "t_sim_end_s": (args.warmup + args.steps + extra) * prm.timeStep if len(cases) == 1 else None,
         "kernels_us": kt,
+        "kernels_sampled": f"{extra} eager steps, {args.timing_steps} after each timed window (average over the run)",
         "pair_stats": stats,
         "probe_after_run": {"front_m": front, "column_height_m": height},
     }

@@ -213,12 +213,16 @@ int sph_PC_correction(float* sortedPos, float* sortedVel, float* sortedDensity,
 typedef struct sph_sim sph_sim;
 
 /* Launch-shape knobs of the fast path, filled with defaults by sph_sim_create
-   (row staging with neighbour data, unroll 2, prefetch) and chosen per
+   (row staging with neighbour data, unroll 2, prefetch, runs of 32 tiles per
+   XCD) and chosen per
    workload by sph_sim_autotune.  None changes the results. */
 typedef struct sph_tuning {
     int nbr_block;       /* threads per block of the neighbour-loop kernels: 64, 128, 192, 256 */
     int nbr_unroll;      /* candidate loads issued together: 1, 2, 4 or 8 */
-    int xcd_remap;       /* 1: each XCD gets a contiguous run of target tiles */
+    int xcd_remap;       /* 0: block b takes target tile b (XCD b % 8);
+                            1: each XCD gets one contiguous run of tiles;
+                            G (a power of two, 2..4096): runs of G consecutive
+                            tiles are dealt to the 8 XCDs in turn */
     int flags;           /* bit 0: per-kernel HIP event timing in sph_sim_step */
     int nbr_lds;         /* 1: one-wave blocks stage the wave's candidate rows in LDS;
                             2: the same one z-plane (three rows) at a time;
@@ -274,7 +278,10 @@ int sph_sim_probe_batch(sph_sim* s, double band, double* front, double* height);
 /* Times the neighbour kernels of the current state over the candidate launch
    shapes (block size, unroll, XCD remap), `reps` evaluations each, and keeps
    the fastest; the state is not advanced.  *best receives the choice and
-   *best_us its kernel A + B time in microseconds (either may be null). */
+   *best_us its kernel A + B time in microseconds (either may be null).
+   Disabled in this build: returns SPH_E_STATE (the search faulted the GPU
+   and the cause is not resolved); a context's default shape is the search's
+   pick for the dam breaks, and sph_sim_set_tuning pins any other. */
 int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* best_us);
 /* Device-to-device state exchange in ORIGINAL particle order (device
    pointers, reference layouts: pos/vel AoS float[3N], rho/p float[N],

@@ -77,6 +77,42 @@ def test_autotune_keeps_results(gpu):
                 np.testing.assert_array_equal(a.pos64(), c.pos64())
 
 
+def test_autotune_refuses_while_disabled(gpu):
+    """The launch-shape search is compiled out (it faulted the GPU with the
+    cause unresolved): the entry point reports a state error before any
+    launch and leaves the context's shape as it was."""
+    from tmp_sph_amd import _lib
+    case = gpu.make_case(gpu.dambreak_cfg(dim=2, dx=0.02))
+    with gpu.Simulation(case) as s:
+        before = s.tuning.as_dict()
+        with pytest.raises(_lib.SphError, match="disabled"):
+            s.autotune(reps=1)
+        assert s.tuning.as_dict() == before
+
+
+@pytest.mark.parametrize("lds", [3, 4])
+def test_xcd_remap_only_moves_tiles_between_blocks(gpu, lds):
+    """Every tile-to-block map (identity, one run per XCD, runs of G tiles
+    dealt to the XCDs in turn, with a partial last round) computes the same
+    tiles: bit-identical steps, single and batched."""
+    from tmp_sph_amd.sph import dambreak_cfg, make_case
+    cases = [make_case(dambreak_cfg(dim=3, dx=0.03, tank=(0.8, 0.4, 0.5), column=(0.3, 0.4, 0.3))), _cases()]
+    for c in cases:
+        ref = None
+        for rm in (0, 1, 2, 16, 64, 4096):
+            with gpu.Simulation(c) as s:
+                s.set_tuning(nbr_lds=lds, xcd_remap=rm)
+                s.step(4)
+                r = s.pos64()
+            if ref is None:
+                ref = r
+                continue
+            np.testing.assert_array_equal(r, ref, err_msg=f"nbr_lds={lds} xcd_remap={rm}")
+    with gpu.Simulation(cases[0]) as s:
+        with pytest.raises(Exception, match="power of two"):
+            s.set_tuning(xcd_remap=48)
+
+
 _VARIANTS = ([(lds, compact, pf, 0, un) for lds in (0, 1) for compact in (0, 1) for pf in (0, 1) for un in (1, 4, 8)]
              + [(2, 0, pf, 0, un) for pf in (0, 1) for un in (1, 4, 8)]
              + [(lds, 2, 0, 0, un) for lds in (0, 2, 3) for un in (1, 2, 8)]

@@ -30,15 +30,27 @@ __device__ __forceinline__ void with_instance(const InstTable& T, int inst, F&& 
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
 // b % 8), each with its own L2.  Neighbouring target tiles share most of
-// their candidates, so the block id is remapped to give every XCD one
-// contiguous run of tiles instead of every eighth tile.
-__device__ __forceinline__ int tile_id(bool remap) {
+// their candidates, so the block id may be remapped:
+//   remap 0   tile = block (every XCD sees every eighth tile);
+//   remap 1   every XCD gets one contiguous run of tiles (a eighth of the
+//             domain each: best reuse, but the fluid is not spread evenly);
+//   remap G   (a power of two >= 2) runs of G consecutive tiles are dealt to
+//             the XCDs in turn, so each XCD's L2 serves whole runs while the
+//             eight XCDs still work on neighbouring parts of the domain.  The
+//             last partial round of 8 G tiles keeps the identity map.
+__device__ __forceinline__ int tile_id(int remap) {
   const int b = blockIdx.x;
-  if (!remap) return b;
+  if (remap == 0) return b;
   const int nb = gridDim.x;
-  const int q = nb >> 3, r = nb & 7;
   const int x = b & 7, k = b >> 3;
-  return x * q + min(x, r) + k;
+  if (remap == 1) {
+    const int q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + k;
+  }
+  const int sh = __builtin_ctz((unsigned)remap);  // G = 1 << sh
+  const int full = nb >> (sh + 3) << (sh + 3);     // tiles in whole rounds of 8 G
+  if (b >= full) return b;
+  return ((((k >> sh) << 3) + x) << sh) + (k & (remap - 1));
 }
 
 struct RowRange {

@@ -620,7 +620,7 @@ hipError_t launch_cells_dense(const InstTable& T, const InstTable& Tc, const Fas
 hipError_t launch_nsA_fine(const InstTable& T, const InstTable& Tc, const FastSet& st, const FastWork& w, int n,
                            const sph_tuning& tu, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int rm = tu.xcd_remap ? 1 : 0;
+  const int rm = tu.xcd_remap;
   const bool pf = tu.nbr_prefetch != 0;
   hipError_t e;
   switch (tu.nbr_unroll) {
@@ -637,7 +637,7 @@ hipError_t launch_nsA_fine(const InstTable& T, const InstTable& Tc, const FastSe
 hipError_t launch_nsB_fine(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int rm = tu.xcd_remap ? 1 : 0;
+  const int rm = tu.xcd_remap;
   const bool pf = tu.nbr_prefetch != 0;
   switch (tu.nbr_unroll) {
     case 1: return pf ? nsB_fine_v<1, true>(T, st, w, n, rm, s) : nsB_fine_v<1, false>(T, st, w, n, rm, s);

@@ -850,7 +850,7 @@ hipError_t nbrlist_u(const InstTable& T, const FastSet& st, const FastWork& w, i
 hipError_t launch_nbrlist_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                                hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0;
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap;
   switch (tu.nbr_unroll) {
     case 1: return nbrlist_u<1>(T, st, w, n, b, rm, s);
     case 2: return nbrlist_u<2>(T, st, w, n, b, rm, s);
@@ -894,7 +894,7 @@ hipError_t nsB_list(const InstTable& T, const FastSet& st, const FastWork& w, in
 hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0, md = mode_of(tu);
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap, md = mode_of(tu);
   if (tu.nbr_list) return nsA_list(T, st, w, n, b, rm, s);
   switch (tu.nbr_unroll) {
     case 1: return nsA_u<1>(T, st, w, n, b, rm, md, s);
@@ -907,7 +907,7 @@ hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork
 hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap ? 1 : 0, md = mode_of(tu);
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap, md = mode_of(tu);
   if (tu.nbr_list) return nsB_list(T, st, w, n, b, rm, s);
   switch (tu.nbr_unroll) {
     case 1: return nsB_u<1>(T, st, w, n, b, rm, md, s);

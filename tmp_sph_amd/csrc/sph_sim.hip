@@ -242,10 +242,12 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   s->bits2 = fine_ok ? key_bits((int)ctot2) : 1;
   s->bits = s->bits1;
   // default launch shape: row staging with neighbour data, unroll 2, prefetch
-  // (the autotuner's pick for the 2-D and 3-D dam breaks from 0.5 M particles up)
+  // (the autotuner's pick for the 2-D and 3-D dam breaks from 0.5 M particles
+  // up), runs of 32 tiles dealt to the XCDs in turn (a third less HBM
+  // traffic in the loops at the same speed in 3-D, 1.5 % faster in 2-D)
   s->tun.nbr_block = 64;
   s->tun.nbr_unroll = 2;
-  s->tun.xcd_remap = 0;
+  s->tun.xcd_remap = 32;
   s->tun.flags = 0;
   s->tun.nbr_lds = 3;
   s->tun.nbr_compact = 0;
@@ -392,6 +394,8 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
   if (t->nbr_unroll != 1 && t->nbr_unroll != 2 && t->nbr_unroll != 4 && t->nbr_unroll != 8)
     return set_error(SPH_E_INVAL, "nbr_unroll must be 1, 2, 4 or 8");
   if (t->nbr_lds < 0 || t->nbr_lds > 4) return set_error(SPH_E_INVAL, "nbr_lds must be 0, 1, 2, 3 or 4");
+  if (t->xcd_remap < 0 || t->xcd_remap > 4096 || (t->xcd_remap > 1 && (t->xcd_remap & (t->xcd_remap - 1))))
+    return set_error(SPH_E_INVAL, "xcd_remap must be 0, 1 or a power of two up to 4096");
   const int cd = t->cell_div <= 0 ? 1 : t->cell_div;
   if (cd != 1 && cd != 2) return set_error(SPH_E_INVAL, "cell_div must be 1 or 2");
   if (cd == 2 && t->nbr_lds != 4) return set_error(SPH_E_INVAL, "cell_div 2 needs nbr_lds 4 (fine rows)");
@@ -403,7 +407,7 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
     if (rc) return rc;
   }
   const bool shape_changed = t->nbr_block != s->tun.nbr_block || t->nbr_unroll != s->tun.nbr_unroll ||
-                             (t->xcd_remap != 0) != (s->tun.xcd_remap != 0) ||
+                             t->xcd_remap != s->tun.xcd_remap ||
                              t->nbr_lds != s->tun.nbr_lds ||
                              t->nbr_compact != s->tun.nbr_compact ||
                              (t->nbr_prefetch != 0) != (s->tun.nbr_prefetch != 0) ||
@@ -411,7 +415,6 @@ extern "C" int sph_sim_set_tuning(sph_sim* s, const sph_tuning* t) {
   s->tun = *t;
   s->tun.cell_div = cd;
   use_grid(s, cd);
-  s->tun.xcd_remap = t->xcd_remap ? 1 : 0;
   s->tun.nbr_prefetch = t->nbr_prefetch ? 1 : 0;
   s->tun.nbr_list = t->nbr_list ? 1 : 0;
   if (shape_changed) drop_graph(s);  // a captured graph holds the old launch shapes
@@ -706,8 +709,22 @@ extern "C" int sph_sim_pair_stats(sph_sim* s, long long* candA, long long* hitsA
 // state is unchanged and every candidate of a grid sees identical work.  The
 // halved grid visits the same pairs in another order, so picking it moves
 // results by rounding only (within the oracle tolerance on that grid).
+//
+// The search is compiled out unless SPH_ENABLE_AUTOTUNE is defined: it took
+// the GPU down with a memory fault, and the retest of the fix of its first
+// cause (list kernels launched before the lists were allocated) faulted again
+// with no cause found, so the entry point reports SPH_E_STATE until one is.
+// A context's default shape is the one the search picked for the dam breaks;
+// sph_sim_set_tuning pins any other.
 extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* best_us) {
   if (!s) return set_error(SPH_E_INVAL, "null sim");
+#ifndef SPH_ENABLE_AUTOTUNE
+  (void)reps;
+  (void)best;
+  (void)best_us;
+  return set_error(SPH_E_STATE, "sph_sim_autotune is disabled in this build (an unresolved GPU memory fault); "
+                                "use sph_sim_set_tuning");
+#else
   if (reps <= 0) reps = 3;
   const sph_tuning saved = s->tun;
   use_grid(s, 1);  // the 2h-grid variants first
@@ -807,6 +824,7 @@ extern "C" int sph_sim_autotune(sph_sim* s, int reps, sph_tuning* best, double* 
   if (best) *best = s->tun;
   if (best_us) *best_us = pick_us;
   return SPH_OK;
+#endif
 }
 
 extern "C" int sph_sim_get_state_device(sph_sim* s, float* pos, float* vel, float* rho, float* p, int* type) {
