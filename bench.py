@@ -87,14 +87,18 @@ def cpu_rate(case, steps: int, runs: int, threads: int, warmup: int = 0) -> dict
     particle-steps/s."""
     from oracle import oracle as orc
     import numpy as np
+    start = case
+    if warmup > 0:  # the warm-up steps on every thread: they only set the start state
+        st, _, _, _ = orc.run(case.params, case, warmup, mode=0)
+        start = _advanced_case(case, st)
     old = orc.num_threads(0)
     orc.num_threads(threads)
+    secs = []
     try:
-        start = case
-        if warmup > 0:
-            st, _, _, _ = orc.run(case.params, case, warmup, mode=0)
-            start = _advanced_case(case, st)
-        secs = [orc.run(start.params, start, steps, mode=0)[3] for _ in range(runs)]
+        for r in range(runs):
+            secs.append(orc.run(start.params, start, steps, mode=0)[3])
+            print(f"#   {case.n} particles, {threads} thread(s): run {r + 1} of {runs}, {steps} steps in "
+                  f"{secs[-1]:.2f} s", file=sys.stderr, flush=True)
     finally:
         orc.num_threads(old)
     rates = [case.n_fluid * steps / s for s in secs]
@@ -147,10 +151,12 @@ def cpu_plan(args) -> int:
     row("c3", c3, 20, 5, 5, 1, steps_one=3)
     t0 = time.time()
     secs, fl = 0.0, 0
-    for c in cs.sweep_cases():
+    for m, c in enumerate(cs.sweep_cases()):
         st, _, _, _ = orc.run(c.params, c, 5, mode=0)
         secs += orc.run(c.params, _advanced_case(c, st), 20, mode=0)[3]
         fl += c.n_fluid
+        if (m + 1) % 8 == 0:
+            print(f"#   c4: {m + 1} members done ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
     out["configs"]["c4"] = {"instances": len(cs.sweep_cases()), "fluid": fl,
                             "all": {"value": fl * 20 / secs, "steps_per_run": 20, "warmup_steps": 5,
                                     "threads": threads, "seconds": secs,
@@ -303,9 +309,13 @@ C5_CFG = dict(dim=2, dx=0.002, tank=(4.0, 1.0, 1.0), column=(0.8, 1.0, 0.5))
 # to collapse); front-band residual summed over the accepted steps 20 g T
 # (0.033 m/s of front velocity at this case's T = 10 dt)
 LEARNED_TOL = {"dev_tol": 0.01, "acc_tol": 0.5, "drift_tol": 20.0, "front_band": 0.02}
+# after a rejected network step, the next 1, 2, 4, ... up to this many
+# intervals are plain without a network attempt (0: attempt every interval)
+LEARNED_MAX_SKIP = 64
 
 
-def learned_eval(weights: Path, t_end: float = 2.0, tol: dict | None = None) -> dict:
+def learned_eval(weights: Path, t_end: float = 2.0, tol: dict | None = None,
+                 max_skip: int = LEARNED_MAX_SKIP) -> dict:
     """The trained surrogate against the plain GPU solver on the held-out
     geometry, both run to t_end (no extrapolation in time), front error at
     every interval, plus the speed ceiling: the plain time per interval
@@ -320,7 +330,7 @@ def learned_eval(weights: Path, t_end: float = 2.0, tol: dict | None = None) -> 
     g.set_params(w["params"])
     t0 = time.time()
     print(f"# learned vs plain on {case.n} particles to t = {t_end} s: plain run first", file=sys.stderr, flush=True)
-    res = sg.learned_vs_plain(g, case, w["k"], t_end, **tol, progress=lambda done, acc: print(
+    res = sg.learned_vs_plain(g, case, w["k"], t_end, **tol, max_skip=max_skip, progress=lambda done, acc: print(
         f"# learned rollout: {done} intervals, {acc} accepted ({time.time() - t0:.0f} s)", file=sys.stderr,
         flush=True))
     print(f"# learned vs plain to t = {res['t_end']:.3f} s: {res['speedup']:.3f}x, front error max "
@@ -349,7 +359,7 @@ def learned_eval(weights: Path, t_end: float = 2.0, tol: dict | None = None) -> 
         "network": f"k={w['k']} plain steps per network step, {w['cfg']['layers']} message-passing blocks, "
                    f"latent 128, BF16 MFMA with FP32 accumulation",
         "weights": str(Path(weights).relative_to(ROOT)) if Path(weights).is_relative_to(ROOT) else str(weights),
-        "judge": tol,
+        "judge": tol, "max_skip": max_skip,
         "accepted": res["rollout"]["accepted"], "intervals": res["intervals"],
         "plain_seconds": res["plain_seconds"], "learned_seconds": res["learned_seconds"],
         "ceiling": {"plain_ms_per_interval": 1e3 * plain_iv, "forward_ms": 1e3 * fwd_s,
@@ -409,7 +419,7 @@ def learned_bench(args) -> int:
     sg.save_weights(save, g, norm, k, meta)
     g.close()
     print(f"# weights saved to {save}", file=sys.stderr, flush=True)
-    learned = learned_eval(save, args.learned_t) if args.learned_t > 0 else None
+    learned = learned_eval(save, args.learned_t, max_skip=args.learned_max_skip) if args.learned_t > 0 else None
     out = {
         "metric": "learned vs plain speedup at <=2% front error",
         "value": learned["speedup"] if learned else None,
@@ -448,6 +458,9 @@ def main() -> int:
     ap.add_argument("--learned-t", type=float, default=2.0, help="simulated time of the learned-vs-plain runs "
                     "(0: none)")
     ap.add_argument("--weights", default=str(WEIGHTS), help="trained surrogate evaluated on the default line")
+    ap.add_argument("--learned-max-skip", type=int, default=LEARNED_MAX_SKIP,
+                    help="learned rollout: longest wait (intervals) before the next network attempt after "
+                         "rejections (0: attempt every interval)")
     ap.add_argument("--no-learned", action="store_true", help="leave the learned-vs-plain evaluation off the line")
     ap.add_argument("--timing-steps", type=int, default=1,
                     help="eager steps with per-kernel HIP events after every timed window")
@@ -594,7 +607,7 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     if not args.no_learned and args.learned_t > 0 and args.config == "c3":
         if Path(args.weights).exists():
-            out["learned"] = learned_eval(Path(args.weights), args.learned_t)
+            out["learned"] = learned_eval(Path(args.weights), args.learned_t, max_skip=args.learned_max_skip)
         else:
             out["learned"] = {"skipped": f"no trained weights at {args.weights}"}
     print(json.dumps(out), flush=True)

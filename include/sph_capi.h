@@ -457,6 +457,8 @@ typedef struct sph_rollout_stats {
     double plain_max_band_dev; /*   density deviation, front-band and whole-flow velocity residuals, */
     double plain_max_bias_dev; /*   which a threshold must clear so that plain physics would pass */
     double max_drift;         /* largest accumulated front-band velocity drift of accepted steps (g T) */
+    int skipped;              /* intervals given to the plain step without a network attempt
+                                 (the back-off after rejections; not judged) */
 } sph_rollout_stats;
 
 /* Advances the single-instance 2-D context `sim` by `intervals` intervals of
@@ -478,15 +480,18 @@ typedef struct sph_rollout_stats {
    - no fluid particle is predicted outside the escape box (the tank faces;
      the solver would park it).
    Otherwise the interval is redone with k plain steps (bit-identical to the
-   plain run).  Particles the solver has already parked are left where they
+   plain run), and with max_skip > 0 the next 1, 2, 4, ... (at most max_skip)
+   intervals after a rejection go to the plain step without a network
+   attempt (the wait doubles with every rejection and clears with an
+   acceptance); max_skip 0 attempts the network at every interval.  Particles the solver has already parked are left where they
    are and out of the judgement.  The first `history` intervals are plain (they
    fill the velocity history).  front/height (may be null) receive the probes
    (band: column-height band) after every interval.  progress (may be null)
    is called every 1000 intervals with the intervals done and accepted. */
 typedef void (*sph_progress_fn)(int done, int accepted, void* user);
 int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, sph_sim* judge, int k, int intervals, float dev_tol, float acc_tol,
-                    float drift_tol, double front_band, double band, double* front, double* height, sph_rollout_stats* stats,
-                    sph_progress_fn progress, void* user);
+                    float drift_tol, double front_band, double band, int max_skip, double* front, double* height,
+                    sph_rollout_stats* stats, sph_progress_fn progress, void* user);
 
 /* The rollout's judge on one given interval of k steps (host arrays, original
    order, 2-D single-instance `judge` context of the case): start state

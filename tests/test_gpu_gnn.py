@@ -220,6 +220,32 @@ def test_rollout_fallback_reproduces_plain_run(gpu):
     g.close()
 
 
+def test_rollout_back_off_after_rejections(gpu):
+    """After each rejected network step the next 1, 2, 4, 4, ... intervals
+    (wait capped at max_skip = 4) are plain without an attempt: with every
+    attempt rejected, 20 intervals are 4 history intervals, attempts at
+    intervals 4, 6, 9, 14 and 19 and 11 skipped ones, and the state equals
+    the plain run bit for bit; with every attempt accepted nothing is
+    skipped."""
+    from tmp_sph_amd import sph
+    from tmp_sph_amd.surrogate import Gnn
+    case = sph.make_case(sph.dambreak_cfg(dim=2, dx=0.02))
+    k, intervals = 5, 20
+    g = Gnn(layers=2, max_nodes=4096, max_edges=120_000, train=False, seed=2)
+    with sph.Simulation(case) as a, sph.Simulation(case) as b:
+        st, f, h = g.rollout(a, k, intervals, dev_tol=-1.0, max_skip=4)
+        assert st["accepted"] == 0 and st["fallbacks"] == 5 and st["skipped"] == 11, st
+        assert st["plain_steps"] == k * intervals
+        b.step(k * intervals)
+        np.testing.assert_array_equal(a.pos64(), b.pos64())
+        fb, hb = b.probe(0.02)
+        assert f[-1] == fb and h[-1] == hb
+    with sph.Simulation(case) as c:
+        st, f, h = g.rollout(c, k, intervals, dev_tol=1e30, acc_tol=1e30, drift_tol=1e30, max_skip=4)
+        assert st["accepted"] == intervals - 4 and st["fallbacks"] == 0 and st["skipped"] == 0
+    g.close()
+
+
 def _developed_interval(k=10, steps=1500):
     """A 2-D dam break (1.0 x 0.5 m column, dx = 0.01 m) after `steps` plain
     steps, the state k steps later, and the case."""

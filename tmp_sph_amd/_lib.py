@@ -200,7 +200,7 @@ _PROTOS = {
     "sph_sim_set_state_device": [P, P, P, P, P, P],
     "sph_sim_instance_params": [P, I, C.POINTER(SphParams)],
     "sph_gnn_judge": [P, I, C.c_double, P, P, P, P, P, P, P, P, P, P],
-    "sph_gnn_rollout": [P, P, P, I, I, C.c_float, C.c_float, C.c_float, C.c_double, C.c_double, P, P, P, P, P],
+    "sph_gnn_rollout": [P, P, P, I, I, C.c_float, C.c_float, C.c_float, C.c_double, C.c_double, I, P, P, P, P, P],
 }
 
 EXPORTS = ["sph_last_error", *_PROTOS]

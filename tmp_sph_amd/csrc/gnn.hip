@@ -1031,9 +1031,10 @@ double judge_bias_dev(const double* J, double gT) {
 }  // namespace
 
 extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, sph_sim* judge_sim, int k, int intervals, float dev_tol,
-                               float acc_tol, float drift_tol, double front_band, double band, double* front, double* height,
-                               sph_rollout_stats* stats, sph_progress_fn progress, void* user) {
-  if (!g || !sim || !judge_sim || k <= 0 || intervals < 0 || !(front_band >= 0.0))
+                               float acc_tol, float drift_tol, double front_band, double band, int max_skip,
+                               double* front, double* height, sph_rollout_stats* stats, sph_progress_fn progress,
+                               void* user) {
+  if (!g || !sim || !judge_sim || k <= 0 || intervals < 0 || !(front_band >= 0.0) || max_skip < 0)
     return set_error(SPH_E_INVAL, "sph_gnn_rollout: bad argument");
   int K = 0, KJ = 0;
   int rc = sph_sim_num_instances(sim, &K);
@@ -1123,9 +1124,24 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, sph_sim* judge_sim, int
   if (!rc) rc = plain_rates(g->pos, vel, g->rho, prs, a0, d0);
   if (!rc && (he = push_history(vel)) != hipSuccess) rc = hip_status(he, "rollout history");
   filled = 1;
+  // back-off after rejections: `wait` intervals go to the plain step without
+  // a network attempt (and without the judge's reading); the next rejection
+  // doubles the wait up to max_skip, an acceptance clears it
+  int wait = 0, next_wait = 1;
+  bool rates_now = true;  // a0/d0 hold the plain rates of the current state
   for (int it = 0; it < intervals && !rc; ++it) {
     const float xband = (float)(f_now - front_band);
     bool plain = filled < H;
+    bool skipped = false;
+    if (!plain && wait > 0) {
+      --wait;
+      plain = skipped = true;
+      ++st.skipped;
+    }
+    if (!plain && !rates_now) {
+      if ((rc = plain_rates(g->pos, vel, g->rho, prs, a0, d0))) break;
+      rates_now = true;
+    }
     if (!plain) {
       bool overflow = false;
       rc = build_graph_features(g, P, n, &overflow);
@@ -1165,21 +1181,32 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, sph_sim* judge_sim, int
         drift[0] += rb[0];
         drift[1] += rb[1];
         st.max_drift = std::max({st.max_drift, std::fabs(drift[0]), std::fabs(drift[1])});
+        next_wait = 1;
       } else {
         plain = true;
         ++st.fallbacks;
+        if (max_skip > 0) {
+          wait = std::min(next_wait, max_skip);
+          next_wait = std::min(2 * next_wait, max_skip);
+        }
       }
     }
     if (plain) {
       rc = captured ? sph_sim_replay(sim, k) : sph_sim_step(sim, k);
       if (!rc) rc = sph_sim_get_state_device(sim, pos2, vel2, rho2, p2, nullptr);
-      if (!rc) rc = plain_rates(pos2, vel2, rho2, p2, a1, d1);
-      if (!rc) rc = read_judge(xband, pos2, vel2, rho2);  // the judge's reading of a plain interval
+      if (!rc && !skipped) {
+        // the judge's reading of a plain interval; its end-state rates are the
+        // next interval's start-state rates
+        rc = plain_rates(pos2, vel2, rho2, p2, a1, d1);
+        if (!rc) rc = read_judge(xband, pos2, vel2, rho2);
+        if (!rc) {
+          st.plain_max_dev = std::max(st.plain_max_dev, J[1] > 0 ? J[2] : 0.0);
+          st.plain_max_band_dev = std::max(st.plain_max_band_dev, band_dev());
+          st.plain_max_bias_dev = std::max(st.plain_max_bias_dev, bias_dev());
+        }
+      }
       if (rc) break;
       st.plain_steps += k;
-      st.plain_max_dev = std::max(st.plain_max_dev, J[1] > 0 ? J[2] : 0.0);
-      st.plain_max_band_dev = std::max(st.plain_max_band_dev, band_dev());
-      st.plain_max_bias_dev = std::max(st.plain_max_bias_dev, bias_dev());
     }
     // the new state (accepted prediction or plain result) becomes the current one
     he = hipMemcpyAsync(g->pos, pos2, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, s);
@@ -1189,6 +1216,7 @@ extern "C" int sph_gnn_rollout(sph_gnn* g, sph_sim* sim, sph_sim* judge_sim, int
     if (he != hipSuccess) { rc = hip_status(he, "rollout state"); break; }
     std::swap(a0, a1);
     std::swap(d0, d1);
+    rates_now = !skipped;  // a skipped interval computed no end-state rates
     if ((he = push_history(vel)) != hipSuccess) { rc = hip_status(he, "rollout history"); break; }
     filled = std::min(filled + 1, H);
     rc = sph_sim_probe(sim, band, &f_now, &h_now);

@@ -39,7 +39,8 @@ class RolloutStats(C.Structure):
                 ("rejected_penetration", C.c_int), ("max_penetration", C.c_double),
                 ("rejected_density", C.c_int), ("rejected_graph", C.c_int), ("rejected_velocity", C.c_int),
                 ("max_band_dev", C.c_double), ("max_bias_dev", C.c_double), ("plain_max_dev", C.c_double),
-                ("plain_max_band_dev", C.c_double), ("plain_max_bias_dev", C.c_double), ("max_drift", C.c_double)]
+                ("plain_max_band_dev", C.c_double), ("plain_max_bias_dev", C.c_double), ("max_drift", C.c_double),
+                ("skipped", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -169,14 +170,15 @@ class Gnn:
         call("sph_gnn_adam", self._h, C.c_float(lr), C.c_float(clip))
 
     def rollout(self, sim: Simulation, k: int, intervals: int, dev_tol: float = 0.01, acc_tol: float = 0.5,
-                drift_tol: float = 20.0, front_band: float = 0.02, band: float = 0.02,
+                drift_tol: float = 20.0, front_band: float = 0.02, band: float = 0.02, max_skip: int = 0,
                 judge: Simulation | None = None, progress=None):
         """Advances `sim` by `intervals` x k steps with the surrogate, judged
         by the plain method's rates at both ends of every interval (evaluated
         in `judge`, a second context of the same case, made from the current
-        state when not given) and backed up by the plain step; returns
-        (stats, front, height).  progress(done, accepted) is called every 1000
-        intervals."""
+        state when not given) and backed up by the plain step; after a
+        rejection the next 1, 2, 4, ... (at most max_skip; 0: none) intervals
+        are plain without a network attempt.  Returns (stats, front, height).
+        progress(done, accepted) is called every 1000 intervals."""
         f = np.zeros(max(intervals, 1))
         h = np.zeros(max(intervals, 1))
         st = RolloutStats()
@@ -188,7 +190,8 @@ class Gnn:
         try:
             cb = _PROGRESS(lambda done, acc, _u: progress(done, acc)) if progress is not None else None
             call("sph_gnn_rollout", self._h, sim._h, judge._h, int(k), int(intervals), C.c_float(dev_tol),
-                 C.c_float(acc_tol), C.c_float(drift_tol), C.c_double(front_band), C.c_double(band), _hp(f), _hp(h), C.byref(st),
+                 C.c_float(acc_tol), C.c_float(drift_tol), C.c_double(front_band), C.c_double(band), int(max_skip),
+                 _hp(f), _hp(h), C.byref(st),
                  C.cast(cb, C.c_void_p) if cb is not None else None, None)
         finally:
             if own:
@@ -455,7 +458,7 @@ def forward_flops(layers: int, n: int, edges: int) -> float:
 
 def learned_vs_plain(g: "Gnn", case: Case, k: int, t_end: float = 2.0, dev_tol: float = 0.01,
                      acc_tol: float = 0.5, drift_tol: float = 20.0, front_band: float = 0.02, band: float = 0.02,
-                     tuning: dict | None = None, progress=None) -> dict:
+                     max_skip: int = 0, tuning: dict | None = None, progress=None) -> dict:
     """Plain GPU solver against the surrogate rollout on one case, both to
     t_end.  The plain run replays its captured step and probes the surge front
     every k steps on the device (one host wait at the end), timed as a whole.
@@ -479,14 +482,15 @@ def learned_vs_plain(g: "Gnn", case: Case, k: int, t_end: float = 2.0, dev_tol: 
         if tuning:
             sim.set_tuning(**tuning)
         st, front_l, _ = g.rollout(sim, k, intervals, dev_tol=dev_tol, acc_tol=acc_tol, drift_tol=drift_tol,
-                                   front_band=front_band,
-                                   band=band, judge=judge, progress=progress)
+                                   front_band=front_band, band=band, max_skip=max_skip, judge=judge,
+                                   progress=progress)
     rel = np.abs(front_l - front_p) / np.maximum(front_p, 1e-9)
     worst = int(np.argmax(rel))
     return {"particles": int(case.n), "fluid": int(case.n_fluid), "dt": dt, "k": k, "intervals": intervals,
             "t_end": intervals * k * dt, "plain_seconds": plain_s, "learned_seconds": st["seconds"],
             "speedup": plain_s / st["seconds"], "tuning": tuning_used,
             "dev_tol": dev_tol, "acc_tol": acc_tol, "drift_tol": drift_tol, "front_band": front_band,
+            "max_skip": max_skip,
             "front_rel_error_max": float(rel.max()), "front_rel_error_max_at_t": float((worst + 1) * k * dt),
             "front_rel_error_final": float(rel[-1]),
             "front_plain_final": float(front_p[-1]), "front_learned_final": float(front_l[-1]),
