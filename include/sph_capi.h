@@ -213,8 +213,8 @@ int sph_PC_correction(float* sortedPos, float* sortedVel, float* sortedDensity,
 typedef struct sph_sim sph_sim;
 
 /* Launch-shape knobs of the fast path, filled with defaults by sph_sim_create
-   (row staging with neighbour data, unroll 2, prefetch, runs of 32 tiles per
-   XCD) and chosen per
+   (row staging with neighbour data, prefetch; 3-D: unroll 2, runs of 32
+   tiles per XCD; 2-D: unroll 4, one run of tiles per XCD) and chosen per
    workload by sph_sim_autotune.  None changes the results. */
 typedef struct sph_tuning {
     int nbr_block;       /* threads per block of the neighbour-loop kernels: 64, 128, 192, 256 */
