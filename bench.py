@@ -370,6 +370,65 @@ def learned_eval(weights: Path, t_end: float = 2.0, tol: dict | None = None,
     }
 
 
+def reference_surface_bench(args) -> int:
+    """The step through the drop-in reference call surface on the C3 case:
+    per half-step calcHash, sortParticles, reorder (v0 permutation table),
+    SPH_NS_simpleversion and PC_prediction / PC_correction on caller-owned
+    device arrays in the reference's AoS layouts, in the order of
+    opt_src_v0/sph/sph_run.cu:269-309 -- what a caller gets by rebinding the
+    reference's own calls, next to the default line's fast-path context."""
+    import numpy as np
+    from tmp_sph_amd import cases, sph
+    desc, cfg = cases.named("c3")
+    case = sph.make_case(cfg)
+    prm = case.params
+    n = case.n
+    sph.set_params(prm)
+    D, E = sph.DeviceArray.from_host, sph.DeviceArray
+    S = [{"pos": D(case.pos), "vel": D(case.vel), "rho": D(case.rho), "p": D(case.p), "type": D(case.type)},
+         {"pos": E((n, 3), np.float32), "vel": E((n, 3), np.float32), "rho": E((n,), np.float32),
+          "p": E((n,), np.float32), "type": E((n,), np.int32)}]
+    tabs = [D(np.zeros(n, np.int32)), D(np.zeros(n, np.int32))]
+    h, idx = E((n,), np.int32), E((n,), np.int32)
+    cs, ce = E((prm.numCells,), np.int32), E((prm.numCells,), np.int32)
+    drho, acc, dofv = E((n,), np.float32), E((n, 3), np.float32), E((n,), np.float32)
+    tpos, tvel, trho = E((n, 3), np.float32), E((n, 3), np.float32), E((n,), np.float32)
+
+    def half(a, b, pred):
+        A, B = S[a], S[b]
+        sph.calcHash(A["pos"], h, idx, n)
+        sph.sortParticles(h, idx, n)
+        sph.reorder(A["pos"], A["vel"], A["rho"], A["p"], A["type"], None, B["pos"], B["vel"], B["rho"], B["p"],
+                    B["type"], None, h, idx, cs, ce, n, tabs[a], tabs[b])
+        sph.SPH_NS_simpleversion(B["pos"], B["vel"], B["rho"], B["p"], B["type"], drho, acc, cs, ce, n, h, 0, dofv)
+        f = sph.PC_prediction if pred else sph.PC_correction
+        f(B["pos"], B["vel"], B["rho"], B["p"], drho, acc, h, idx, tpos, tvel, trho, n, B["type"])
+
+    def step():
+        half(0, 1, True)
+        half(1, 0, False)
+
+    steps = args.steps if args.steps > 0 else 200
+    for _ in range(args.warmup):
+        step()
+    sph.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    sph.synchronize()
+    el = time.perf_counter() - t0
+    print(f"# reference call surface: {steps} steps, {1e3 * el / steps:.3f} ms/step", file=sys.stderr, flush=True)
+    out = {"metric": "particle-steps/s through the reference call surface (drop-in)",
+           "value": case.n_fluid * steps / el, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * el / steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic dam-break lattice released from rest",
+           "config": {"workload": desc + ", through calcHash / sortParticles / reorder / SPH_NS_simpleversion / "
+                                         "PC_prediction / PC_correction on AoS device arrays",
+                      "particles_total": n, "particles_fluid": case.n_fluid}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def learned_bench(args) -> int:
     """Trains the surrogate and evaluates it against the plain GPU solver.
 
@@ -449,7 +508,7 @@ def main() -> int:
     ap.add_argument("--eager", action="store_true",
                     help="launch the timed steps one kernel at a time instead of replaying the captured graph "
                          "(for kernel-trace profiling: rocprofv3's tracer crashes on long graph replays)")
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned", "surrogate", "cpu-plan"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "learned", "surrogate", "cpu-plan", "c3-ref"])
     ap.add_argument("--train-seconds", type=float, default=900.0, help="learned: stop training after this long")
     ap.add_argument("--train-iters", type=int, default=1500, help="learned: Adam iterations (minibatches of 4)")
     ap.add_argument("--train-members", type=int, default=8, help="learned: sweep members (dx = 0.002 m) recorded")
@@ -489,6 +548,8 @@ def main() -> int:
         return surrogate_bench(args)
     if args.config == "cpu-plan":
         return cpu_plan(args)
+    if args.config == "c3-ref":
+        return reference_surface_bench(args)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): there is no multi-GPU path; "
               "the slab decomposition it would use is described in the design notes", file=sys.stderr)

@@ -188,12 +188,24 @@ int sph_recover(float* pos, float* vel, float* density, float* pressure, int num
    (wall pressure extrapolation, velocity divergence dofv) then kernel B (drho/dt with
    delta-SPH diffusion, dv/dt with pressure, artificial viscosity, acoustic damper and
    gravity).  Wall pressure/density are written in place.  rhop_sum and w_sum are
-   accepted and unused, as in the reference from v0 on. */
+   accepted and unused, as in the reference from v0 on.  By default (ns mode 0)
+   the kernels are the fast path's row-staged loops run on the caller's arrays
+   (packed into library-owned scratch, allocated on first use and grown with n;
+   results within 1e-5 of the reference order, rates for every particle as the
+   reference computes them); ns mode 1 runs kernels in the reference's FP32
+   operation order, which reproduce the reference bit for bit with the linear
+   EOS.  The first call after a parameter change uploads the parameters and
+   synchronises the stream. */
 int sph_SPH_NS_simpleversion(const float* sortedPos, const float* sortedVel,
                              float* sortedDensity, float* sortedPressure, const int* sortedType,
                              float* densitydt, float* Veldt, const int* cellStart,
                              const int* cellEnd, int numParticles, const int* particleHash,
                              int timestep, float* dofv, float* rhop_sum, float* w_sum);
+/* Selects the kernels behind sph_SPH_NS_simpleversion (no counterpart in the
+   reference: it has one implementation): 0 (default) the row-staged fast
+   loops, 1 the reference's FP32 operation order.  Process-wide. */
+int sph_set_ns_mode(int mode);
+int sph_get_ns_mode(int* mode);
 /* PC_prediction (src/header/sph.cuh:26; opt_src_v0/sph/sph_run.cu:388-435): fluid only;
    saves (x, v, rho) into the tmp arrays at the sorted slot and advances half a step. */
 int sph_PC_prediction(float* sortedPos, float* sortedVel, float* sortedDensity,

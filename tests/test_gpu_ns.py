@@ -23,7 +23,7 @@ def _sorted_state(orc, case):
     return st
 
 
-def _gpu_ns(sph, prm, st):
+def _gpu_ns(sph, prm, st, mode=0):
     n = st["pos"].shape[0]
     D = sph.DeviceArray.from_host
     d = {k: D(st[k]) for k in ("pos", "vel", "rho", "p", "type", "cellStart", "cellEnd", "hash")}
@@ -31,13 +31,42 @@ def _gpu_ns(sph, prm, st):
     acc = sph.DeviceArray((n, 3), np.float32)
     dofv = sph.DeviceArray((n,), np.float32)
     sph.set_params(prm)
-    sph.SPH_NS_simpleversion(d["pos"], d["vel"], d["rho"], d["p"], d["type"], drho, acc, d["cellStart"],
-                             d["cellEnd"], n, d["hash"], 0, dofv)
+    sph.set_ns_mode(mode)
+    try:
+        sph.SPH_NS_simpleversion(d["pos"], d["vel"], d["rho"], d["p"], d["type"], drho, acc, d["cellStart"],
+                                 d["cellEnd"], n, d["hash"], 0, dofv)
+    finally:
+        sph.set_ns_mode(0)
     return {"drho": drho.numpy(), "acc": acc.numpy(), "dofv": dofv.numpy(), "rho": d["rho"].numpy(),
             "p": d["p"].numpy()}
 
 
-def _check_ns(prm, got, want, fluid, wall):
+def _pressure_sensitivity(prm, st, rho):
+    """Per particle, sum over its in-support neighbours j of
+    m |F(r_ij)| |r_ij| / (rho_i rho_j): the acceleration a pressure change of 1
+    on every neighbour can cause through the pressure term of kernel B
+    (F = -5 adh (1 - q/2)^3 / h^2, the kernel gradient factor)."""
+    from scipy.spatial import cKDTree
+    pos = st["pos"].astype(np.float64)
+    pairs = cKDTree(pos).query_pairs(float(np.sqrt(prm.rr_cut)) * 1.0001, output_type="ndarray")
+    d = pos[pairs[:, 0]] - pos[pairs[:, 1]]
+    r = np.sqrt((d * d).sum(1))
+    q = r / prm.h
+    f = np.where(q <= 2.0, 5.0 * prm.adh * (1.0 - q / 2.0) ** 3 / (prm.h * prm.h), 0.0)
+    term = prm.particleMass * f * r / (np.abs(rho[pairs[:, 0]]) * np.abs(rho[pairs[:, 1]]))
+    out = np.zeros(pos.shape[0])
+    np.add.at(out, pairs[:, 0], term)
+    np.add.at(out, pairs[:, 1], term)
+    return out
+
+
+def _check_ns(prm, got, want, fluid, wall, st=None):
+    """Rates within TOL of the oracle, magnitude-normalised.  With `st` (the
+    fast loops, whose wall pressures carry their own 1e-5 x rho0 g h
+    contract) a wall target's acceleration is also allowed what that
+    pressure tolerance on its neighbours can move it: near the free surface a
+    wall's acceleration is the tiny residual of a barely wet neighbour's
+    extrapolated pressure, itself a small difference of large terms."""
     assert rel_err(got["drho"][fluid], want["drho"][fluid], want["mag_drho"][fluid], want["sens_drho"][fluid]) < TOL
     assert rel_err(got["acc"][fluid], want["acc"][fluid], want["mag_acc"][fluid]) < TOL
     assert rel_err(got["dofv"], want["dofv"], want["mag_dofv"]) < TOL
@@ -46,11 +75,19 @@ def _check_ns(prm, got, want, fluid, wall):
     assert rel_err(got["rho"][wall], want["rho"][wall], prm.restDensity) < TOL
     # walls: the reference computes kernel B for them too
     assert rel_err(got["drho"][wall], want["drho"][wall], want["mag_drho"][wall], want["sens_drho"][wall]) < TOL
-    assert rel_err(got["acc"][wall], want["acc"][wall], want["mag_acc"][wall]) < TOL
+    scale = want["mag_acc"][wall]
+    if st is not None:
+        sens = pscale * _pressure_sensitivity(prm, st, want["rho"])[wall]
+        scale = np.maximum(scale, sens[:, None])
+    assert rel_err(got["acc"][wall], want["acc"][wall], scale) < TOL
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("which", ["2d_tait", "2d_linear", "3d_tait", "ref"])
-def test_sph_ns_matches_oracle(gpu, orc, which):
+def test_sph_ns_matches_oracle(gpu, orc, which, mode):
+    """SPH_NS_simpleversion on the fast loops (mode 0, the default) and in the
+    reference's operation order (mode 1) against the oracle, every particle
+    (walls' rates included); mode 1 is bit-exact with the linear EOS."""
     from tmp_sph_amd.sph import dambreak_cfg, make_case
     if which == "2d_tait":
         case = advanced(orc, case2d(0.02), 150, mode=0)
@@ -69,11 +106,11 @@ def test_sph_ns_matches_oracle(gpu, orc, which):
     st = _sorted_state(orc, case)
     want = orc.SPH_NS(prm, st["pos"], st["vel"], st["rho"], st["p"], st["type"], st["cellStart"], st["cellEnd"],
                       st["hash"])
-    got = _gpu_ns(gpu, prm, st)
+    got = _gpu_ns(gpu, prm, st, mode)
     fluid = st["type"] == 1
-    _check_ns(prm, got, want, fluid, ~fluid)
+    _check_ns(prm, got, want, fluid, ~fluid, st if mode == 0 else None)
     assert np.abs(want["acc"][fluid]).max() > 0  # a non-trivial flow state
-    if prm.eos == 0:
+    if prm.eos == 0 and mode == 1:
         # linear EOS: same operations in the same order with no contraction,
         # so the reference surface reproduces the oracle bit for bit
         for k in ("drho", "acc", "dofv", "p", "rho"):
@@ -124,14 +161,18 @@ def test_pc_bit_exact(gpu, orc):
         np.testing.assert_array_equal(e[k].numpy(), want2[k], err_msg=f"corrector {k}")
 
 
-def test_reference_order_step(gpu, orc):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_reference_order_step(gpu, orc, mode):
     """One step through the reference call surface in the order of
-    opt_src_v0/sph/sph_run.cu:269-309, against the oracle's FP32 step."""
+    opt_src_v0/sph/sph_run.cu:269-309, against the oracle's FP32 step, with
+    SPH_NS_simpleversion on the fast loops (mode 0) and in reference order
+    (mode 1)."""
     sph = gpu
     case = advanced(orc, case2d(0.02), 100, mode=0)
     prm = case.params
     n = case.n
     sph.set_params(prm)
+    sph.set_ns_mode(mode)
     D, E = sph.DeviceArray.from_host, sph.DeviceArray
     A = {"pos": D(case.pos), "vel": D(case.vel), "rho": D(case.rho), "p": D(case.p), "type": D(case.type)}
     B = {"pos": E((n, 3), np.float32), "vel": E((n, 3), np.float32), "rho": E((n,), np.float32),
@@ -155,6 +196,7 @@ def test_reference_order_step(gpu, orc):
                 A["type"], None, h, idx, cs, ce, n, tr, tl)
     sph.SPH_NS_simpleversion(A["pos"], A["vel"], A["rho"], A["p"], A["type"], drho, acc, cs, ce, n, h, 0, dofv)
     sph.PC_correction(A["pos"], A["vel"], A["rho"], A["p"], drho, acc, h, idx, tpos, tvel, trho, n, A["type"])
+    sph.set_ns_mode(0)
     ids = np.arange(n, dtype=np.int32) + tr.numpy()  # i + table[i] = original id (second reorder wrote table_r)
     assert np.array_equal(np.sort(ids), np.arange(n))
     got = {k: np.empty_like(v.numpy()) for k, v in A.items()}

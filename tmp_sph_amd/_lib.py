@@ -144,6 +144,8 @@ _PROTOS = {
     "sph_reorder": [P] * 16 + [I, P, P],
     "sph_recover": [P] * 4 + [I] + [P] * 9,
     "sph_SPH_NS_simpleversion": [P] * 9 + [I, P, I, P, P, P],
+    "sph_set_ns_mode": [I],
+    "sph_get_ns_mode": [P],
     "sph_PC_prediction": [P] * 11 + [I, P],
     "sph_PC_correction": [P] * 11 + [I, P],
     "sph_sim_create": [C.POINTER(SphParams), I, P, P, P, P, P, C.POINTER(C.c_void_p)],

@@ -29,6 +29,26 @@ struct SortPool {
   unsigned* hist = nullptr;
   unsigned* totals = nullptr;
 } g_pool;
+
+// SPH_NS_simpleversion: 0 runs the fast path's row-staged loops on the
+// caller's arrays (packed into NsPool), 1 the reference-order kernels.
+int g_ns_mode = 0;
+
+// Scratch of SPH_NS_simpleversion on the row-staged loops, grown on demand:
+// the sorted state in the fast layout, the per-cell fluid flags, dofv and the
+// rates, and a device copy of the parameters.
+struct NsPool {
+  int cap = 0;
+  int cells = 0;
+  FastSet st{};
+  float* DV = nullptr;
+  float4* RATE = nullptr;
+  int* cellFluid = nullptr;
+  sph_params* dP = nullptr;
+  int* dcoff = nullptr;
+  sph_params uploaded{};
+  bool have_params = false;
+} g_ns;
 }  // namespace
 
 int set_error(int code, const char* msg) {
@@ -105,6 +125,79 @@ int ensure_pool(int n) {
   SPH_HIP(hipMalloc(&g_pool.hist, (size_t)radix_hist_words((int)nn) * sizeof(unsigned)), "sort scratch");
   SPH_HIP(hipMalloc(&g_pool.totals, 256 * sizeof(unsigned)), "sort scratch");
   g_pool.cap = (int)nn;
+  return SPH_OK;
+}
+
+// Grows the NS scratch to n particles and `cells` cells.
+int ensure_ns_pool(int n, int cells) {
+  NsPool& q = g_ns;
+  if (!q.dP) {
+    SPH_HIP(hipMalloc(&q.dP, sizeof(sph_params)), "SPH_NS scratch");
+    SPH_HIP(hipMalloc(&q.dcoff, sizeof(int)), "SPH_NS scratch");
+    SPH_HIP(hipMemset(q.dcoff, 0, sizeof(int)), "SPH_NS scratch");
+  }
+  if (q.cap < n || !q.st.PR) {
+    hipFree(q.st.PR);
+    hipFree(q.st.VP);
+    hipFree(q.st.TY);
+    hipFree(q.DV);
+    hipFree(q.RATE);
+    q.st = FastSet{};
+    q.DV = nullptr;
+    q.RATE = nullptr;
+    q.cap = 0;
+    const size_t nn = (size_t)(n > 1 ? n : 1);
+    SPH_HIP(hipMalloc(&q.st.PR, (nn + CAND_PAD) * sizeof(float4)), "SPH_NS scratch");
+    SPH_HIP(hipMalloc(&q.st.VP, (nn + CAND_PAD) * sizeof(float4)), "SPH_NS scratch");
+    SPH_HIP(hipMalloc(&q.st.TY, nn * sizeof(int)), "SPH_NS scratch");
+    SPH_HIP(hipMalloc(&q.DV, (nn + CAND_PAD) * sizeof(float)), "SPH_NS scratch");
+    SPH_HIP(hipMalloc(&q.RATE, nn * sizeof(float4)), "SPH_NS scratch");
+    q.cap = (int)nn;
+  }
+  if (q.cells < cells || !q.cellFluid) {
+    hipFree(q.cellFluid);
+    q.cellFluid = nullptr;
+    q.cells = 0;
+    SPH_HIP(hipMalloc(&q.cellFluid, (size_t)(cells > 1 ? cells : 1) * sizeof(int)), "SPH_NS scratch");
+    q.cells = cells;
+  }
+  return SPH_OK;
+}
+
+// Kernels A and B of the row-staged fast path on the caller's sorted AoS
+// state: pack, A (dofv, walls' pressure and density), B (rates of every
+// particle, as the reference computes them), unpack.
+int ns_fast(const sph_params& P, hipStream_t s, const float* sPos, const float* sVel, float* sRho, float* sP,
+            const int* sType, float* drho, float* acc, const int* cellStart, const int* cellEnd, int n,
+            const int* hash, float* dofv) {
+  int rc = ensure_ns_pool(n, P.numCells);
+  if (rc) return rc;
+  NsPool& q = g_ns;
+  if (!q.have_params || std::memcmp(&q.uploaded, &P, sizeof(P)) != 0) {
+    SPH_HIP(hipMemcpyAsync(q.dP, &P, sizeof(P), hipMemcpyHostToDevice, s), "SPH_NS params");
+    SPH_HIP(hipStreamSynchronize(s), "SPH_NS params");  // the host copy is a local
+    q.uploaded = P;
+    q.have_params = true;
+  }
+  InstTable T{q.dP, q.dcoff, 1, P.dim, P.numCells};
+  FastWork w{};
+  w.keys = const_cast<int*>(hash);  // read only
+  w.cellStart = const_cast<int*>(cellStart);
+  w.cellEnd = const_cast<int*>(cellEnd);
+  w.cellFluid = q.cellFluid;
+  w.DV = q.DV;
+  w.RATE = q.RATE;
+  sph_tuning tu{};
+  tu.nbr_block = 64;
+  tu.nbr_unroll = P.dim == 3 ? 2 : 4;  // the contexts' default shapes
+  tu.xcd_remap = P.dim == 3 ? 32 : 1;
+  tu.nbr_lds = 3;
+  tu.nbr_prefetch = 1;
+  tu.cell_div = 1;
+  SPH_HIP(launch_ref_pack(sPos, sVel, sRho, sP, sType, hash, q.st, q.cellFluid, P.numCells, n, s), "SPH_NS pack");
+  SPH_HIP(launch_nsA_fast(T, q.st, w, n, tu, s), "SPH_NS kernel A");
+  SPH_HIP(launch_nsB_fast(T, q.st, w, n, tu, s, true), "SPH_NS kernel B");
+  SPH_HIP(launch_ref_unpack(q.st, w, sType, drho, acc, dofv, sRho, sP, n, s), "SPH_NS unpack");
   return SPH_OK;
 }
 
@@ -345,9 +438,32 @@ extern "C" int sph_SPH_NS_simpleversion(const float* sPos, const float* sVel, fl
   if (n < 0 || (n > 0 && (!sPos || !sVel || !sRho || !sP || !sType || !densitydt || !Veldt || !cellStart ||
                           !cellEnd || !hash || !dofv)))
     return set_error(SPH_E_INVAL, "sph_SPH_NS_simpleversion: bad argument");
+  int mode;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    mode = g_ns_mode;
+  }
+  if (mode == 0) {
+    std::lock_guard<std::mutex> lk(g_mu);  // one user of the shared scratch at a time
+    return ns_fast(P, s, sPos, sVel, sRho, sP, sType, densitydt, Veldt, cellStart, cellEnd, n, hash, dofv);
+  }
   SPH_HIP(launch_ns_aos(P, sPos, sVel, sRho, sP, sType, densitydt, Veldt, cellStart, cellEnd, hash, dofv,
                         n, s),
           "SPH_NS_simpleversion");
+  return SPH_OK;
+}
+
+extern "C" int sph_set_ns_mode(int mode) {
+  if (mode != 0 && mode != 1) return set_error(SPH_E_INVAL, "sph_set_ns_mode: mode must be 0 or 1");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_ns_mode = mode;
+  return SPH_OK;
+}
+
+extern "C" int sph_get_ns_mode(int* mode) {
+  if (!mode) return set_error(SPH_E_INVAL, "sph_get_ns_mode: null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  *mode = g_ns_mode;
   return SPH_OK;
 }
 

@@ -100,6 +100,16 @@ struct InstTable {
   int total_cells;
 };
 
+// The reference surface's SPH_NS_simpleversion on the fast loops: the
+// caller's sorted AoS arrays packed into st (PR/VP/TY, n + CAND_PAD entries)
+// with the per-cell fluid flags (`cells` entries, cleared here), and the
+// rates, dofv and the walls' pressure and density unpacked back.
+hipError_t launch_ref_pack(const float* sPos, const float* sVel, const float* sRho, const float* sP,
+                           const int* sType, const int* hash, const FastSet& st, int* cellFluid, int cells, int n,
+                           hipStream_t s);
+hipError_t launch_ref_unpack(const FastSet& st, const FastWork& w, const int* sType, float* drho, float* acc,
+                             float* dofv, float* sRho, float* sP, int n, hipStream_t s);
+
 // zero (may be null): one word set to 0 by the launch (the dense table's run counter).
 hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s,
                             int* zero = nullptr);
@@ -120,8 +130,10 @@ hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork
 // iterate the lists.  Same pairs in the same order as the scan kernels.
 hipError_t launch_nbrlist_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                                hipStream_t s);
+// walls: also the rates of non-fluid targets (the reference computes them;
+// the integrator ignores them); otherwise their RATE is zero.
 hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
-                           hipStream_t s);
+                           hipStream_t s, bool walls = false);
 // Fine-row loops (tuning nbr_lds 4): any stencil reach, per-lane trimmed
 // candidate ranges, one cell-table read per wave and stencil row.
 hipError_t launch_nsA_fine(const InstTable& T, const InstTable& Tc, const FastSet& st, const FastWork& w, int n,

@@ -623,7 +623,8 @@ __global__ __launch_bounds__(STAGED ? 64 : 256, STAGED >= 2 ? 6 : 1) void nsB_ke
                                                                 const int* __restrict__ keys,
                                                                 const int* __restrict__ cs,
                                                                 const int* __restrict__ ce,
-                                                                float4* __restrict__ RATE, int n, int remap) {
+                                                                float4* __restrict__ RATE, int n, int remap,
+                                                                int walls) {
   using C = Cfg<DIM, U, STAGED, COMPACT>;
   __shared__ float4 tile[C::TILE];
   __shared__ float4 tilev[C::TILEX];
@@ -635,13 +636,14 @@ __global__ __launch_bounds__(STAGED ? 64 : 256, STAGED >= 2 ? 6 : 1) void nsB_ke
   if (!STAGED && !valid) return;
   const int ic = valid ? i : n - 1;
   const float4 pi = PR[ic];
-  const bool fluid = pi.w > 0.0f;
-  if (valid && !fluid) RATE[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (!STAGED && !fluid) return;
+  // walls != 0: non-fluid targets get their rates too (the reference surface)
+  const bool target = pi.w > 0.0f || walls != 0;
+  if (valid && !target) RATE[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (!STAGED && !target) return;
   const float4 vi = VP[ic];
   const float dvi = DV[ic];
   const int key = keys[ic];
-  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid && fluid, PR, VP, DV, cs, ce,
+  with_visit<DIM, U, STAGED, COMPACT, PF>(T, inst_of(TY[ic]), key, pi, valid && target, PR, VP, DV, cs, ce,
                                           nullptr, tile, tilev, tiled, lists, slots,
                                       [&](const sph_params& P, auto visit, bool store) {
                                         nsB_target<DIM>(P, i, pi, vi, dvi, visit, VP, DV, RATE, store);
@@ -762,17 +764,17 @@ hipError_t nsA_v(const InstTable& T, const FastSet& st, const FastWork& w, int n
 
 template <int U, int STAGED, int COMPACT, bool PF>
 hipError_t nsB_v(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap,
-                 hipStream_t s) {
+                 int walls, hipStream_t s) {
   const int b = STAGED ? 64 : block;
   const dim3 g(blocks_for(n, b)), bl(b);
   if (T.dim == 3)
     hipLaunchKernelGGL((nsB_kernel<3, U, STAGED, COMPACT, PF>), g, bl, 0, s, T, (const float4*)st.PR,
                        (const float4*)st.VP, (const int*)st.TY, (const float*)w.DV, (const int*)w.keys,
-                       (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap);
+                       (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap, walls);
   else
     hipLaunchKernelGGL((nsB_kernel<2, U, STAGED, COMPACT, PF>), g, bl, 0, s, T, (const float4*)st.PR,
                        (const float4*)st.VP, (const int*)st.TY, (const float*)w.DV, (const int*)w.keys,
-                       (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap);
+                       (const int*)w.cellStart, (const int*)w.cellEnd, w.RATE, n, remap, walls);
   return hipGetLastError();
 }
 
@@ -800,23 +802,23 @@ hipError_t nsA_u(const InstTable& T, const FastSet& st, const FastWork& w, int n
 
 template <int U>
 hipError_t nsB_u(const InstTable& T, const FastSet& st, const FastWork& w, int n, int block, int remap, int mode,
-                 hipStream_t s) {
+                 int walls, hipStream_t s) {
   switch (mode) {
-    case 1: return nsB_v<U, true, false, false>(T, st, w, n, block, remap, s);
-    case 2: return nsB_v<U, false, true, false>(T, st, w, n, block, remap, s);
-    case 3: return nsB_v<U, true, true, false>(T, st, w, n, block, remap, s);
-    case 4: return nsB_v<U, false, false, true>(T, st, w, n, block, remap, s);
-    case 5: return nsB_v<U, true, false, true>(T, st, w, n, block, remap, s);
-    case 6: return nsB_v<U, false, true, true>(T, st, w, n, block, remap, s);
-    case 7: return nsB_v<U, true, true, true>(T, st, w, n, block, remap, s);
-    case 8: case 10: return nsB_v<U, 2, false, false>(T, st, w, n, block, remap, s);
-    case 12: case 14: return nsB_v<U, 2, false, true>(T, st, w, n, block, remap, s);
-    case 16: return nsB_v<U, 0, 2, false>(T, st, w, n, block, remap, s);
-    case 24: return nsB_v<U, 2, 2, false>(T, st, w, n, block, remap, s);
-    case 32: return nsB_v<U, 3, 0, false>(T, st, w, n, block, remap, s);
-    case 36: return nsB_v<U, 3, 0, true>(T, st, w, n, block, remap, s);
-    case 48: return nsB_v<U, 3, 2, false>(T, st, w, n, block, remap, s);
-    default: return nsB_v<U, false, false, false>(T, st, w, n, block, remap, s);
+    case 1: return nsB_v<U, true, false, false>(T, st, w, n, block, remap, walls, s);
+    case 2: return nsB_v<U, false, true, false>(T, st, w, n, block, remap, walls, s);
+    case 3: return nsB_v<U, true, true, false>(T, st, w, n, block, remap, walls, s);
+    case 4: return nsB_v<U, false, false, true>(T, st, w, n, block, remap, walls, s);
+    case 5: return nsB_v<U, true, false, true>(T, st, w, n, block, remap, walls, s);
+    case 6: return nsB_v<U, false, true, true>(T, st, w, n, block, remap, walls, s);
+    case 7: return nsB_v<U, true, true, true>(T, st, w, n, block, remap, walls, s);
+    case 8: case 10: return nsB_v<U, 2, false, false>(T, st, w, n, block, remap, walls, s);
+    case 12: case 14: return nsB_v<U, 2, false, true>(T, st, w, n, block, remap, walls, s);
+    case 16: return nsB_v<U, 0, 2, false>(T, st, w, n, block, remap, walls, s);
+    case 24: return nsB_v<U, 2, 2, false>(T, st, w, n, block, remap, walls, s);
+    case 32: return nsB_v<U, 3, 0, false>(T, st, w, n, block, remap, walls, s);
+    case 36: return nsB_v<U, 3, 0, true>(T, st, w, n, block, remap, walls, s);
+    case 48: return nsB_v<U, 3, 2, false>(T, st, w, n, block, remap, walls, s);
+    default: return nsB_v<U, false, false, false>(T, st, w, n, block, remap, walls, s);
   }
 }
 
@@ -905,15 +907,18 @@ hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork
 }
 
 hipError_t launch_nsB_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
-                           hipStream_t s) {
+                           hipStream_t s, bool walls) {
   if (n <= 0) return hipSuccess;
-  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap, md = mode_of(tu);
-  if (tu.nbr_list) return nsB_list(T, st, w, n, b, rm, s);
+  const int b = norm_block(tu.nbr_block), rm = tu.xcd_remap, md = mode_of(tu), wl = walls ? 1 : 0;
+  if (tu.nbr_list) {
+    if (walls) return hipErrorInvalidValue;  // the list kernels serve fluid targets only
+    return nsB_list(T, st, w, n, b, rm, s);
+  }
   switch (tu.nbr_unroll) {
-    case 1: return nsB_u<1>(T, st, w, n, b, rm, md, s);
-    case 2: return nsB_u<2>(T, st, w, n, b, rm, md, s);
-    case 8: return nsB_u<8>(T, st, w, n, b, rm, md, s);
-    default: return nsB_u<4>(T, st, w, n, b, rm, md, s);
+    case 1: return nsB_u<1>(T, st, w, n, b, rm, md, wl, s);
+    case 2: return nsB_u<2>(T, st, w, n, b, rm, md, wl, s);
+    case 8: return nsB_u<8>(T, st, w, n, b, rm, md, wl, s);
+    default: return nsB_u<4>(T, st, w, n, b, rm, md, wl, s);
   }
 }
 

@@ -101,14 +101,18 @@ struct ExtraB {
   __device__ __forceinline__ NbrB from(const float4* tv, const float* td, int t) const { return NbrB{tv[t], td[t]}; }
 };
 
-// Kernel B for fluid target i (all neighbours).
+// Kernel B for target i (all neighbours): fluid targets always; wall targets
+// when the caller wants their rates too (the reference computes them; its
+// integrator ignores them).  A wall target's density is |PR.w| and it has no
+// artificial-viscosity or damper pairs (fluid-fluid only).
 template <int DIM, class Visit>
 __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const float4& pi, const float4& vi, float dvi,
                                            Visit visit, const float4* __restrict__ VP,
                                            const float* __restrict__ DV, float4* __restrict__ RATE,
                                            bool store = true) {
 #pragma clang fp contract(off)
-  const float rho_i = pi.w;
+  const bool fi = pi.w > 0.0f;
+  const float rho_i = fabsf(pi.w);
   const float inv_h = 1.0f / P.h;
   const float cfr = -5.0f * P.adh / (P.h * P.h);
   const float m = P.particleMass;
@@ -145,7 +149,7 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         const float frj = fr * inv_rho_j;
         drho = fmaf(frj * m, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
         float vt = -(vi.w + vj.w) * pm_i;
-        if (fj) vt = fmaf(avc, fmaf(damper, nb.dv, dofv_i) + afa * dvdx * inv_re, vt);
+        if (fi & fj) vt = fmaf(avc, fmaf(damper, nb.dv, dofv_i) + afa * dvdx * inv_re, vt);
         vt *= frj;
         ax = fmaf(vt, dx, ax);
         if (DIM == 3) ay = fmaf(vt, dy, ay);

@@ -6,8 +6,11 @@
 //   computeBoundary_Delta_acoustic_D opt_src_v1/equation/ns.cu:30-134
 //   computeGovering_equationD        opt_src_v1/equation/ns.cu:137-230
 //   PC_predictionD / PC_correctionD  opt_src_v0/sph/sph_run.cu:388-484
-// with the reference's FP32 evaluation order, so they are the drop-in path.
-// The fast path (sph_fast_kernels.hip) has its own layout and launch shapes.
+// with the reference's FP32 evaluation order (SPH_NS_simpleversion in its
+// reference-order mode), plus the pack / unpack kernels that let the
+// default SPH_NS_simpleversion run the fast path's row-staged loops on the
+// caller's arrays.  The fast path (sph_fast_kernels.hip) has its own layout
+// and launch shapes.
 #include "sph_device.h"
 #include "sph_kernels.h"
 
@@ -329,6 +332,63 @@ hipError_t launch_ns_aos(const sph_params& P, const float* sPos, const float* sV
     hipLaunchKernelGGL(nsB_aos_kernel<3>, g, b, 0, s, P, sPos, sVel, sRho, sP, sType, dofv,
                        cellStart, cellEnd, hash, drho, acc, n);
   }
+  return hipGetLastError();
+}
+
+// The caller's sorted AoS state in the fast layout (one instance): PR =
+// (x, y, z, density signed + fluid / - otherwise), VP = (v, p), TY = type;
+// the cells holding fluid flagged for kernel A's row trimming (cleared
+// before).
+__global__ void ref_pack_kernel(const float* __restrict__ sPos, const float* __restrict__ sVel,
+                                const float* __restrict__ sRho, const float* __restrict__ sP,
+                                const int* __restrict__ sType, const int* __restrict__ hash, float4* __restrict__ PR,
+                                float4* __restrict__ VP, int* __restrict__ TY, int* __restrict__ cellFluid, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = sType[i];
+  const float r = fabsf(sRho[i]);
+  PR[i] = make_float4(sPos[3 * i], sPos[3 * i + 1], sPos[3 * i + 2], t == 1 ? r : -r);
+  VP[i] = make_float4(sVel[3 * i], sVel[3 * i + 1], sVel[3 * i + 2], sP[i]);
+  TY[i] = t & 0xff;
+  if (t == 1) cellFluid[hash[i]] = 1;  // every writer stores the same value
+}
+
+// Rates and the walls' extrapolated pressure and density back into the
+// caller's arrays (reference layouts).
+__global__ void ref_unpack_kernel(const float4* __restrict__ PR, const float4* __restrict__ VP,
+                                  const float* __restrict__ DV, const float4* __restrict__ RATE,
+                                  const int* __restrict__ sType, float* __restrict__ drho, float* __restrict__ acc,
+                                  float* __restrict__ dofv, float* __restrict__ sRho, float* __restrict__ sP, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 r = RATE[i];
+  drho[i] = r.x;
+  acc[3 * i] = r.y;
+  acc[3 * i + 1] = r.z;
+  acc[3 * i + 2] = r.w;
+  dofv[i] = DV[i];
+  if (sType[i] != 1) {
+    sP[i] = VP[i].w;
+    sRho[i] = -PR[i].w;
+  }
+}
+
+hipError_t launch_ref_pack(const float* sPos, const float* sVel, const float* sRho, const float* sP,
+                           const int* sType, const int* hash, const FastSet& st, int* cellFluid, int cells, int n,
+                           hipStream_t s) {
+  hipError_t e = fill_int(cellFluid, 0, cells, s);
+  if (e != hipSuccess || n <= 0) return e;
+  hipLaunchKernelGGL(ref_pack_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, sPos, sVel, sRho, sP, sType, hash,
+                     st.PR, st.VP, st.TY, cellFluid, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_unpack(const FastSet& st, const FastWork& w, const int* sType, float* drho, float* acc,
+                             float* dofv, float* sRho, float* sP, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ref_unpack_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, (const float4*)st.PR,
+                     (const float4*)st.VP, (const float*)w.DV, (const float4*)w.RATE, sType, drho, acc, dofv, sRho,
+                     sP, n);
   return hipGetLastError();
 }
 

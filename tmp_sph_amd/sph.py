@@ -221,6 +221,18 @@ def SPH_NS_simpleversion(sortedPos, sortedVel, sorteddensity, sortedpressure, so
          _p(particleHash), int(timestep), _p(dofv), _p(rhop_sum), _p(w_sum))
 
 
+def set_ns_mode(mode: int) -> None:
+    """Kernels behind SPH_NS_simpleversion: 0 the row-staged fast loops
+    (default), 1 the reference's FP32 operation order."""
+    call("sph_set_ns_mode", int(mode))
+
+
+def get_ns_mode() -> int:
+    m = C.c_int()
+    call("sph_get_ns_mode", C.byref(m))
+    return m.value
+
+
 def PC_prediction(sortedPos, sortedVel, sorteddensity, sortedpressure, densitydt, Veldt, particleHash,
                   particleIndex, Pos_tmp, Vel_tmp, density_tmp, numParticles, sorted_particle_type) -> None:
     """opt_src_v0/sph/sph_run.cu:486-500."""
