@@ -384,6 +384,7 @@ def reference_surface_bench(args) -> int:
     prm = case.params
     n = case.n
     sph.set_params(prm)
+    sph.reserve_scratch(n)
     D, E = sph.DeviceArray.from_host, sph.DeviceArray
     S = [{"pos": D(case.pos), "vel": D(case.vel), "rho": D(case.rho), "p": D(case.p), "type": D(case.type)},
          {"pos": E((n, 3), np.float32), "vel": E((n, 3), np.float32), "rho": E((n,), np.float32),

@@ -206,6 +206,12 @@ int sph_SPH_NS_simpleversion(const float* sortedPos, const float* sortedVel,
    loops, 1 the reference's FP32 operation order.  Process-wide. */
 int sph_set_ns_mode(int mode);
 int sph_get_ns_mode(int* mode);
+/* Allocates, now, the scratch that sph_sortParticles and
+   sph_SPH_NS_simpleversion would otherwise allocate on their first call for
+   n particles and the current parameters' cells (no counterpart in the
+   reference: Thrust allocates inside sort_by_key), so that no later call of
+   up to n particles allocates.  Needs sph_set_params. */
+int sph_reserve_scratch(int n);
 /* PC_prediction (src/header/sph.cuh:26; opt_src_v0/sph/sph_run.cu:388-435): fluid only;
    saves (x, v, rho) into the tmp arrays at the sorted slot and advances half a step. */
 int sph_PC_prediction(float* sortedPos, float* sortedVel, float* sortedDensity,

@@ -173,6 +173,7 @@ def test_reference_order_step(gpu, orc, mode):
     n = case.n
     sph.set_params(prm)
     sph.set_ns_mode(mode)
+    sph.reserve_scratch(n)  # the calls below then allocate nothing
     D, E = sph.DeviceArray.from_host, sph.DeviceArray
     A = {"pos": D(case.pos), "vel": D(case.vel), "rho": D(case.rho), "p": D(case.p), "type": D(case.type)}
     B = {"pos": E((n, 3), np.float32), "vel": E((n, 3), np.float32), "rho": E((n,), np.float32),

@@ -146,6 +146,7 @@ _PROTOS = {
     "sph_SPH_NS_simpleversion": [P] * 9 + [I, P, I, P, P, P],
     "sph_set_ns_mode": [I],
     "sph_get_ns_mode": [P],
+    "sph_reserve_scratch": [I],
     "sph_PC_prediction": [P] * 11 + [I, P],
     "sph_PC_correction": [P] * 11 + [I, P],
     "sph_sim_create": [C.POINTER(SphParams), I, P, P, P, P, P, C.POINTER(C.c_void_p)],

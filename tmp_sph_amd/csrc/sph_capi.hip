@@ -453,6 +453,18 @@ extern "C" int sph_SPH_NS_simpleversion(const float* sPos, const float* sVel, fl
   return SPH_OK;
 }
 
+extern "C" int sph_reserve_scratch(int n) {
+  if (n < 0) return set_error(SPH_E_INVAL, "sph_reserve_scratch: n must be >= 0");
+  sph_params P;
+  hipStream_t s;
+  int rc = current_params(&P, &s);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  rc = ensure_pool(n);
+  if (!rc) rc = ensure_ns_pool(n, P.numCells);
+  return rc;
+}
+
 extern "C" int sph_set_ns_mode(int mode) {
   if (mode != 0 && mode != 1) return set_error(SPH_E_INVAL, "sph_set_ns_mode: mode must be 0 or 1");
   std::lock_guard<std::mutex> lk(g_mu);

@@ -227,6 +227,12 @@ def set_ns_mode(mode: int) -> None:
     call("sph_set_ns_mode", int(mode))
 
 
+def reserve_scratch(n: int) -> None:
+    """Allocates the scratch of sortParticles and SPH_NS_simpleversion for n
+    particles now (after set_params), so those calls do not allocate."""
+    call("sph_reserve_scratch", int(n))
+
+
 def get_ns_mode() -> int:
     m = C.c_int()
     call("sph_get_ns_mode", C.byref(m))
