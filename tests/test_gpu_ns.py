@@ -161,48 +161,55 @@ def test_pc_bit_exact(gpu, orc):
         np.testing.assert_array_equal(e[k].numpy(), want2[k], err_msg=f"corrector {k}")
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_reference_order_step(gpu, orc, mode):
-    """One step through the reference call surface in the order of
-    opt_src_v0/sph/sph_run.cu:269-309, against the oracle's FP32 step, with
-    SPH_NS_simpleversion on the fast loops (mode 0) and in reference order
-    (mode 1)."""
-    sph = gpu
-    case = advanced(orc, case2d(0.02), 100, mode=0)
+def _ref_surface_run(sph, case, nsteps, mode):
+    """`nsteps` full steps through the reference call surface in the order of
+    opt_src_v0/sph/sph_run.cu:269-309 (per half-step calcHash, sortParticles,
+    reorder with the permutation table, SPH_NS_simpleversion, PC); returns
+    the state in original particle order."""
     prm = case.params
     n = case.n
     sph.set_params(prm)
     sph.set_ns_mode(mode)
     sph.reserve_scratch(n)  # the calls below then allocate nothing
     D, E = sph.DeviceArray.from_host, sph.DeviceArray
-    A = {"pos": D(case.pos), "vel": D(case.vel), "rho": D(case.rho), "p": D(case.p), "type": D(case.type)}
-    B = {"pos": E((n, 3), np.float32), "vel": E((n, 3), np.float32), "rho": E((n,), np.float32),
-         "p": E((n,), np.float32), "type": E((n,), np.int32)}
-    tl, tr = D(np.zeros(n, np.int32)), D(np.zeros(n, np.int32))
+    S = [{"pos": D(case.pos), "vel": D(case.vel), "rho": D(case.rho), "p": D(case.p), "type": D(case.type)},
+         {"pos": E((n, 3), np.float32), "vel": E((n, 3), np.float32), "rho": E((n,), np.float32),
+          "p": E((n,), np.float32), "type": E((n,), np.int32)}]
+    tabs = [D(np.zeros(n, np.int32)), D(np.zeros(n, np.int32))]
     h, idx = E((n,), np.int32), E((n,), np.int32)
     cs, ce = E((prm.numCells,), np.int32), E((prm.numCells,), np.int32)
     drho, acc, dofv = E((n,), np.float32), E((n, 3), np.float32), E((n,), np.float32)
     tpos, tvel, trho = E((n, 3), np.float32), E((n, 3), np.float32), E((n,), np.float32)
-    # half-step 1 (state in A)
-    sph.calcHash(A["pos"], h, idx, n)
-    sph.sortParticles(h, idx, n)
-    sph.reorder(A["pos"], A["vel"], A["rho"], A["p"], A["type"], None, B["pos"], B["vel"], B["rho"], B["p"],
-                B["type"], None, h, idx, cs, ce, n, tl, tr)
-    sph.SPH_NS_simpleversion(B["pos"], B["vel"], B["rho"], B["p"], B["type"], drho, acc, cs, ce, n, h, 0, dofv)
-    sph.PC_prediction(B["pos"], B["vel"], B["rho"], B["p"], drho, acc, h, idx, tpos, tvel, trho, n, B["type"])
-    # half-step 2 (state in B)
-    sph.calcHash(B["pos"], h, idx, n)
-    sph.sortParticles(h, idx, n)
-    sph.reorder(B["pos"], B["vel"], B["rho"], B["p"], B["type"], None, A["pos"], A["vel"], A["rho"], A["p"],
-                A["type"], None, h, idx, cs, ce, n, tr, tl)
-    sph.SPH_NS_simpleversion(A["pos"], A["vel"], A["rho"], A["p"], A["type"], drho, acc, cs, ce, n, h, 0, dofv)
-    sph.PC_correction(A["pos"], A["vel"], A["rho"], A["p"], drho, acc, h, idx, tpos, tvel, trho, n, A["type"])
-    sph.set_ns_mode(0)
-    ids = np.arange(n, dtype=np.int32) + tr.numpy()  # i + table[i] = original id (second reorder wrote table_r)
+    try:
+        for _ in range(nsteps):
+            for a, b, pred in ((0, 1, True), (1, 0, False)):
+                A, B = S[a], S[b]
+                sph.calcHash(A["pos"], h, idx, n)
+                sph.sortParticles(h, idx, n)
+                sph.reorder(A["pos"], A["vel"], A["rho"], A["p"], A["type"], None, B["pos"], B["vel"], B["rho"],
+                            B["p"], B["type"], None, h, idx, cs, ce, n, tabs[a], tabs[b])
+                sph.SPH_NS_simpleversion(B["pos"], B["vel"], B["rho"], B["p"], B["type"], drho, acc, cs, ce, n, h,
+                                         0, dofv)
+                f = sph.PC_prediction if pred else sph.PC_correction
+                f(B["pos"], B["vel"], B["rho"], B["p"], drho, acc, h, idx, tpos, tvel, trho, n, B["type"])
+    finally:
+        sph.set_ns_mode(0)
+    ids = np.arange(n, dtype=np.int32) + tabs[1].numpy()  # i + table[i] = original id (the second half wrote tabs[1])
     assert np.array_equal(np.sort(ids), np.arange(n))
-    got = {k: np.empty_like(v.numpy()) for k, v in A.items()}
-    for k, v in A.items():
+    got = {k: np.empty_like(v.numpy()) for k, v in S[0].items()}
+    for k, v in S[0].items():
         got[k][ids] = v.numpy()
+    return got
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_reference_order_step(gpu, orc, mode):
+    """One step through the reference call surface against the oracle's FP32
+    step, with SPH_NS_simpleversion on the fast loops (mode 0) and in
+    reference order (mode 1)."""
+    case = advanced(orc, case2d(0.02), 100, mode=0)
+    prm = case.params
+    got = _ref_surface_run(gpu, case, 1, mode)
     want, _, _, _ = orc.run(prm, case, 1, mode=0)
     mags = orc.eval_rates(prm, case, mode=0)
     dt = prm.timeStep
@@ -213,3 +220,22 @@ def test_reference_order_step(gpu, orc, mode):
     rtol = TOL * mags["mag_drho"][fl].max() * dt
     assert np.max(np.abs(got["rho"][fl] - want["rho"][fl])) <= rtol + 1e-6 * prm.restDensity
     np.testing.assert_array_equal(got["type"], want["type"])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_reference_surface_run_follows_the_host_run(gpu, orc, mode):
+    """600 steps (t = 0.1 s, the column collapsing) of the 2-D dam break at
+    dx = 0.02 m through the reference call surface: the surge front and
+    every fluid particle's position stay within 1e-4 m of the oracle's FP32
+    run from the same state (the step-to-step differences are rounding-level,
+    1e-5 of the rates)."""
+    case = case2d(0.02)
+    nsteps = 600
+    got = _ref_surface_run(gpu, case, nsteps, mode)
+    want, _, _, _ = orc.run(case.params, case, nsteps, mode=0)
+    fl = want["type"] == 1
+    np.testing.assert_array_equal(got["type"], want["type"])
+    front_got, front_want = got["pos"][fl, 0].max(), want["pos"][fl, 0].max()
+    assert front_want > 1.05  # the surge has left the column (1.0 m)
+    assert abs(front_got - front_want) <= 1e-4, (front_got, front_want)
+    assert np.abs(got["pos"][fl] - want["pos"][fl]).max() <= 1e-4
