@@ -435,14 +435,15 @@ def learned_bench(args) -> int:
 
     Training data: windows of one batched plain run (to t = 2 s) of the first
     --train-members sweep members with dx = 0.002 m (velocity history of 5
-    frames 10 steps apart, target = mean rates over the next 10 steps).
+    frames k steps apart, target = mean rates over the next k steps; k = 10
+    unless --k says otherwise).
     Training: Adam on minibatches of 4 snapshots on the GPU, --train-iters
     iterations, snapshot order from --seed.  Saves the weights
     (--save-weights) and evaluates them with learned_eval to --learned-t."""
     import numpy as np
     from tmp_sph_amd import cases as cs
     from tmp_sph_amd import surrogate as sg
-    k, history, layers = 10, 5, 10
+    k, history, layers = args.k, 5, 10
     t0 = time.time()
     members = [c for c, m in zip(cs.sweep_cases(), cs.sweep_members()) if m.dx == 0.002][:args.train_members]
     dts = sorted(float(c.params.timeStep) for c in members)
@@ -514,6 +515,8 @@ def main() -> int:
     ap.add_argument("--train-iters", type=int, default=1500, help="learned: Adam iterations (minibatches of 4)")
     ap.add_argument("--train-members", type=int, default=8, help="learned: sweep members (dx = 0.002 m) recorded")
     ap.add_argument("--seed", type=int, default=0, help="learned: training seed (snapshot order, initial weights)")
+    ap.add_argument("--k", type=int, default=10, help="learned: plain steps per network step (a configuration of "
+                    "its own: the default line evaluates the committed k = 10 weights)")
     ap.add_argument("--save-weights", default="", help="learned: weights file (default gpurun_out/surrogate_seedS.npz)")
     ap.add_argument("--learned-t", type=float, default=2.0, help="simulated time of the learned-vs-plain runs "
                     "(0: none)")
