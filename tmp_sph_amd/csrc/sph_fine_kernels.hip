@@ -299,7 +299,7 @@ struct FineVisit {
       }
       bool acc[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) acc[u] = j0 + u < je && accept(j0 + u, pjs[u]);
+      for (int u = 0; u < U; ++u) acc[u] = (j0 + u < je) & accept(j0 + u, pjs[u]);
       if constexpr (!PF) {
 #pragma unroll
         for (int u = 0; u < U; ++u)

@@ -213,7 +213,7 @@ struct PlainVisit {
         }
         bool acc[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc[u] = j0 + u < g.e && accept(j0 + u, pjs[u]);
+        for (int u = 0; u < U; ++u) acc[u] = (j0 + u < g.e) & accept(j0 + u, pjs[u]);
         if constexpr (!PF) {
           // the accepted candidates' loads go out together: one wait per group
 #pragma unroll

@@ -55,10 +55,12 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
   const float cut = P.rr_cut;
   float sp = 0.0f, sw = 0.0f, dv = 0.0f;
   visit(
-      [&](int j, const float4& pj) {
+      [&](int, const float4& pj) {
+        // masks, no short-circuit branches; a fluid target meets itself
+        // (rr = 0) and that pair adds exactly zero (dvdx = 0)
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        return (rr <= cut) & (j != i) & (pj.w > 0.0f);  // masks, no short-circuit branches
+        return (rr <= cut) & (pj.w > 0.0f);
       },
       ExtraA{VP},
       [&](int j, const float4& pj, const float4& vj) {
@@ -127,10 +129,12 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
   const float pm_i = m * inv_rho_i;
   float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
   visit(
-      [&](int j, const float4& pj) {
+      [&](int, const float4& pj) {
+        // the target meets itself (rr = 0): dx = dy = dz = 0, dvdx = 0 and
+        // rho_i - rho_j = 0 make that pair add exactly zero
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        return (rr <= cut) & (j != i);
+        return rr <= cut;
       },
       ExtraB{VP, DV},
       [&](int j, const float4& pj, const NbrB& nb) {
