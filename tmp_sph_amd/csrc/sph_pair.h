@@ -53,32 +53,51 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
   const float mg = 0.0f - P.gravity;
   const float m = P.particleMass;
   const float cut = P.rr_cut;
+  const float adh = P.adh;
   float sp = 0.0f, sw = 0.0f, dv = 0.0f;
-  visit(
-      [&](int, const float4& pj) {
-        // masks, no short-circuit branches; a fluid target meets itself
-        // (rr = 0) and that pair adds exactly zero (dvdx = 0)
-        float dx, dy, dz;
-        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        return (rr <= cut) & (pj.w > 0.0f);
-      },
-      ExtraA{VP},
-      [&](int j, const float4& pj, const float4& vj) {
-        float dx, dy, dz;
-        const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-        const float q = fast_sqrt(rr) * inv_h;
-        const float t = fmaf(-0.5f, q, 1.0f);
-        const float t3 = t * t * t;
-        if (!fi) {
-          const float w = (t3 * t) * (fmaf(2.0f, q, 1.0f) * P.adh);
-          sp = fmaf(fmaf(-pj.w, mg * dz, vj.w), w, sp);
-          sw += w;
-        } else {
-          const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
-                                      : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
-          dv = fmaf(-(cfr * t3 * m), dvdx * fast_rcp(pj.w), dv);
-        }
-      });
+  auto accept = [&](int, const float4& pj) {
+    // masks, no short-circuit branches; a fluid target meets itself (rr = 0)
+    // and that pair adds exactly zero (dvdx = 0)
+    float dx, dy, dz;
+    const float rr = geom<DIM>(pi, pj, dx, dy, dz);
+    return (rr <= cut) & (pj.w > 0.0f);
+  };
+  auto wall_pair = [&](int, const float4& pj, const float4& vj) {  // pressure extrapolation
+    float dx, dy, dz;
+    const float rr = geom<DIM>(pi, pj, dx, dy, dz);
+    const float q = fast_sqrt(rr) * inv_h;
+    const float t = fmaf(-0.5f, q, 1.0f);
+    const float t3 = t * t * t;
+    const float w = (t3 * t) * (fmaf(2.0f, q, 1.0f) * adh);
+    sp = fmaf(fmaf(-pj.w, mg * dz, vj.w), w, sp);
+    sw += w;
+  };
+  auto fluid_pair = [&](int, const float4& pj, const float4& vj) {  // velocity divergence
+    float dx, dy, dz;
+    const float rr = geom<DIM>(pi, pj, dx, dy, dz);
+    const float q = fast_sqrt(rr) * inv_h;
+    const float t = fmaf(-0.5f, q, 1.0f);
+    const float t3 = t * t * t;
+    const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
+                                : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
+    dv = fmaf(-(cfr * t3 * m), dvdx * fast_rcp(pj.w), dv);
+  };
+  // Waves of one target kind (most of them) run a body without the branch on
+  // the kind; the choice is uniform over the lanes that run the visit.
+  const bool walls_here = __ballot(store && !fi) != 0ull;
+  const bool fluid_here = __ballot(store && fi) != 0ull;
+  if (!walls_here) {
+    visit(accept, ExtraA{VP}, fluid_pair);
+  } else if (!fluid_here) {
+    visit(accept, ExtraA{VP}, wall_pair);
+  } else {
+    visit(accept, ExtraA{VP}, [&](int j, const float4& pj, const float4& vj) {
+      if (!fi)
+        wall_pair(j, pj, vj);
+      else
+        fluid_pair(j, pj, vj);
+    });
+  }
   if (!store) return;
   DV[i] = dv;
   if (!fi) {
