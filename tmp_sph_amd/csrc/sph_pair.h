@@ -54,6 +54,8 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
   const float m = P.particleMass;
   const float cut = P.rr_cut;
   const float adh = P.adh;
+  const float tq = -0.5f * inv_h;
+  const float ncm = -(cfr * m);
   float sp = 0.0f, sw = 0.0f, dv = 0.0f;
   auto accept = [&](int, const float4& pj) {
     // masks, no short-circuit branches; a fluid target meets itself (rr = 0)
@@ -73,14 +75,14 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
     sw += w;
   };
   auto fluid_pair = [&](int, const float4& pj, const float4& vj) {  // velocity divergence
+    // t = 1 - q/2 straight from sqrt(rr), -cfr m folded (reassociated
+    // against ns.cu:95-104: rounding moves in the last bits)
     float dx, dy, dz;
     const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-    const float q = fast_sqrt(rr) * inv_h;
-    const float t = fmaf(-0.5f, q, 1.0f);
-    const float t3 = t * t * t;
+    const float t = fmaf(fast_sqrt(rr), tq, 1.0f);
     const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
                                 : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
-    dv = fmaf(-(cfr * t3 * m), dvdx * fast_rcp(pj.w), dv);
+    dv = fmaf((t * t * t) * ncm, dvdx * fast_rcp(pj.w), dv);
   };
   // Waves of one target kind (most of them) run a body without the branch on
   // the kind; the choice is uniform over the lanes that run the visit.
@@ -132,20 +134,29 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
                                            const float* __restrict__ DV, float4* __restrict__ RATE,
                                            bool store = true) {
 #pragma clang fp contract(off)
+  // Target constants folded so that a pair costs as few VALU operations as
+  // possible: t = 1 - q/2 straight from sqrt(rr); the kernel factor cfr and
+  // the mass m ride on the per-target factors of the momentum terms; the
+  // artificial-viscosity coefficient afa rides on avc.  The products are
+  // reassociated against the reference's order (ns.cu:205-215): rounding
+  // moves in the last bits, within the parity contract.
   const bool fi = pi.w > 0.0f;
   const float rho_i = fabsf(pi.w);
   const float inv_h = 1.0f / P.h;
+  const float tq = -0.5f * inv_h;
   const float cfr = -5.0f * P.adh / (P.h * P.h);
   const float m = P.particleMass;
+  const float cm = cfr * m;
   const float dhc = P.delta * P.h * P.cs;
   const float eta2 = P.eta * P.eta;
   const float cut = P.rr_cut;
   const float damper = P.damper;
-  const float afa = P.afa;
   const float dofv_i = damper * dvi;
   const float inv_rho_i = 1.0f / rho_i;
-  const float avc = P.h * P.cs * P.restDensity * m * inv_rho_i;
-  const float pm_i = m * inv_rho_i;
+  const float pmc = m * inv_rho_i * cfr;                                  // pressure term factor
+  const float pic = -(vi.w * pmc);                                         // its target part
+  const float avcc = P.h * P.cs * P.restDensity * m * inv_rho_i * cfr;   // damper term factor
+  const float avca = avcc * P.afa;                                         // viscosity term factor
   float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
   visit(
       [&](int, const float4& pj) {
@@ -156,24 +167,21 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         return rr <= cut;
       },
       ExtraB{VP, DV},
-      [&](int j, const float4& pj, const NbrB& nb) {
+      [&](int, const float4& pj, const NbrB& nb) {
         float dx, dy, dz;
         const float rr = geom<DIM>(pi, pj, dx, dy, dz);
         const float4 vj = nb.v;
         const bool fj = pj.w > 0.0f;
         const float rho_j = fabsf(pj.w);
-        const float q = fast_sqrt(rr) * inv_h;
-        const float t = fmaf(-0.5f, q, 1.0f);
-        const float fr = cfr * (t * t * t);
-        const float inv_rho_j = fast_rcp(rho_j);
+        const float t = fmaf(fast_sqrt(rr), tq, 1.0f);
+        const float w = (t * t * t) * fast_rcp(rho_j);  // kernel shape over rho_j
         const float inv_re = fast_rcp(rr + eta2);
         const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
                                     : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
-        const float frj = fr * inv_rho_j;
-        drho = fmaf(frj * m, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
-        float vt = -(vi.w + vj.w) * pm_i;
-        if (fi & fj) vt = fmaf(avc, fmaf(damper, nb.dv, dofv_i) + afa * dvdx * inv_re, vt);
-        vt *= frj;
+        drho = fmaf(w * cm, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
+        float vt = fmaf(-pmc, vj.w, pic);
+        if (fi & fj) vt = fmaf(avca, dvdx * inv_re, fmaf(avcc, fmaf(damper, nb.dv, dofv_i), vt));
+        vt *= w;
         ax = fmaf(vt, dx, ax);
         if (DIM == 3) ay = fmaf(vt, dy, ay);
         az = fmaf(vt, dz, az);
