@@ -28,15 +28,19 @@ struct ExtraA {  // kernel A: velocity, pressure
 // ------------------------------------------------------------ per target
 // The pair arithmetic below is written with explicit fmaf and compiled with
 // contraction off, so that every loop variant (whose code the compiler
-// schedules differently) rounds identically.  The squared distance is the
-// reference's unfused sum, so the pair set is exactly the reference's.
+// schedules differently) rounds identically.  The squared distance is an fma
+// chain (three operations instead of five per candidate); it differs from the
+// reference's unfused sum by roundings of rr, which can move only a pair at
+// the cut-off radius (q = 2 within a few ulps) in or out of the set, and
+// there the kernel and its gradient vanish (t = 1 - q/2 = 0), so such a
+// pair's terms are zero to within 1e-20 of the others.
 template <int DIM>
 __device__ __forceinline__ float geom(const float4& pi, const float4& pj, float& dx, float& dy, float& dz) {
 #pragma clang fp contract(off)
   dx = pi.x - pj.x;
   dy = DIM == 3 ? pi.y - pj.y : 0.0f;
   dz = pi.z - pj.z;
-  return DIM == 3 ? dx * dx + dy * dy + dz * dz : dx * dx + dz * dz;
+  return DIM == 3 ? fmaf(dx, dx, fmaf(dy, dy, dz * dz)) : fmaf(dx, dx, dz * dz);
 }
 
 // Kernel A for target i (fluid neighbours only).  WALL_RHO: also store a
