@@ -231,9 +231,9 @@ int sph_PC_correction(float* sortedPos, float* sortedVel, float* sortedDensity,
 typedef struct sph_sim sph_sim;
 
 /* Launch-shape knobs of the fast path, filled with defaults by sph_sim_create
-   (row staging with neighbour data, prefetch; 3-D: unroll 2, runs of 32
-   tiles per XCD; 2-D: unroll 4, one run of tiles per XCD) and chosen per
-   workload by sph_sim_autotune.  None changes the results. */
+   (row staging with neighbour data, prefetch, unroll 4; 3-D: runs of 32
+   tiles per XCD; 2-D: one run of tiles per XCD) and chosen per workload
+   with sph_sim_set_tuning.  None changes the results. */
 typedef struct sph_tuning {
     int nbr_block;       /* threads per block of the neighbour-loop kernels: 64, 128, 192, 256 */
     int nbr_unroll;      /* candidate loads issued together: 1, 2, 4 or 8 */

@@ -189,7 +189,7 @@ int ns_fast(const sph_params& P, hipStream_t s, const float* sPos, const float* 
   w.RATE = q.RATE;
   sph_tuning tu{};
   tu.nbr_block = 64;
-  tu.nbr_unroll = P.dim == 3 ? 2 : 4;  // the contexts' default shapes
+  tu.nbr_unroll = 4;  // the contexts' default shapes
   tu.xcd_remap = P.dim == 3 ? 32 : 1;
   tu.nbr_lds = 3;
   tu.nbr_prefetch = 1;

@@ -242,12 +242,12 @@ extern "C" int sph_sim_create_batch(const sph_params* params, int K, const int* 
   s->bits2 = fine_ok ? key_bits((int)ctot2) : 1;
   s->bits = s->bits1;
   // default launch shape: row staging with neighbour data and prefetch (the
-  // autotuner's pick for the 2-D and 3-D dam breaks from 0.5 M particles up);
-  // 3-D: unroll 2, runs of 32 tiles dealt to the XCDs in turn (a third less
-  // HBM traffic in the loops at the same speed); 2-D: unroll 4, one
-  // contiguous run of tiles per XCD (C2 0.355 -> 0.341 ms per step)
+  // fastest measured for the 2-D and 3-D dam breaks from 0.5 M particles up),
+  // unroll 4; 3-D: runs of 32 tiles dealt to the XCDs in turn (a third less
+  // HBM traffic in the loops at the same speed); 2-D: one contiguous run of
+  // tiles per XCD (C2 0.355 -> 0.341 ms per step)
   s->tun.nbr_block = 64;
-  s->tun.nbr_unroll = params[0].dim == 3 ? 2 : 4;
+  s->tun.nbr_unroll = 4;
   s->tun.xcd_remap = params[0].dim == 3 ? 32 : 1;
   s->tun.flags = 0;
   s->tun.nbr_lds = 3;
