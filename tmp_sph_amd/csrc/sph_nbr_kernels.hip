@@ -615,8 +615,10 @@ __global__ __launch_bounds__(STAGED ? 64 : 256, STAGED >= 2 ? 6 : 1) void nsA_ke
                                       });
 }
 
+// 5 waves per SIMD (up to 96 VGPRs): faster for kernel B than 6 (80 VGPRs);
+// kernel A, with the smaller body, runs best at 6
 template <int DIM, int U, int STAGED, int COMPACT, bool PF>
-__global__ __launch_bounds__(STAGED ? 64 : 256, STAGED >= 2 ? 6 : 1) void nsB_kernel(InstTable T, const float4* __restrict__ PR,
+__global__ __launch_bounds__(STAGED ? 64 : 256, STAGED >= 2 ? 5 : 1) void nsB_kernel(InstTable T, const float4* __restrict__ PR,
                                                                 const float4* __restrict__ VP,
                                                                 const int* __restrict__ TY,
                                                                 const float* __restrict__ DV,
