@@ -606,6 +606,8 @@ def main() -> int:
         elapsed += dt_w
         windows.append({"first_step": args.warmup + done + extra, "steps": k, "ms_per_step": 1000.0 * dt_w / k})
         done += k
+        print(f"#   window {len(windows)}: {done} of {args.steps} steps, {1000.0 * dt_w / k:.3f} ms/step",
+              file=sys.stderr, flush=True)
         sim.step(args.timing_steps)
         sim.synchronize()
         extra += args.timing_steps
@@ -624,9 +626,13 @@ def main() -> int:
     us = kt[dom]
     byts = _algorithmic_bytes(dom, prm.dim, n_total, n_fluid)
     achieved = byts / (us * 1e-6) / 1e9
-    # counted flops: 9 per candidate + 55 per in-support pair (kernel B), 9 + 25 (kernel A)
-    flops = (9.0 * stats["candB"] + 55.0 * stats["hitsB"]) if dom == "nsB" else \
-        (9.0 * stats["candA"] + 25.0 * stats["hitsA"])
+    # counted flops of the pair code (an fma counts two): the cut-off test 8 per
+    # candidate in 3-D (5 in 2-D); an in-support pair 42 in kernel B (37 in
+    # 2-D), 18 in kernel A (15 in 2-D)
+    d3 = prm.dim == 3
+    ft = 8.0 if d3 else 5.0
+    flops = (ft * stats["candB"] + (42.0 if d3 else 37.0) * stats["hitsB"]) if dom == "nsB" else \
+        (ft * stats["candA"] + (18.0 if d3 else 15.0) * stats["hitsA"])
     valu = flops / (us * 1e-6) / 1e12
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{dom}_{args.config}.json"
