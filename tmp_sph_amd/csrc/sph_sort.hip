@@ -19,7 +19,7 @@ namespace {
 constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_IPT_MAX = 16;  // keys per thread and tile of the largest tile
-constexpr int RS_MIN_TILES = 512;  // small sorts use smaller tiles to fill the GPU
+constexpr int RS_MIN_TILES = 256;  // small sorts use smaller tiles to fill the GPU
 
 static_assert(RS_THREADS == 256, "one thread per digit in the per-digit phases");
 
