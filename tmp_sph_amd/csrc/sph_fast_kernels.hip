@@ -33,10 +33,21 @@ namespace {
 inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 
+// With cellStart (not null) it also clears the cell tables for the gather
+// that follows the sort (cellStart = -1, cellFluid = 0 for all ncells cells),
+// which saves that launch.
 __global__ void hash_fast_kernel(InstTable T, const float4* __restrict__ PR, const int* __restrict__ TY,
-                                 int* __restrict__ keys, int* __restrict__ idx, int n, int* __restrict__ zero) {
+                                 int* __restrict__ keys, int* __restrict__ idx, int n, int* __restrict__ zero,
+                                 int* __restrict__ cellStart, int* __restrict__ cellFluid, long long ncells) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (zero && i == 0) *zero = 0;
+  if (cellStart) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long c = i; c < ncells; c += stride) {
+      cellStart[c] = -1;
+      cellFluid[c] = 0;
+    }
+  }
   if (i >= n) return;
   const float4 p = PR[i];
   with_instance(T, inst_of(TY[i]), [&](const sph_params& P, int coff) {
@@ -321,18 +332,19 @@ hipError_t launch_dim(int dim, K2 k2, K3 k3, dim3 g, dim3 b, hipStream_t s, A...
 }  // namespace
 
 hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s,
-                            int* zero) {
-  if (n <= 0 && !zero) return hipSuccess;
+                            int* zero, int* cellStart, int* cellFluid) {
+  if (n <= 0 && !zero && !cellStart) return hipSuccess;
+  const long long nc = cellStart ? T.total_cells : 0;
   hipLaunchKernelGGL(hash_fast_kernel, dim3(blocks_for(n > 0 ? n : 1, 256)), dim3(256), 0, s, T, in.PR, in.TY, keys,
-                     idx, n, zero);
+                     idx, n, zero, cellStart, cellFluid, nc);
   return hipGetLastError();
 }
 
 hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
                                     const int* idx, int* cellStart, int* cellEnd, int* cellFluid, int n,
-                                    hipStream_t s) {
+                                    hipStream_t s, bool cleared) {
   const long long nc = T.total_cells;
-  if (nc > 0) {  // both cell tables cleared by one launch
+  if (nc > 0 && !cleared) {  // both cell tables cleared by one launch
     const long long blocks = std::min<long long>((nc + 255) / 256, 4096);
     hipLaunchKernelGGL(clear_cells_kernel, dim3((unsigned)blocks), dim3(256), 0, s, cellStart, cellFluid, nc);
     hipError_t e = hipGetLastError();

@@ -111,8 +111,10 @@ hipError_t launch_ref_unpack(const FastSet& st, const FastWork& w, const int* sT
                              float* dofv, float* sRho, float* sP, int n, hipStream_t s);
 
 // zero (may be null): one word set to 0 by the launch (the dense table's run counter).
+// zero: an int set to 0 on the way; cellStart/cellFluid: cell tables cleared
+// on the way (for launch_cells_gather_fast with cleared = true).
 hipError_t launch_hash_fast(const InstTable& T, const FastSet& in, int* keys, int* idx, int n, hipStream_t s,
-                            int* zero = nullptr);
+                            int* zero = nullptr, int* cellStart = nullptr, int* cellFluid = nullptr);
 // Dense cell table (cellFirst) of the sorted keys, the gather into sorted
 // order and the coarse fluid flags (Tc: the instances' own grids): the cell
 // build of the fine-row loops.
@@ -120,9 +122,10 @@ hipError_t launch_cells_dense(const InstTable& T, const InstTable& Tc, const Fas
                               const int* keys, const int* idx, const FastWork& w, int n, hipStream_t s);
 // Capacity of the run queue for a table of `cells` cells.
 int dense_runs_cap(long long cells);
+// cleared: the hash already cleared cellStart and cellFluid.
 hipError_t launch_cells_gather_fast(const InstTable& T, const FastSet& in, const FastSet& out, const int* keys,
                                     const int* idx, int* cellStart, int* cellEnd, int* cellFluid, int n,
-                                    hipStream_t s);
+                                    hipStream_t s, bool cleared = false);
 hipError_t launch_nsA_fast(const InstTable& T, const FastSet& st, const FastWork& w, int n, const sph_tuning& tu,
                            hipStream_t s);
 // Neighbour-list mode: one scan records each target's in-support neighbours

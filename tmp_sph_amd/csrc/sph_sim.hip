@@ -70,8 +70,9 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
   FastWork w = s->w;
   hipError_t e;
   const bool dense = s->tun.nbr_lds == 4;  // the fine-row loops read the dense cell table
-  e = timed(s, timing, "hash", [&] {
-    return launch_hash_fast(s->T, s->set[from], w.keys, w.idx, n, s->stream, dense ? w.runCount : nullptr);
+  e = timed(s, timing, "hash", [&] {  // (it clears the cell tables for the gather too)
+    return launch_hash_fast(s->T, s->set[from], w.keys, w.idx, n, s->stream, dense ? w.runCount : nullptr,
+                            dense ? nullptr : w.cellStart, dense ? nullptr : w.cellFluid);
   });
   if (e != hipSuccess) return e;
   bool alt = false;
@@ -88,7 +89,7 @@ hipError_t sort_phase(sph_sim* s, int from, int to, bool timing, FastWork* out) 
   e = timed(s, timing, "cells_gather", [&] {
     if (dense) return launch_cells_dense(s->T, s->T1, s->set[from], s->set[to], w.keys, w.idx, w, n, s->stream);
     return launch_cells_gather_fast(s->T, s->set[from], s->set[to], w.keys, w.idx, w.cellStart, w.cellEnd,
-                                    w.cellFluid, n, s->stream);
+                                    w.cellFluid, n, s->stream, true);
   });
   *out = w;
   return e;
