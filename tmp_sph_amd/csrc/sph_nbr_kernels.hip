@@ -29,7 +29,7 @@
 //            then runs the pair body over each lane's set bits in ascending
 //            order: a wave executes as many bodies per group as its busiest
 //            lane accepted, not one per candidate.
-// A context starts with rows, unroll 2 and prefetch (neighbour data read with
+// A context starts with rows, unroll 4 and prefetch (neighbour data read with
 // every candidate), the fastest shape measured for the 2-D and 3-D dam breaks
 // from 0.5 M particles up.
 #include "sph_pair.h"
