@@ -186,6 +186,31 @@ def test_escaped_particle_is_parked(gpu):
     assert np.all(st["vel"][fl[-1]] == 0) and st["p"][fl[-1]] == 0
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lone_fluid_particle_meets_only_itself(gpu, dim):
+    """A fluid particle with nothing else in reach (in the air, away from the
+    column and the walls) pairs only with itself in the neighbour loops, and
+    that pair's terms are exactly zero (dx = dy = dz = 0, dvdx = 0, equal
+    densities): drho = dofv = 0 and acc = (0, 0, g) bit for bit, for the
+    default row-staged loop, the plain global loop and the fine rows."""
+    case = case2d(0.02) if dim == 2 else case3d(0.04)
+    k = int(np.flatnonzero(case.type == 1)[-1])
+    case.pos[k] = (3.0, 0.0, 0.7) if dim == 2 else (0.6, 0.2, 0.35)
+    case.vel[k] = (0.3, 0.1 if dim == 3 else 0.0, -0.2)
+    case.rho[k] = case.params.restDensity * 1.002
+    case.p[k] = 500.0
+    reach = np.sqrt(np.float64(case.params.rr_cut))
+    others = np.delete(case.pos.astype(np.float64), k, axis=0)
+    assert np.min(np.linalg.norm(others - case.pos[k].astype(np.float64), axis=1)) > 1.5 * reach
+    shapes = ({}, dict(nbr_lds=0, nbr_unroll=2, nbr_block=256, nbr_prefetch=0), dict(nbr_lds=4, cell_div=1))
+    with gpu.Simulation(case) as sim:
+        for shape in shapes:
+            sim.set_tuning(**shape)
+            r = sim.eval_rates()
+            assert r["drho"][k] == 0.0 and r["dofv"][k] == 0.0, shape
+            np.testing.assert_array_equal(r["acc"][k], np.float32([0.0, 0.0, case.params.gravity]), err_msg=str(shape))
+
+
 def _gpu_flow_state(gpu, case, steps):
     """The case advanced `steps` steps by the GPU (graph replay), returned as a
     new case in original particle order: a developed flow state at full size
