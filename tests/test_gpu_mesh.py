@@ -75,6 +75,30 @@ def test_sort_many_ties_and_full_range(gpu, orc):
         np.testing.assert_array_equal(dk.numpy(), keys[want_i])
 
 
+@pytest.mark.parametrize("n", [1_048_576 + 77, 600_001, 300_001, 140_003, 70_001, 65_537, 3])
+def test_sort_every_tile_size_with_a_ragged_tail(gpu, n):
+    """Sizes that select each tile size of the sort (16, 8, 4, 2 and 1 keys per
+    thread, at least 256 tiles) with a last tile that ends inside a wave: the
+    lanes past the end take part in no ranking and store nothing."""
+    sph = gpu
+    prm = sph.reference_params()
+    prm.gridxdim, prm.gridydim, prm.gridzdim = 1_500_000, 1, 1
+    prm.numCells = 1_500_000
+    sph.set_params(prm)
+    rng = np.random.default_rng(n)
+    near = np.sort(rng.integers(0, prm.numCells, n))
+    swap = rng.integers(0, max(n - 1, 1), n // 50)
+    near[swap], near[swap + 1] = near[swap + 1], near[swap].copy()
+    for keys in (rng.integers(0, prm.numCells, n), near):
+        keys = keys.astype(np.int32)
+        dk = sph.DeviceArray.from_host(keys)
+        di = sph.DeviceArray.from_host(np.arange(n, dtype=np.int32))
+        sph.sortParticles(dk, di, n)
+        want_i = np.argsort(keys, kind="stable").astype(np.int32)
+        np.testing.assert_array_equal(di.numpy(), want_i, err_msg=f"n={n}")
+        np.testing.assert_array_equal(dk.numpy(), keys[want_i], err_msg=f"n={n}")
+
+
 @pytest.mark.parametrize("num_cells", [37, 300, 2048, 2049, 65536, 1 << 20, 1_500_000, 1 << 22, (1 << 22) + 1])
 def test_sort_digit_plans(gpu, num_cells):
     """Key ranges from 6 to 23 bits (one to three 8-bit passes, a partial last

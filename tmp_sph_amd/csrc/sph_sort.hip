@@ -23,20 +23,26 @@ constexpr int RS_MIN_TILES = 256;  // small sorts use smaller tiles to fill the 
 
 static_assert(RS_THREADS == 256, "one thread per digit in the per-digit phases");
 
+// The digit of the previous lane: DPP wave_shr:1 (lane l reads lane l-1 within
+// the same instruction; lane 0 keeps its own value, which is never used).
+__device__ __forceinline__ unsigned prev_lane(unsigned d) {
+  return (unsigned)__builtin_amdgcn_update_dpp((int)d, (int)d, 0x138, 0xf, 0xf, false);
+}
+
 // Lanes of the wave whose digit equals this lane's digit (among `active`,
 // a prefix of the wave).
 //
 // The cell keys arrive nearly sorted (the previous step's order), and every
 // LSD pass keeps each bucket of the previous digit in key order, so in almost
 // every wave the digits do not decrease from lane to lane.  Then equal digits
-// form one run of consecutive lanes, found with one shuffle and one ballot.
+// form one run of consecutive lanes, found with one DPP shift and one ballot.
 // Otherwise (a bucket boundary, or unsorted keys) the digit is matched bit by
 // bit with one ballot per bit.
 __device__ __forceinline__ unsigned long long match_digit(unsigned d, int nbits,
                                                           unsigned long long active) {
   const int lane = __lane_id();
   const bool on = (active >> lane) & 1ull;
-  const unsigned dp = __shfl_up(d, 1);
+  const unsigned dp = prev_lane(d);
   const bool start = lane == 0 || d != dp;
   if (__ballot(on && lane > 0 && d < dp) == 0ull) {  // wave-uniform: non-decreasing digits
     const unsigned long long starts = __ballot(on && start);
@@ -106,30 +112,68 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const int* __restrict__
   hist[(size_t)tid * numTiles + blockIdx.x] = cnt[tid];
 }
 
+// Exclusive scan of one value per thread over a 256-thread block: a shuffle
+// scan per wave, then the wave totals through `wsum` (one barrier).  `all`
+// receives the block total.  `wsum` may be reused only after another barrier.
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* wsum, unsigned* all) {
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+  unsigned x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  unsigned off = 0u, tot = 0u;
+#pragma unroll
+  for (int k = 0; k < RS_WAVES; ++k) {
+    const unsigned s = wsum[k];
+    off += k < w ? s : 0u;
+    tot += s;
+  }
+  *all = tot;
+  return off + x - v;
+}
+
+// One block per digit turns its row of per-tile counts into exclusive tile
+// offsets.  Up to SCAN_REG entries per thread are read once into registers
+// (every load in flight together); longer rows take a loop.
+constexpr int SCAN_REG = 8;
+
 __global__ __launch_bounds__(256) void rs_scan_rows(unsigned* __restrict__ hist, int numTiles,
                                                     unsigned* __restrict__ totals) {
-  __shared__ unsigned s[256];
+  __shared__ unsigned wsum[RS_WAVES];
   unsigned* row = hist + (size_t)blockIdx.x * numTiles;
   const int per = (numTiles + 255) / 256;
   const int b = threadIdx.x * per;
   const int e = min(b + per, numTiles);
-  unsigned sum = 0;
-  for (int t = b; t < e; ++t) sum += row[t];
-  s[threadIdx.x] = sum;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    unsigned v = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += v;
-    __syncthreads();
+  unsigned all;
+  if (per <= SCAN_REG) {
+    unsigned c[SCAN_REG];
+    unsigned sum = 0u;
+#pragma unroll
+    for (int k = 0; k < SCAN_REG; ++k) {
+      c[k] = b + k < e ? row[b + k] : 0u;
+      sum += c[k];
+    }
+    unsigned run = block_exclusive_scan(sum, wsum, &all);
+#pragma unroll
+    for (int k = 0; k < SCAN_REG; ++k) {
+      if (b + k < e) row[b + k] = run;
+      run += c[k];
+    }
+  } else {
+    unsigned sum = 0u;
+    for (int t = b; t < e; ++t) sum += row[t];
+    unsigned run = block_exclusive_scan(sum, wsum, &all);
+    for (int t = b; t < e; ++t) {
+      const unsigned c = row[t];
+      row[t] = run;
+      run += c;
+    }
   }
-  unsigned run = threadIdx.x ? s[threadIdx.x - 1] : 0u;
-  for (int t = b; t < e; ++t) {
-    unsigned c = row[t];
-    row[t] = run;
-    run += c;
-  }
-  if (threadIdx.x == 255) totals[blockIdx.x] = s[255];
+  if (threadIdx.x == 0) totals[blockIdx.x] = all;
 }
 
 template <int RS_IPT>
@@ -139,42 +183,44 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     const unsigned* __restrict__ totals, int numTiles) {
   __shared__ unsigned wcnt[RS_WAVES][256];
   __shared__ unsigned dbase[256];
+  __shared__ unsigned wsum[RS_WAVES];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  for (int k = tid; k < RS_WAVES * 256; k += RS_THREADS) (&wcnt[0][0])[k] = 0u;
-  // exclusive scan of the digit totals -> first output slot of each digit
+  const long long base = (long long)blockIdx.x * (RS_THREADS * RS_IPT) + (long long)w * (64 * RS_IPT);
+  // The digit totals and this tile's offsets are loaded first, then every key
+  // and value of the wave, all before the first barrier: the wave waits for
+  // HBM once (loaded inside the ranking rounds, each round waited for its own
+  // loads), and the scan of the totals waits only for the first two loads.
   const unsigned own = totals[tid];
-  dbase[tid] = own;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    unsigned v = tid >= off ? dbase[tid - off] : 0u;
-    __syncthreads();
-    dbase[tid] += v;
-    __syncthreads();
-  }
-  const unsigned myBase = dbase[tid] - own + hist[(size_t)tid * numTiles + blockIdx.x];
-  __syncthreads();
-  dbase[tid] = myBase;
+  const unsigned tileOff = hist[(size_t)tid * numTiles + blockIdx.x];
+  __builtin_amdgcn_sched_barrier(0);  // keep these two loads ahead of the rest
+  // Lanes past the end load the last element (n >= 2 here): one straight-line
+  // path keeps the compiler's wait counts exact.  Those lanes stay out of the
+  // ballots and are never stored.
+  int kk[RS_IPT], vv[RS_IPT];
+#pragma unroll
+  for (int r = 0; r < RS_IPT; ++r) kk[r] = keys_in[min(base + r * 64 + lane, (long long)n - 1)];
+#pragma unroll
+  for (int r = 0; r < RS_IPT; ++r) vv[r] = vals_in[min(base + r * 64 + lane, (long long)n - 1)];
+  for (int k = tid; k < RS_WAVES * 256; k += RS_THREADS) (&wcnt[0][0])[k] = 0u;
+  // first output slot of each digit in this tile: exclusive scan of the digit
+  // totals plus the tile's offset within the digit (the scan's barrier also
+  // orders the clearing of wcnt before the ranking rounds)
+  unsigned all;
+  dbase[tid] = block_exclusive_scan(own, wsum, &all) + tileOff;
 
   const unsigned mask = (1u << nbits) - 1u;
   const unsigned long long ltmask = (1ull << lane) - 1ull;
-  const long long base = (long long)blockIdx.x * (RS_THREADS * RS_IPT) + (long long)w * (64 * RS_IPT);
-  int kk[RS_IPT], vv[RS_IPT];
   unsigned rk[RS_IPT];
 #pragma unroll
   for (int r = 0; r < RS_IPT; ++r) {
-    const long long i = base + r * 64 + lane;
-    const bool valid = i < n;
-    const int key = valid ? keys_in[i] : 0;
-    const int val = valid ? vals_in[i] : 0;
-    const unsigned d = (((unsigned)key) >> shift) & mask;
+    const bool valid = base + r * 64 + lane < n;
+    const unsigned d = (((unsigned)kk[r]) >> shift) & mask;
     const unsigned long long active = __ballot(valid);
     const unsigned long long peers = match_digit(d, nbits, active);
     const int leader = __ffsll((long long)peers) - 1;
     const unsigned before = wcnt[w][d];
     rk[r] = before + (unsigned)__popcll(peers & ltmask);
     if (valid && lane == leader) wcnt[w][d] = before + (unsigned)__popcll(peers);
-    kk[r] = key;
-    vv[r] = val;
   }
   __syncthreads();
   {
