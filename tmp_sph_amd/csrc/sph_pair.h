@@ -57,9 +57,13 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
   const float mg = 0.0f - P.gravity;
   const float m = P.particleMass;
   const float cut = P.rr_cut;
-  const float adh = P.adh;
   const float tq = -0.5f * inv_h;
-  const float ncm = -(cfr * m);
+  const float uq = 2.0f * inv_h;
+  const float ncm = -(cfr * m);  // applied once to the divergence sum
+  // The wall pressure is the ratio of the two wall sums, so the kernel
+  // constant adh cancels: neither sum carries it, and the emptiness threshold
+  // of the weight sum is scaled instead.
+  const float swmin = 1.0E-8f / P.adh;
   float sp = 0.0f, sw = 0.0f, dv = 0.0f;
   auto accept = [&](int, const float4& pj) {
     // masks, no short-circuit branches; a fluid target meets itself (rr = 0)
@@ -71,22 +75,22 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
   auto wall_pair = [&](int, const float4& pj, const float4& vj) {  // pressure extrapolation
     float dx, dy, dz;
     const float rr = geom<DIM>(pi, pj, dx, dy, dz);
-    const float q = fast_sqrt(rr) * inv_h;
-    const float t = fmaf(-0.5f, q, 1.0f);
+    const float s = fast_sqrt(rr);
+    const float t = fmaf(s, tq, 1.0f);  // 1 - q/2
     const float t3 = t * t * t;
-    const float w = (t3 * t) * (fmaf(2.0f, q, 1.0f) * adh);
+    const float w = (t3 * t) * fmaf(s, uq, 1.0f);  // (1 - q/2)^4 (1 + 2q)
     sp = fmaf(fmaf(-pj.w, mg * dz, vj.w), w, sp);
     sw += w;
   };
   auto fluid_pair = [&](int, const float4& pj, const float4& vj) {  // velocity divergence
-    // t = 1 - q/2 straight from sqrt(rr), -cfr m folded (reassociated
-    // against ns.cu:95-104: rounding moves in the last bits)
+    // t = 1 - q/2 straight from sqrt(rr), -cfr m applied to the sum
+    // (reassociated against ns.cu:95-104: rounding moves in the last bits)
     float dx, dy, dz;
     const float rr = geom<DIM>(pi, pj, dx, dy, dz);
     const float t = fmaf(fast_sqrt(rr), tq, 1.0f);
     const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
                                 : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
-    dv = fmaf((t * t * t) * ncm, dvdx * fast_rcp(pj.w), dv);
+    dv = fmaf(t * t * t, dvdx * fast_rcp(pj.w), dv);
   };
   // Waves of one target kind (most of them) run a body without the branch on
   // the kind; the choice is uniform over the lanes that run the visit.
@@ -105,9 +109,9 @@ __device__ __forceinline__ void nsA_target(const sph_params& P, int i, const flo
     });
   }
   if (!store) return;
-  DV[i] = dv;
+  DV[i] = dv * ncm;
   if (!fi) {
-    float pw = fabsf(sw) > 1.0E-8f ? sp / sw : 0.0f;
+    float pw = fabsf(sw) > swmin ? sp / sw : 0.0f;
     if (pw < 0.0f) pw = 0.0f;
     reinterpret_cast<float*>(VP)[4 * i + 3] = pw;
     if (WALL_RHO) reinterpret_cast<float*>(PR)[4 * i + 3] = -eos_density(P, pw);
@@ -140,8 +144,9 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
 #pragma clang fp contract(off)
   // Target constants folded so that a pair costs as few VALU operations as
   // possible: t = 1 - q/2 straight from sqrt(rr); the kernel factor cfr and
-  // the mass m ride on the per-target factors of the momentum terms; the
-  // artificial-viscosity coefficient afa rides on avc.  The products are
+  // the mass m ride on the per-target factors of the momentum terms and
+  // multiply the density sum once; the artificial-viscosity coefficient afa
+  // rides on avc.  The products are
   // reassociated against the reference's order (ns.cu:205-215): rounding
   // moves in the last bits, within the parity contract.
   const bool fi = pi.w > 0.0f;
@@ -182,7 +187,7 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         const float inv_re = fast_rcp(rr + eta2);
         const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
                                     : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
-        drho = fmaf(w * cm, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
+        drho = fmaf(w, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
         float vt = fmaf(-pmc, vj.w, pic);
         if (fi & fj) vt = fmaf(avca, dvdx * inv_re, fmaf(avcc, fmaf(damper, nb.dv, dofv_i), vt));
         vt *= w;
@@ -190,7 +195,7 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         if (DIM == 3) ay = fmaf(vt, dy, ay);
         az = fmaf(vt, dz, az);
       });
-  if (store) RATE[i] = make_float4(drho, ax, ay, az + P.gravity);
+  if (store) RATE[i] = make_float4(drho * cm, ax, ay, az + P.gravity);
 }
 
 // ------------------------------------------------------------ staging
