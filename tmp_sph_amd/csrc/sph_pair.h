@@ -166,7 +166,9 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
   const float pic = -(vi.w * pmc);                                         // its target part
   const float avcc = P.h * P.cs * P.restDensity * m * inv_rho_i * cfr;   // damper term factor
   const float avca = avcc * P.afa;                                         // viscosity term factor
-  float drho = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
+  // the density rate as two sums, sum w dvdx and sum w (rho_i - rho_j) rr / (rr + eta^2),
+  // combined once per target with rho_i, dhc and the constants
+  float dr1 = 0.0f, dr2 = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
   visit(
       [&](int, const float4& pj) {
         // the target meets itself (rr = 0): dx = dy = dz = 0, dvdx = 0 and
@@ -187,7 +189,8 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         const float inv_re = fast_rcp(rr + eta2);
         const float dvdx = DIM == 3 ? fmaf(vi.x - vj.x, dx, fmaf(vi.y - vj.y, dy, (vi.z - vj.z) * dz))
                                     : fmaf(vi.x - vj.x, dx, (vi.z - vj.z) * dz);
-        drho = fmaf(w, fmaf(rho_i, dvdx, (rho_i - rho_j) * rr * (dhc * inv_re)), drho);
+        dr1 = fmaf(w, dvdx, dr1);
+        dr2 = fmaf(w, (rho_i - rho_j) * rr * inv_re, dr2);
         float vt = fmaf(-pmc, vj.w, pic);
         if (fi & fj) vt = fmaf(avca, dvdx * inv_re, fmaf(avcc, fmaf(damper, nb.dv, dofv_i), vt));
         vt *= w;
@@ -195,7 +198,7 @@ __device__ __forceinline__ void nsB_target(const sph_params& P, int i, const flo
         if (DIM == 3) ay = fmaf(vt, dy, ay);
         az = fmaf(vt, dz, az);
       });
-  if (store) RATE[i] = make_float4(drho * cm, ax, ay, az + P.gravity);
+  if (store) RATE[i] = make_float4(cm * fmaf(rho_i, dr1, dhc * dr2), ax, ay, az + P.gravity);
 }
 
 // ------------------------------------------------------------ staging
