@@ -627,11 +627,11 @@ def main() -> int:
     byts = _algorithmic_bytes(dom, prm.dim, n_total, n_fluid)
     achieved = byts / (us * 1e-6) / 1e9
     # counted flops of the pair code (an fma counts two): the cut-off test 8 per
-    # candidate in 3-D (5 in 2-D); an in-support pair 41 in kernel B (36 in
+    # candidate in 3-D (5 in 2-D); an in-support pair 40 in kernel B (35 in
     # 2-D), 17 in kernel A's fluid body (14 in 2-D)
     d3 = prm.dim == 3
     ft = 8.0 if d3 else 5.0
-    flops = (ft * stats["candB"] + (41.0 if d3 else 36.0) * stats["hitsB"]) if dom == "nsB" else \
+    flops = (ft * stats["candB"] + (40.0 if d3 else 35.0) * stats["hitsB"]) if dom == "nsB" else \
         (ft * stats["candA"] + (17.0 if d3 else 14.0) * stats["hitsA"])
     valu = flops / (us * 1e-6) / 1e12
     traffic = None
